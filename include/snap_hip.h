@@ -1,0 +1,262 @@
+/*
+ * snap_hip.h -- C ABI of the MI355X (gfx950) mini-KBA solve library.
+ *
+ * Drop-in boundary for the GPU task variants of Legion-SNAP's steady-state
+ * solve.  Each snap_hip_run_* entry point replaces one `extern` host function
+ * that the reference's GPU task bodies call (cited per declaration), with the
+ * same argument meaning.  Legion types become plain structs:
+ *
+ *   Realm::AffineAccessor<T,N>  ->  snap_accessorN_t { base, stride[N] }:
+ *       the element of point p is at  (char*)base + sum_i p[i]*stride[i]
+ *       (byte strides, absolute points, exactly Realm's affine form);
+ *   std::vector<Accessor>       ->  pointer + count (one accessor per group);
+ *   Rect<3>                     ->  snap_rect3_t (inclusive lo/hi);
+ *   DeferredValue<bool>         ->  int32_t* in device memory (0 or 1);
+ *   the Realm task stream       ->  snap_stream_t (a hipStream_t; NULL is
+ *                                   the null stream).
+ *
+ * Record types: MomentQuad is a record of 4 doubles and MomentTriple of 3
+ * (snap_types.h:30-50).  For cmom = nmom^2 > 4 the records widen: qtot holds
+ * cmom doubles, fluxm / q2grpm hold cmom-1, slgg / s_xs hold nmom (one per
+ * Legendre order).  Elements inside a record are contiguous doubles.
+ *
+ * Errors: every entry point returns SNAP_OK (0) or a negative code; nothing
+ * calls exit() (the reference asserts or exits, gpu_sweep.cu:84-88,
+ * gpu_outer.cu:218-220).  snap_hip_last_error() gives the message.
+ * Launches are asynchronous on the given stream.
+ */
+#ifndef SNAP_HIP_H
+#define SNAP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SNAP_OK 0
+#define SNAP_ERR_INVALID (-1)      /* bad argument */
+#define SNAP_ERR_UNSUPPORTED (-2)  /* valid in the reference, not built here */
+#define SNAP_ERR_HIP (-3)          /* a HIP runtime call failed */
+#define SNAP_ERR_NO_CONTEXT (-4)   /* initialize_gpu_context not called on this device */
+#define SNAP_ERR_NO_MEMORY (-5)    /* device allocation failed */
+
+#define SNAP_PRECISION_FP64 0      /* FP64 sweep: the plain drop-in */
+#define SNAP_PRECISION_MIXED 1     /* FP32 sweep, FP64 moments/source/convergence */
+
+typedef void *snap_stream_t;
+
+typedef struct { void *base; int64_t stride[1]; } snap_accessor1_t;
+typedef struct { void *base; int64_t stride[2]; } snap_accessor2_t;
+typedef struct { void *base; int64_t stride[3]; } snap_accessor3_t;
+typedef struct { int64_t lo[3]; int64_t hi[3]; } snap_rect3_t;
+
+/* ---------------------------------------------------------------------- */
+/* Library                                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* Message of the last error returned on the calling thread. */
+const char *snap_hip_last_error(void);
+/* Library version string. */
+const char *snap_hip_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Drop-in task entry points                                               */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces initialize_gpu_context (decl init.cc:215-220, def
+ * gpu_sweep.cu:71-273), called by InitGPUSweep::gpu_implementation
+ * (init.cc:224-236).  Uploads the quadrature to the current device and sizes
+ * the per-stream sweep workspace for one chunk.  ec is [num_octants][cmom]
+ * [num_angles] with cmom = num_moments^2 (octant stride cmom*num_angles).
+ * Calling it again on the same device replaces the context. */
+int snap_hip_initialize_gpu_context(const double *ec, const double *mu,
+                                    const double *eta, const double *xi,
+                                    const double *w, int num_angles,
+                                    int num_moments, int num_octants,
+                                    int nx_per_chunk, int ny_per_chunk,
+                                    int nz_per_chunk);
+
+/* Arguments of run_gpu_sweep (decl sweep.cc:1524-1546, def
+ * gpu_sweep.cu:2046-2295), in the same order. */
+typedef struct {
+  int64_t origin[3];              /* chunk lower bound (dom.bounds.lo) */
+  snap_accessor3_t qtot;          /* RO  cmom doubles per cell */
+  snap_accessor3_t flux;          /* RW  1 double per cell; the sweep ADDS its contribution */
+  snap_accessor3_t fluxm;         /* RW  cmom-1 doubles per cell (base NULL when cmom == 1) */
+  snap_accessor3_t dinv;          /* RO  num_angles doubles per cell */
+  snap_accessor3_t time_flux_in;  /* RO  unused: time-independent path only */
+  snap_accessor3_t time_flux_out; /* WO  unused: time-independent path only */
+  snap_accessor3_t t_xs;          /* RO  1 double per cell */
+  snap_accessor2_t ghostx;        /* RW  num_angles doubles at point (y,z); base NULL = vacuum */
+  snap_accessor2_t ghosty;        /* RW  num_angles doubles at point (x,z); base NULL = vacuum */
+  snap_accessor2_t ghostz;        /* RW  num_angles doubles at point (x,y); base NULL = vacuum */
+  snap_accessor3_t qim;           /* RO  MMS angular source: must be unused (mms_source = 0) */
+  int x_range, y_range, z_range;
+  int corner;                     /* octant: bit0 +x, bit1 +y, bit2 +z */
+  int stride_x_positive, stride_y_positive, stride_z_positive;
+  int mms_source;                 /* must be 0 */
+  int num_moments;                /* nmom; cmom = nmom^2 */
+  double hi, hj, hk;              /* 2/dx, 2/dy, 2/dz */
+  double vdelt;                   /* must be 0.0 (steady state) */
+  int num_angles;
+  int fixup;                      /* negative-flux fixup on/off */
+} snap_sweep_args_t;
+
+/* Replaces run_gpu_sweep: one (corner, group) diamond-difference sweep over
+ * one chunk, reading the upstream ghost faces and writing the downstream
+ * ones, adding the octant's scalar flux and flux moments into flux / fluxm
+ * with device atomics (the reference's relaxed-coherence
+ * SIMULTANEOUS read-write, sweep.cc:54-55, gpu_sweep.cu:1705). */
+int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream_t stream);
+
+/* Replaces run_flux0_outer_source (decl outer.cc:220-225, def
+ * gpu_outer.cu:104-223): qo0[g] = qi0[g] + sum_{g'!=g} slgg[g][mat][g'].l0 * flux0[g'].
+ * slgg[g] is a 2-D accessor at point (material, g') of nmom doubles. */
+int snap_hip_run_flux0_outer_source(snap_rect3_t subgrid_bounds,
+                                    const snap_accessor3_t *fa_qi0,
+                                    const snap_accessor3_t *fa_flux0,
+                                    const snap_accessor2_t *fa_slgg,
+                                    const snap_accessor3_t *fa_qo0,
+                                    snap_accessor3_t fa_mat, int num_groups,
+                                    snap_stream_t stream);
+
+/* Replaces run_fluxm_outer_source (decl outer.cc:226-231, def
+ * gpu_outer.cu:320-441): qom[g][m] = sum_{g'!=g} slgg[g][mat][g'].L(m) * fluxm[g'][m]
+ * for every flux moment m = 1..cmom-1 (L(m) its Legendre order from lma). */
+int snap_hip_run_fluxm_outer_source(snap_rect3_t subgrid_bounds,
+                                    const snap_accessor3_t *fa_fluxm,
+                                    const snap_accessor2_t *fa_slgg,
+                                    const snap_accessor3_t *fa_qom,
+                                    snap_accessor3_t fa_mat, int num_groups,
+                                    int num_moments, const int lma[4],
+                                    snap_stream_t stream);
+
+/* Replaces run_inner_source_single_moment (decl inner.cc:163-167, def
+ * gpu_inner.cu:44-60): qtot.l0 = q2grp0 + flux0 * s_xs.l0 (one group). */
+int snap_hip_run_inner_source_single_moment(snap_rect3_t subgrid_bounds,
+                                            snap_accessor3_t fa_sxs,
+                                            snap_accessor3_t fa_flux0,
+                                            snap_accessor3_t fa_q2grp0,
+                                            snap_accessor3_t fa_qtot,
+                                            snap_stream_t stream);
+
+/* Replaces run_inner_source_multi_moment (decl inner.cc:168-175, def
+ * gpu_inner.cu:97-117): adds qtot[m] = q2grpm[m-1] + fluxm[m-1]*s_xs.L(m). */
+int snap_hip_run_inner_source_multi_moment(snap_rect3_t subgrid_bounds,
+                                           snap_accessor3_t fa_sxs,
+                                           snap_accessor3_t fa_flux0,
+                                           snap_accessor3_t fa_q2grp0,
+                                           snap_accessor3_t fa_fluxm,
+                                           snap_accessor3_t fa_q2grpm,
+                                           snap_accessor3_t fa_qtot,
+                                           int num_moments, const int lma[4],
+                                           snap_stream_t stream);
+
+/* Replaces run_inner_convergence (decl inner.cc:327-331, def
+ * gpu_inner.cu:206-236): *result = 1 when every |flux0/flux0pi - 1| <= epsi
+ * (|flux0| when |flux0pi| < 1e-12), else 0.  result is device memory. */
+int snap_hip_run_inner_convergence(snap_rect3_t subgrid_bounds, int32_t *result,
+                                   const snap_accessor3_t *fa_flux0,
+                                   const snap_accessor3_t *fa_flux0pi,
+                                   int num_groups, double epsi,
+                                   snap_stream_t stream);
+
+/* Replaces run_outer_convergence (decl outer.cc:406-410, def
+ * gpu_outer.cu:533-563); the caller passes epsi = 100*convergence_eps as
+ * TestOuterConvergence::gpu_implementation does (outer.cc:439). */
+int snap_hip_run_outer_convergence(snap_rect3_t subgrid_bounds, int32_t *result,
+                                   const snap_accessor3_t *fa_flux0,
+                                   const snap_accessor3_t *fa_flux0po,
+                                   int num_groups, double epsi,
+                                   snap_stream_t stream);
+
+/* Replaces run_expand_cross_section (decl expxs.cc:118-122, def
+ * gpu_expxs.cu:46-266): xs[g][p] = sig[g][mat[p]]. */
+int snap_hip_run_expand_cross_section(const snap_accessor1_t *sig_ptrs,
+                                      snap_accessor3_t fa_mat,
+                                      const snap_accessor3_t *fa_xs,
+                                      int num_groups, snap_rect3_t subgrid_bounds,
+                                      snap_stream_t stream);
+
+/* Replaces run_expand_scattering_cross_section (decl expxs.cc:253-258, def
+ * gpu_expxs.cu:289-531): xs[g][p] = slgg[g][mat[p]][group_start+g], a record
+ * of record_doubles doubles (4 for MomentQuad). */
+int snap_hip_run_expand_scattering_cross_section(const snap_accessor2_t *fa_slgg,
+                                                 snap_accessor3_t fa_mat,
+                                                 const snap_accessor3_t *fa_xs,
+                                                 int num_groups,
+                                                 snap_rect3_t subgrid_bounds,
+                                                 int group_start, int record_doubles,
+                                                 snap_stream_t stream);
+
+/* Replaces run_geometry_param (decl expxs.cc:395-399, def
+ * gpu_sweep.cu:313-606): dinv[g][p][a] = 1/(xs + vdelt[g] + hi*mu[a] +
+ * hj*eta[a] + hk*xi[a]) with the angles of the current context.  vdelts is a
+ * host array of num_groups values. */
+int snap_hip_run_geometry_param(const snap_accessor3_t *fa_xs,
+                                const snap_accessor3_t *fa_dinv,
+                                const double *vdelts, int num_groups, double hi,
+                                double hj, double hk, snap_rect3_t subgrid_bounds,
+                                int num_angles, snap_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* Legion-free solver: the transport_solve loop (snap.cc:225-568)          */
+/* ---------------------------------------------------------------------- */
+
+typedef struct {
+  int nx, ny, nz;
+  double lx, ly, lz;
+  int ng, nang, nmom, nmat;
+  const int32_t *mat;       /* host [nz][ny][nx], 0-based material index */
+  const double *sigt;       /* host [nmat][ng] */
+  const double *slgg;       /* host [ng_to][nmat][ng_from][nmom] */
+  const double *qi;         /* host [ng][nz][ny][nx] */
+  const double *sigt_cell;  /* host [ng][nz][ny][nx] or NULL: per-cell total xs */
+  double epsi;
+  int iitm, oitm;
+  int fixup;
+} snap_problem_t;
+
+typedef struct {
+  int precision;            /* SNAP_PRECISION_FP64 or SNAP_PRECISION_MIXED */
+  int tile[3];              /* sweep tile edges (x,y,z); 0 = choose */
+  int waves_per_tile;       /* waves per tile workgroup; 0 = choose */
+  int first_sweep_fp64;     /* MIXED: run the first inner's sweep in FP64 */
+} snap_solve_options_t;
+
+typedef struct {
+  int inner_total;          /* inner iterations run */
+  int outer_total;          /* outer iterations run */
+  int converged;            /* outer convergence reached before oitm */
+  int sweeps_fp64;          /* inner sweeps done in FP64 */
+  int sweeps_fp32;          /* inner sweeps done in FP32 (MIXED) */
+  double sweep_ms;          /* GPU time in sweep kernels, all inners */
+  double solve_ms;          /* GPU time of the whole loop */
+  double units_per_sweep;   /* cells * 8*nang * ng */
+  double grind_ns;          /* sweep_ms*1e6 / (units_per_sweep * inner_total) */
+} snap_solve_stats_t;
+
+typedef struct snap_solver snap_solver_t;
+
+/* Upload the problem, expand cross sections and geometry (snap.cc:419-450). */
+int snap_hip_solver_create(const snap_problem_t *problem,
+                           const snap_solve_options_t *options,
+                           snap_solver_t **solver);
+/* Run the outer/inner loop from a zero flux to convergence or oitm. */
+int snap_hip_solver_run(snap_solver_t *solver, snap_solve_stats_t *stats);
+/* Copy the scalar flux [ng][cells] and flux moments [ng][cells][cmom-1]
+ * (may be NULL) to host memory. */
+int snap_hip_solver_get_flux(snap_solver_t *solver, double *flux0, double *fluxm);
+/* Time `count` inner sweeps (all octants and groups) on the source left by
+ * the last run (or the first inner's source).  Writes the mean GPU time of one
+ * inner sweep and the mean duration of one launch of the sweep kernel. */
+int snap_hip_solver_time_sweeps(snap_solver_t *solver, int count, int precision,
+                                double *ms_per_sweep, double *ms_per_launch,
+                                int *launches_per_sweep);
+void snap_hip_solver_destroy(snap_solver_t *solver);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,177 @@
+// snap_internal.h -- shared internals of libsnap_hip (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "snap_hip.h"
+
+namespace snap {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define SNAP_HIP_TRY(expr)                                                     \
+  do {                                                                         \
+    hipError_t snap_e_ = (expr);                                               \
+    if (snap_e_ != hipSuccess)                                                 \
+      return ::snap::fail(SNAP_ERR_HIP, "%s: %s (%s:%d)", #expr,               \
+                          hipGetErrorString(snap_e_), __FILE__, __LINE__);     \
+  } while (0)
+
+#define SNAP_TRY(expr)                                                         \
+  do {                                                                         \
+    int snap_rc_ = (expr);                                                     \
+    if (snap_rc_ != SNAP_OK) return snap_rc_;                                  \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// accessors: Realm's affine form with local coordinates and a group stride
+// ---------------------------------------------------------------------------
+// element (x, y, z) of group g is at base + x*s0 + y*s1 + z*s2 + g*sg bytes;
+// x, y, z are relative to the subgrid's lower bound (folded into base).
+struct Acc3 {
+  char *base;
+  int64_t s0, s1, s2, sg;
+  __host__ __device__ inline char *at(int x, int y, int z, int g) const {
+    return base + (int64_t)x * s0 + (int64_t)y * s1 + (int64_t)z * s2 + (int64_t)g * sg;
+  }
+};
+
+// element (p0, p1) of group g: base + p0*s0 + p1*s1 + g*sg
+struct Acc2 {
+  char *base;
+  int64_t s0, s1, sg;
+  __host__ __device__ inline char *at(int p0, int p1, int g) const {
+    return base + (int64_t)p0 * s0 + (int64_t)p1 * s1 + (int64_t)g * sg;
+  }
+};
+
+struct Extent3 {
+  int nx, ny, nz;
+  __host__ __device__ inline int64_t cells() const { return (int64_t)nx * ny * nz; }
+};
+
+// Dense [g][z][y][x][rec] layout of `rec` elements of `elem` bytes.
+inline Acc3 dense_acc3(void *base, Extent3 e, int rec, int elem) {
+  Acc3 a;
+  a.base = (char *)base;
+  a.s0 = (int64_t)rec * elem;
+  a.s1 = a.s0 * e.nx;
+  a.s2 = a.s1 * e.ny;
+  a.sg = a.s2 * e.nz;
+  return a;
+}
+
+// Convert `n` per-group ABI accessors over `bounds` into one Acc3 when they
+// share strides and their bases are equally spaced (one SOA instance).
+int group_acc3(const snap_accessor3_t *acc, int n, const snap_rect3_t &bounds, Acc3 *out,
+               const char *what);
+int group_acc2(const snap_accessor2_t *acc, int n, Acc2 *out, const char *what);
+int single_acc3(const snap_accessor3_t &acc, const snap_rect3_t &bounds, Acc3 *out,
+                const char *what, bool allow_null = false);
+int rect_extent(const snap_rect3_t &bounds, Extent3 *e);
+
+// Legendre order of flux moment m >= 1 in 3-D (lma = {1,3,5,7}).
+__host__ __device__ inline int moment_order(int m) {
+  int l = 1, start = 1;
+  while (m >= start + 2 * l + 1) {
+    start += 2 * l + 1;
+    l++;
+  }
+  return l;
+}
+
+inline int grid_for(int64_t n, int block, int max_blocks = 65536 * 8) {
+  int64_t b = (n + block - 1) / block;
+  if (b < 1) b = 1;
+  if (b > max_blocks) b = max_blocks;
+  return (int)b;
+}
+
+// ---------------------------------------------------------------------------
+// quadrature on the device
+// ---------------------------------------------------------------------------
+struct QuadDev {
+  int nang = 0, nmom = 0, cmom = 0;
+  double *mu = nullptr, *eta = nullptr, *xi = nullptr, *w = nullptr;
+  double *ec = nullptr;  // [8][cmom][nang]
+};
+int quad_upload(QuadDev *q, const double *ec, const double *mu, const double *eta,
+                const double *xi, const double *w, int nang, int nmom);
+void quad_free(QuadDev *q);
+// compute_derived_globals (snap.cc:1171-1342), host side, 3-D branch.
+int quad_compute(int nang, int nmom, std::vector<double> &mu, std::vector<double> &eta,
+                 std::vector<double> &xi, std::vector<double> &w, std::vector<double> &ec);
+
+// ---------------------------------------------------------------------------
+// element-wise task kernels (snap_xs.hip, snap_source.hip)
+// ---------------------------------------------------------------------------
+int launch_expand_xs(Acc2 sig, Acc3 mat, Acc3 xs, Extent3 e, int ng, hipStream_t st);
+int launch_expand_scat(Acc2 slgg, int group_start, Acc3 mat, Acc3 xs, Extent3 e, int ng,
+                       int rec, hipStream_t st);
+int launch_geometry(Acc3 xs, Acc3 dinv, const double *vdelt_host, Extent3 e, int ng, int nang,
+                    double hi, double hj, double hk, const QuadDev &q, bool fp32_out,
+                    hipStream_t st);
+int launch_outer_source(Acc3 qi, Acc3 flux0, Acc2 slgg, Acc3 qo0, Acc3 mat, Extent3 e, int ng,
+                        hipStream_t st);
+int launch_outer_source_m(Acc3 fluxm, Acc2 slgg, Acc3 qom, Acc3 mat, Extent3 e, int ng,
+                          int cmom, const int *lma, int nmom, hipStream_t st);
+int launch_inner_source(Acc3 sxs, Acc3 flux0, Acc3 q2grp0, Acc3 fluxm, Acc3 q2grpm, Acc3 qtot,
+                        Extent3 e, int ng, int cmom, const int *lma, int nmom, hipStream_t st);
+int launch_convergence(Acc3 flux0, Acc3 flux0p, Extent3 e, int ng, double epsi, int32_t *result,
+                       hipStream_t st);
+int launch_octant_reduce(const double *part, double *out, int64_t n, hipStream_t st);
+int launch_fill_int(int32_t *p, int32_t v, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// tiled wavefront sweep (snap_sweep.hip)
+// ---------------------------------------------------------------------------
+struct SweepPlan {
+  Extent3 ext{};
+  int TX = 0, TY = 0, TZ = 0;
+  int nTx = 0, nTy = 0, nTz = 0, ntiles = 0;
+  int nang = 0, cmom = 0, waves = 0;
+  int cpw = 0;                 // cells per 64-lane wavefront
+  int nplanes = 0;             // tile diagonals
+  int ncellplanes = 0;         // cell diagonals inside one tile
+  std::vector<int> plane_off;  // host: offset of each tile diagonal in plane_tiles
+  int *plane_tiles = nullptr;  // device: packed tile coordinates, by diagonal
+  int *cell_off = nullptr;     // device: offsets of cell diagonals in cells
+  int16_t *cells = nullptr;    // device: packed (i | j<<5 | k<<10), by diagonal
+  size_t lds_bytes(int real_bytes) const;
+  size_t face_doubles_per_sweep() const;  // x + y + z faces, all tiles
+};
+int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int TZ, int waves);
+void plan_free(SweepPlan *p);
+void plan_default_tile(Extent3 e, int nang, int cmom, int *TX, int *TY, int *TZ, int *waves);
+
+struct SweepLaunch {
+  const SweepPlan *plan;
+  const QuadDev *quad;
+  int nsweeps;       // grid.y
+  int ng_job;        // sweeps s: corner = corner0 + s / ng_job, group = s % ng_job
+  int corner0;
+  int fixup;
+  double hi, hj, hk;
+  Acc3 qtot, dinv, txs;        // inputs, group stride in sg (dinv/txs FP32 when fp32)
+  int sink_atomic;             // 1: atomic add into flux/fluxm; 0: write partials
+  Acc3 flux, fluxm;
+  double *part_flux, *part_fluxm;  // [8][ng_job][cells], [8][ng_job][cells][cmom-1]
+  Acc2 gx, gy, gz;             // ghost faces in physical coordinates (base NULL = vacuum)
+  void *faces;                 // [nsweeps][face_doubles_per_sweep] elements of Real
+  bool fp32;                   // FP32 sweep arithmetic and faces
+};
+// Enqueue every tile diagonal of the sweep (plan->nplanes launches).  When
+// `events` is given (2*nplanes events) each launch is bracketed by a pair.
+int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events = nullptr);
+// Quadrature of the drop-in context on device `dev` (NULL before init).
+const QuadDev *context_quad(int dev);
+
+}  // namespace snap

@@ -1,0 +1,368 @@
+// snap_solver.hip -- Legion-free host driver of the steady-state solve.
+//
+// Reproduces Snap::transport_solve (snap.cc:225-568) for one time step on one
+// GPU: expand cross sections and geometry once (snap.cc:419-450), then outer
+// iterations of {outer source, save flux0 -> flux0po, up to iitm inner
+// iterations of {inner source, save flux0 -> flux0pi, 8 corners x all groups
+// of sweeps, inner test}, outer test}.  The reference issues iterations ahead
+// under predicates and stops at the first converged test (runahead tunables,
+// mapper.cc:109-123; predicate_false_future = true, inner.cc:241,
+// outer.cc:312); breaking at the first converged test gives the same
+// iteration counts.  The outer test is skipped on the first outer iteration
+// and requires the last inner test to have passed (snap.cc:521-525,
+// outer.cc:362-365).
+//
+// All 8*ng sweeps of one inner iteration are one job: each tile-diagonal
+// launch covers every (corner, group) pair.  Sweeps write per-octant partial
+// fluxes, which are summed in octant order, so results are deterministic.
+#include <cstring>
+#include <memory>
+
+#include "snap_internal.h"
+
+namespace snap {
+
+struct DeviceBuffer {
+  void *p = nullptr;
+  size_t bytes = 0;
+  ~DeviceBuffer() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t n) {
+    bytes = n ? n : 16;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      p = nullptr;
+      return fail(SNAP_ERR_NO_MEMORY, "device allocation of %zu bytes failed", bytes);
+    }
+    return SNAP_OK;
+  }
+  template <typename T>
+  T *as() const {
+    return (T *)p;
+  }
+};
+
+}  // namespace snap
+
+using namespace snap;
+
+struct snap_solver {
+  snap_problem_t prob{};
+  snap_solve_options_t opt{};
+  Extent3 e{};
+  int ng = 0, nang = 0, nmom = 0, cmom = 0, nmat = 0;
+  int64_t cells = 0;
+  double hi = 0, hj = 0, hk = 0;
+  QuadDev quad;
+  SweepPlan plan;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int32_t *h_flag = nullptr;  // pinned
+  DeviceBuffer mat, sigt, slgg, qi, txs, txs32, sxs, dinv, dinv32;
+  DeviceBuffer flux0, flux0pi, flux0po, fluxm, q2grp0, q2grpm, qtot;
+  DeviceBuffer part_flux, part_fluxm, faces, flag;
+  bool have_fp32 = false;
+
+  ~snap_solver() {
+    plan_free(&plan);
+    quad_free(&quad);
+    for (auto &x : ev)
+      if (x) (void)hipEventDestroy(x);
+    if (h_flag) (void)hipHostFree(h_flag);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  Acc3 g_acc(const DeviceBuffer &b, int rec, int elem) const { return dense_acc3(b.p, e, rec, elem); }
+
+  int upload(DeviceBuffer &b, const void *h, size_t n) {
+    SNAP_TRY(b.alloc(n));
+    SNAP_HIP_TRY(hipMemcpy(b.p, h, n, hipMemcpyHostToDevice));
+    return SNAP_OK;
+  }
+
+  int sweep(bool fp32, hipEvent_t *events) {
+    SweepLaunch L;
+    memset(&L, 0, sizeof(L));
+    L.plan = &plan;
+    L.quad = &quad;
+    L.nsweeps = 8 * ng;
+    L.ng_job = ng;
+    L.corner0 = 0;
+    L.fixup = prob.fixup ? 1 : 0;
+    L.hi = hi;
+    L.hj = hj;
+    L.hk = hk;
+    L.qtot = g_acc(qtot, cmom, 8);
+    L.dinv = fp32 ? g_acc(dinv32, nang, 4) : g_acc(dinv, nang, 8);
+    L.txs = fp32 ? g_acc(txs32, 1, 4) : g_acc(txs, 1, 8);
+    L.sink_atomic = 0;
+    L.part_flux = part_flux.as<double>();
+    L.part_fluxm = cmom > 1 ? part_fluxm.as<double>() : nullptr;
+    L.faces = faces.p;
+    L.fp32 = fp32;
+    return sweep_enqueue(L, stream, events);
+  }
+
+  int reduce_octants() {
+    SNAP_TRY(launch_octant_reduce(part_flux.as<double>(), flux0.as<double>(), (int64_t)ng * cells,
+                                  stream));
+    if (cmom > 1)
+      SNAP_TRY(launch_octant_reduce(part_fluxm.as<double>(), fluxm.as<double>(),
+                                    (int64_t)ng * cells * (cmom - 1), stream));
+    return SNAP_OK;
+  }
+
+  int outer_source() {
+    const int lma[4] = {1, 3, 5, 7};
+    Acc2 sl;
+    sl.base = slgg.as<char>();
+    sl.s1 = (int64_t)nmom * 8;
+    sl.s0 = sl.s1 * ng;
+    sl.sg = sl.s0 * nmat;
+    const Acc3 m = g_acc(mat, 1, 4);
+    SNAP_TRY(launch_outer_source(g_acc(qi, 1, 8), g_acc(flux0, 1, 8), sl, g_acc(q2grp0, 1, 8), m,
+                                 e, ng, stream));
+    if (cmom > 1)
+      SNAP_TRY(launch_outer_source_m(g_acc(fluxm, cmom - 1, 8), sl, g_acc(q2grpm, cmom - 1, 8), m,
+                                     e, ng, cmom, lma, nmom, stream));
+    return SNAP_OK;
+  }
+
+  int inner_source() {
+    const int lma[4] = {1, 3, 5, 7};
+    Acc3 none{};
+    return launch_inner_source(g_acc(sxs, nmom, 8), g_acc(flux0, 1, 8), g_acc(q2grp0, 1, 8),
+                               cmom > 1 ? g_acc(fluxm, cmom - 1, 8) : none,
+                               cmom > 1 ? g_acc(q2grpm, cmom - 1, 8) : none, g_acc(qtot, cmom, 8),
+                               e, ng, cmom, lma, nmom, stream);
+  }
+
+  int test(const DeviceBuffer &prev, double eps, bool *converged) {
+    SNAP_TRY(launch_convergence(g_acc(flux0, 1, 8), g_acc(prev, 1, 8), e, ng, eps, flag.as<int32_t>(),
+                                stream));
+    SNAP_HIP_TRY(hipMemcpyAsync(h_flag, flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    SNAP_HIP_TRY(hipStreamSynchronize(stream));
+    *converged = (*h_flag != 0);
+    return SNAP_OK;
+  }
+
+  int copy(DeviceBuffer &dst, const DeviceBuffer &src, size_t n) {
+    SNAP_HIP_TRY(hipMemcpyAsync(dst.p, src.p, n, hipMemcpyDeviceToDevice, stream));
+    return SNAP_OK;
+  }
+};
+
+extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_options_t *o,
+                                      snap_solver_t **out) {
+  if (!p || !out) return fail(SNAP_ERR_INVALID, "NULL problem or output");
+  *out = nullptr;
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(SNAP_ERR_INVALID, "empty grid");
+  if (p->ng < 1 || p->nang < 1 || p->nmat < 1) return fail(SNAP_ERR_INVALID, "ng, nang, nmat must be >= 1");
+  if (p->nmom < 1 || p->nmom > 4) return fail(SNAP_ERR_INVALID, "nmom must be 1..4");
+  if (!(p->epsi > 0.0 && p->epsi < 1e-2)) return fail(SNAP_ERR_INVALID, "epsi must be in (0, 1e-2)");
+  if (p->iitm < 1 || p->oitm < 1) return fail(SNAP_ERR_INVALID, "iitm and oitm must be >= 1");
+  if (!(p->lx > 0.0 && p->ly > 0.0 && p->lz > 0.0)) return fail(SNAP_ERR_INVALID, "domain lengths must be positive");
+  if (!p->mat || !p->sigt || !p->slgg || !p->qi) return fail(SNAP_ERR_INVALID, "NULL problem table");
+  std::unique_ptr<snap_solver> s(new snap_solver());
+  s->prob = *p;
+  if (o) s->opt = *o;
+  if (s->opt.precision != SNAP_PRECISION_FP64 && s->opt.precision != SNAP_PRECISION_MIXED)
+    return fail(SNAP_ERR_INVALID, "unknown precision %d", s->opt.precision);
+  s->e = Extent3{p->nx, p->ny, p->nz};
+  s->ng = p->ng;
+  s->nang = p->nang;
+  s->nmom = p->nmom;
+  s->cmom = p->nmom * p->nmom;
+  s->nmat = p->nmat;
+  s->cells = s->e.cells();
+  // snap.cc:1178-1180
+  s->hi = 2.0 / (p->lx / double(p->nx));
+  s->hj = 2.0 / (p->ly / double(p->ny));
+  s->hk = 2.0 / (p->lz / double(p->nz));
+  for (int64_t c = 0; c < s->cells; c++)
+    if (p->mat[c] < 0 || p->mat[c] >= p->nmat)
+      return fail(SNAP_ERR_INVALID, "material index %d at cell %lld out of range", p->mat[c], (long long)c);
+
+  std::vector<double> mu, eta, xi, w, ec;
+  SNAP_TRY(quad_compute(p->nang, p->nmom, mu, eta, xi, w, ec));
+  SNAP_TRY(quad_upload(&s->quad, ec.data(), mu.data(), eta.data(), xi.data(), w.data(), p->nang, p->nmom));
+  int TX = s->opt.tile[0], TY = s->opt.tile[1], TZ = s->opt.tile[2], waves = s->opt.waves_per_tile;
+  {
+    int dx, dy, dz, dw;
+    plan_default_tile(s->e, s->nang, s->cmom, &dx, &dy, &dz, &dw);
+    if (TX <= 0) TX = dx;
+    if (TY <= 0) TY = dy;
+    if (TZ <= 0) TZ = dz;
+    if (waves <= 0) waves = dw;
+  }
+  SNAP_TRY(plan_create(&s->plan, s->e, s->nang, s->cmom, TX, TY, TZ, waves));
+
+  SNAP_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  for (auto &x : s->ev) SNAP_HIP_TRY(hipEventCreate(&x));
+  SNAP_HIP_TRY(hipHostMalloc((void **)&s->h_flag, sizeof(int32_t), hipHostMallocDefault));
+
+  const size_t gc = (size_t)s->ng * s->cells;
+  const size_t nm1 = s->cmom > 1 ? (size_t)(s->cmom - 1) : 0;
+  SNAP_TRY(s->upload(s->mat, p->mat, sizeof(int32_t) * s->cells));
+  SNAP_TRY(s->upload(s->sigt, p->sigt, sizeof(double) * p->nmat * p->ng));
+  SNAP_TRY(s->upload(s->slgg, p->slgg, sizeof(double) * (size_t)p->ng * p->nmat * p->ng * p->nmom));
+  SNAP_TRY(s->upload(s->qi, p->qi, sizeof(double) * gc));
+  SNAP_TRY(s->txs.alloc(sizeof(double) * gc));
+  SNAP_TRY(s->sxs.alloc(sizeof(double) * gc * s->nmom));
+  SNAP_TRY(s->dinv.alloc(sizeof(double) * gc * s->nang));
+  SNAP_TRY(s->flux0.alloc(sizeof(double) * gc));
+  SNAP_TRY(s->flux0pi.alloc(sizeof(double) * gc));
+  SNAP_TRY(s->flux0po.alloc(sizeof(double) * gc));
+  SNAP_TRY(s->q2grp0.alloc(sizeof(double) * gc));
+  SNAP_TRY(s->qtot.alloc(sizeof(double) * gc * s->cmom));
+  SNAP_TRY(s->fluxm.alloc(sizeof(double) * gc * nm1));
+  SNAP_TRY(s->q2grpm.alloc(sizeof(double) * gc * nm1));
+  SNAP_TRY(s->part_flux.alloc(sizeof(double) * 8 * gc));
+  SNAP_TRY(s->part_fluxm.alloc(sizeof(double) * 8 * gc * nm1));
+  SNAP_TRY(s->faces.alloc(sizeof(double) * s->plan.face_doubles_per_sweep() * 8 * s->ng));
+  SNAP_TRY(s->flag.alloc(sizeof(int32_t)));
+
+  // ExpandCrossSection(sigt -> t_xs), ExpandScatteringCrossSection,
+  // CalculateGeometryParam (snap.cc:419-450)
+  const hipStream_t st = s->stream;
+  if (p->sigt_cell) {
+    SNAP_HIP_TRY(hipMemcpyAsync(s->txs.p, p->sigt_cell, sizeof(double) * gc, hipMemcpyHostToDevice, st));
+  } else {
+    Acc2 sig;
+    sig.base = s->sigt.as<char>();
+    sig.s0 = (int64_t)p->ng * 8;
+    sig.s1 = 0;
+    sig.sg = 8;
+    SNAP_TRY(launch_expand_xs(sig, s->g_acc(s->mat, 1, 4), s->g_acc(s->txs, 1, 8), s->e, s->ng, st));
+  }
+  {
+    Acc2 sl;
+    sl.base = s->slgg.as<char>();
+    sl.s1 = (int64_t)s->nmom * 8;
+    sl.s0 = sl.s1 * s->ng;
+    sl.sg = sl.s0 * s->nmat;
+    SNAP_TRY(launch_expand_scat(sl, 0, s->g_acc(s->mat, 1, 4), s->g_acc(s->sxs, s->nmom, 8), s->e,
+                                s->ng, s->nmom, st));
+  }
+  SNAP_TRY(launch_geometry(s->g_acc(s->txs, 1, 8), s->g_acc(s->dinv, s->nang, 8), nullptr, s->e,
+                           s->ng, s->nang, s->hi, s->hj, s->hk, s->quad, false, st));
+  if (s->opt.precision == SNAP_PRECISION_MIXED) {
+    SNAP_TRY(s->dinv32.alloc(sizeof(float) * gc * s->nang));
+    SNAP_TRY(s->txs32.alloc(sizeof(float) * gc));
+    SNAP_TRY(launch_geometry(s->g_acc(s->txs, 1, 8), s->g_acc(s->dinv32, s->nang, 4), nullptr, s->e,
+                             s->ng, s->nang, s->hi, s->hj, s->hk, s->quad, true, st));
+    // t_xs in float for the FP32 fixup
+    std::vector<double> h(gc);
+    std::vector<float> hf(gc);
+    SNAP_HIP_TRY(hipMemcpyAsync(h.data(), s->txs.p, sizeof(double) * gc, hipMemcpyDeviceToHost, st));
+    SNAP_HIP_TRY(hipStreamSynchronize(st));
+    for (size_t i = 0; i < gc; i++) hf[i] = (float)h[i];
+    SNAP_HIP_TRY(hipMemcpy(s->txs32.p, hf.data(), sizeof(float) * gc, hipMemcpyHostToDevice));
+    s->have_fp32 = true;
+  }
+  SNAP_HIP_TRY(hipStreamSynchronize(st));
+  *out = s.release();
+  return SNAP_OK;
+}
+
+extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) {
+  if (!s) return fail(SNAP_ERR_INVALID, "NULL solver");
+  snap_solve_stats_t st;
+  memset(&st, 0, sizeof(st));
+  const size_t gc = (size_t)s->ng * s->cells;
+  const size_t nm1 = s->cmom > 1 ? (size_t)(s->cmom - 1) : 0;
+  const hipStream_t stream = s->stream;
+  const bool mixed = s->opt.precision == SNAP_PRECISION_MIXED;
+  SNAP_HIP_TRY(hipMemsetAsync(s->flux0.p, 0, sizeof(double) * gc, stream));
+  if (nm1) SNAP_HIP_TRY(hipMemsetAsync(s->fluxm.p, 0, sizeof(double) * gc * nm1, stream));
+  SNAP_HIP_TRY(hipEventRecord(s->ev[2], stream));
+  float sweep_ms = 0.f;
+  bool inner_converged = false;
+  for (int otno = 0; otno < s->prob.oitm; otno++) {
+    SNAP_TRY(s->outer_source());
+    SNAP_TRY(s->copy(s->flux0po, s->flux0, sizeof(double) * gc));
+    st.outer_total++;
+    inner_converged = false;
+    for (int inno = 0; inno < s->prob.iitm; inno++) {
+      SNAP_TRY(s->inner_source());
+      SNAP_TRY(s->copy(s->flux0pi, s->flux0, sizeof(double) * gc));
+      const bool fp32 = mixed && !(s->opt.first_sweep_fp64 && st.inner_total == 0);
+      SNAP_HIP_TRY(hipEventRecord(s->ev[0], stream));
+      SNAP_TRY(s->sweep(fp32, nullptr));
+      SNAP_HIP_TRY(hipEventRecord(s->ev[1], stream));
+      SNAP_TRY(s->reduce_octants());
+      if (fp32)
+        st.sweeps_fp32++;
+      else
+        st.sweeps_fp64++;
+      st.inner_total++;
+      SNAP_TRY(s->test(s->flux0pi, s->prob.epsi, &inner_converged));
+      float ms = 0.f;
+      SNAP_HIP_TRY(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+      sweep_ms += ms;
+      if (inner_converged) break;
+    }
+    if (otno == 0) continue;
+    if (!inner_converged) continue;
+    bool outer_converged = false;
+    SNAP_TRY(s->test(s->flux0po, 100.0 * s->prob.epsi, &outer_converged));
+    if (outer_converged) {
+      st.converged = 1;
+      break;
+    }
+  }
+  SNAP_HIP_TRY(hipEventRecord(s->ev[3], stream));
+  SNAP_HIP_TRY(hipEventSynchronize(s->ev[3]));
+  float solve_ms = 0.f;
+  SNAP_HIP_TRY(hipEventElapsedTime(&solve_ms, s->ev[2], s->ev[3]));
+  st.sweep_ms = sweep_ms;
+  st.solve_ms = solve_ms;
+  st.units_per_sweep = (double)s->cells * 8.0 * s->nang * s->ng;
+  st.grind_ns = st.inner_total ? (sweep_ms * 1e6) / (st.units_per_sweep * st.inner_total) : 0.0;
+  if (stats) *stats = st;
+  return SNAP_OK;
+}
+
+extern "C" int snap_hip_solver_get_flux(snap_solver_t *s, double *flux0, double *fluxm) {
+  if (!s) return fail(SNAP_ERR_INVALID, "NULL solver");
+  const size_t gc = (size_t)s->ng * s->cells;
+  SNAP_HIP_TRY(hipStreamSynchronize(s->stream));
+  if (flux0) SNAP_HIP_TRY(hipMemcpy(flux0, s->flux0.p, sizeof(double) * gc, hipMemcpyDeviceToHost));
+  if (fluxm && s->cmom > 1)
+    SNAP_HIP_TRY(hipMemcpy(fluxm, s->fluxm.p, sizeof(double) * gc * (s->cmom - 1), hipMemcpyDeviceToHost));
+  return SNAP_OK;
+}
+
+extern "C" int snap_hip_solver_time_sweeps(snap_solver_t *s, int count, int precision,
+                                           double *ms_per_sweep, double *ms_per_launch,
+                                           int *launches_per_sweep) {
+  if (!s || count < 1) return fail(SNAP_ERR_INVALID, "bad timing request");
+  const bool fp32 = precision == SNAP_PRECISION_MIXED;
+  if (fp32 && !s->have_fp32) return fail(SNAP_ERR_INVALID, "solver was created without mixed precision");
+  const int np = s->plan.nplanes;
+  std::vector<hipEvent_t> evs((size_t)2 * np * count);
+  for (auto &x : evs) SNAP_HIP_TRY(hipEventCreate(&x));
+  SNAP_HIP_TRY(hipEventRecord(s->ev[0], s->stream));
+  for (int i = 0; i < count; i++) SNAP_TRY(s->sweep(fp32, evs.data() + (size_t)2 * np * i));
+  SNAP_HIP_TRY(hipEventRecord(s->ev[1], s->stream));
+  SNAP_HIP_TRY(hipEventSynchronize(s->ev[1]));
+  float total = 0.f;
+  SNAP_HIP_TRY(hipEventElapsedTime(&total, s->ev[0], s->ev[1]));
+  double launch_sum = 0.0;
+  for (int i = 0; i < np * count; i++) {
+    float ms = 0.f;
+    SNAP_HIP_TRY(hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]));
+    launch_sum += ms;
+  }
+  for (auto &x : evs) (void)hipEventDestroy(x);
+  if (ms_per_sweep) *ms_per_sweep = total / count;
+  if (ms_per_launch) *ms_per_launch = launch_sum / (double)(np * count);
+  if (launches_per_sweep) *launches_per_sweep = np;
+  return SNAP_OK;
+}
+
+extern "C" void snap_hip_solver_destroy(snap_solver_t *s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  delete s;
+}

@@ -1,0 +1,203 @@
+// snap_source.hip -- outer (group-to-group) and inner sources, convergence
+// tests and the octant reduction of the sweep's partial fluxes.
+//
+// Every sum runs in the order the reference's CPU task bodies use (ascending
+// source group, skipping the in-group term), so the FP64 results equal the
+// CPU arithmetic operation for operation.
+#include "snap_internal.h"
+
+namespace snap {
+
+struct OrderTable {
+  int8_t order[16];  // Legendre order of flux moment m (m = 1..cmom-1)
+};
+
+static OrderTable make_order_table(const int *lma, int nmom) {
+  OrderTable t{};
+  int m = 1;
+  for (int l = 1; l < nmom; l++)
+    for (int j = 0; j < lma[l] && m < 16; j++) t.order[m++] = (int8_t)l;
+  return t;
+}
+
+__device__ inline void cell_xyz(int64_t c, const Extent3 &e, int &x, int &y, int &z) {
+  x = (int)(c % e.nx);
+  y = (int)((c / e.nx) % e.ny);
+  z = (int)(c / ((int64_t)e.nx * e.ny));
+}
+
+// CalcOuterSource, scalar part (outer.cc:146-171; GPU gpu_outer.cu:22-74):
+// qo0[g1] = qi[g1] + sum_{g2 != g1} slgg[g1][mat][g2].l0 * flux0[g2].
+__global__ void k_outer_source(Acc3 qi, Acc3 flux0, Acc2 slgg, Acc3 qo0, Acc3 mat, Extent3 e,
+                               int ng) {
+  const int64_t cells = e.cells();
+  const int64_t n = cells * ng;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = t % cells;
+    const int g1 = (int)(t / cells);
+    int x, y, z;
+    cell_xyz(c, e, x, y, z);
+    const int m = *(const int32_t *)mat.at(x, y, z, 0);
+    double q = *(const double *)qi.at(x, y, z, g1);
+    for (int g2 = 0; g2 < ng; g2++) {
+      if (g2 == g1) continue;
+      const double cs = *(const double *)slgg.at(m, g2, g1);
+      q += cs * *(const double *)flux0.at(x, y, z, g2);
+    }
+    *(double *)qo0.at(x, y, z, g1) = q;
+  }
+}
+
+int launch_outer_source(Acc3 qi, Acc3 flux0, Acc2 slgg, Acc3 qo0, Acc3 mat, Extent3 e, int ng,
+                        hipStream_t st) {
+  const int64_t n = e.cells() * ng;
+  hipLaunchKernelGGL(k_outer_source, dim3(grid_for(n, 256)), dim3(256), 0, st, qi, flux0, slgg,
+                     qo0, mat, e, ng);
+  SNAP_HIP_TRY(hipGetLastError());
+  return SNAP_OK;
+}
+
+// CalcOuterSource, moment part (outer.cc:175-214; GPU gpu_outer.cu:225-293),
+// over all cmom-1 moments: qom[g1][m-1] = sum_{g2 != g1} slgg[g1][mat][g2].L(m) * fluxm[g2][m-1].
+__global__ void k_outer_source_m(Acc3 fluxm, Acc2 slgg, Acc3 qom, Acc3 mat, Extent3 e, int ng,
+                                 int cmom, OrderTable ord) {
+  const int64_t cells = e.cells();
+  const int nm = cmom - 1;
+  const int64_t n = cells * ng * nm;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int mi = (int)(t % nm);
+    const int64_t r = t / nm;
+    const int64_t c = r % cells;
+    const int g1 = (int)(r / cells);
+    int x, y, z;
+    cell_xyz(c, e, x, y, z);
+    const int m = *(const int32_t *)mat.at(x, y, z, 0);
+    const int l = ord.order[mi + 1];
+    double acc = 0.0;
+    for (int g2 = 0; g2 < ng; g2++) {
+      if (g2 == g1) continue;
+      const double cs = ((const double *)slgg.at(m, g2, g1))[l];
+      acc += cs * ((const double *)fluxm.at(x, y, z, g2))[mi];
+    }
+    ((double *)qom.at(x, y, z, g1))[mi] = acc;
+  }
+}
+
+int launch_outer_source_m(Acc3 fluxm, Acc2 slgg, Acc3 qom, Acc3 mat, Extent3 e, int ng, int cmom,
+                          const int *lma, int nmom, hipStream_t st) {
+  if (cmom <= 1) return SNAP_OK;
+  if (cmom > 16) return fail(SNAP_ERR_INVALID, "cmom %d > 16", cmom);
+  const OrderTable ord = make_order_table(lma, nmom);
+  const int64_t n = e.cells() * ng * (cmom - 1);
+  hipLaunchKernelGGL(k_outer_source_m, dim3(grid_for(n, 256)), dim3(256), 0, st, fluxm, slgg,
+                     qom, mat, e, ng, cmom, ord);
+  SNAP_HIP_TRY(hipGetLastError());
+  return SNAP_OK;
+}
+
+// CalcInnerSource (inner.cc:106-160; GPU gpu_inner.cu:22-95):
+// qtot[0] = q2grp0 + flux0*s_xs[0]; qtot[m] = q2grpm[m-1] + fluxm[m-1]*s_xs[L(m)].
+__global__ void k_inner_source(Acc3 sxs, Acc3 flux0, Acc3 q2grp0, Acc3 fluxm, Acc3 q2grpm,
+                               Acc3 qtot, Extent3 e, int ng, int cmom, OrderTable ord) {
+  const int64_t cells = e.cells();
+  const int64_t n = cells * ng * cmom;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int m = (int)(t % cmom);
+    const int64_t r = t / cmom;
+    const int64_t c = r % cells;
+    const int g = (int)(r / cells);
+    int x, y, z;
+    cell_xyz(c, e, x, y, z);
+    const double *sx = (const double *)sxs.at(x, y, z, g);
+    double v;
+    if (m == 0) {
+      v = *(const double *)q2grp0.at(x, y, z, g) + *(const double *)flux0.at(x, y, z, g) * sx[0];
+    } else {
+      v = ((const double *)q2grpm.at(x, y, z, g))[m - 1] +
+          ((const double *)fluxm.at(x, y, z, g))[m - 1] * sx[ord.order[m]];
+    }
+    ((double *)qtot.at(x, y, z, g))[m] = v;
+  }
+}
+
+int launch_inner_source(Acc3 sxs, Acc3 flux0, Acc3 q2grp0, Acc3 fluxm, Acc3 q2grpm, Acc3 qtot,
+                        Extent3 e, int ng, int cmom, const int *lma, int nmom, hipStream_t st) {
+  if (cmom > 16) return fail(SNAP_ERR_INVALID, "cmom %d > 16", cmom);
+  const OrderTable ord = make_order_table(lma, nmom);
+  const int64_t n = e.cells() * ng * cmom;
+  hipLaunchKernelGGL(k_inner_source, dim3(grid_for(n, 256)), dim3(256), 0, st, sxs, flux0,
+                     q2grp0, fluxm, q2grpm, qtot, e, ng, cmom, ord);
+  SNAP_HIP_TRY(hipGetLastError());
+  return SNAP_OK;
+}
+
+// TestInnerConvergence / TestOuterConvergence (inner.cc:283-324,
+// outer.cc:354-403; GPU gpu_inner.cu:119-204, gpu_outer.cu:443-531):
+// a cell fails when |flux0/flux0p - 1| > epsi (|flux0| when |flux0p| < 1e-12).
+// *result must hold 1 on entry; any failing block clears it.
+__global__ void k_convergence(Acc3 flux0, Acc3 flux0p, Extent3 e, int ng, double epsi,
+                              int32_t *result) {
+  const int64_t cells = e.cells();
+  const int64_t n = cells * ng;
+  int bad = 0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = t % cells;
+    const int g = (int)(t / cells);
+    int x, y, z;
+    cell_xyz(c, e, x, y, z);
+    double fp = *(const double *)flux0p.at(x, y, z, g);
+    double df = 1.0;
+    if (fabs(fp) < 1.0e-12) {
+      fp = 1.0;
+      df = 0.0;
+    }
+    df = fabs((*(const double *)flux0.at(x, y, z, g) / fp) - df);
+    bad |= (df > epsi) ? 1 : 0;
+  }
+  if (__syncthreads_or(bad) && threadIdx.x == 0) atomicAnd(result, 0);
+}
+
+__global__ void k_fill_int(int32_t *p, int32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+int launch_fill_int(int32_t *p, int32_t v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, p, v);
+  SNAP_HIP_TRY(hipGetLastError());
+  return SNAP_OK;
+}
+
+int launch_convergence(Acc3 flux0, Acc3 flux0p, Extent3 e, int ng, double epsi, int32_t *result,
+                       hipStream_t st) {
+  SNAP_TRY(launch_fill_int(result, 1, st));
+  const int64_t n = e.cells() * ng;
+  hipLaunchKernelGGL(k_convergence, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, flux0,
+                     flux0p, e, ng, epsi, result);
+  SNAP_HIP_TRY(hipGetLastError());
+  return SNAP_OK;
+}
+
+// Sum of the eight octant partials in octant order 0..7.
+__global__ void k_octant_reduce(const double *__restrict__ part, double *__restrict__ out,
+                                int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+#pragma unroll
+    for (int o = 0; o < 8; o++) acc += part[(int64_t)o * n + i];
+    out[i] = acc;
+  }
+}
+
+int launch_octant_reduce(const double *part, double *out, int64_t n, hipStream_t st) {
+  if (n <= 0) return SNAP_OK;
+  hipLaunchKernelGGL(k_octant_reduce, dim3(grid_for(n, 256)), dim3(256), 0, st, part, out, n);
+  SNAP_HIP_TRY(hipGetLastError());
+  return SNAP_OK;
+}
+
+}  // namespace snap

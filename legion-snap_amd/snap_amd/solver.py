@@ -1,0 +1,107 @@
+"""Host-side front end of the HIP solve (Snap::transport_solve, snap.cc:225-568).
+
+``Solver`` owns one problem resident in HBM; ``run()`` performs the
+outer/inner loop on the GPU and returns the iteration counts and timings.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _lib
+from .problems import Problem
+
+FP64 = "fp64"
+MIXED = "mixed"
+_PREC = {FP64: _lib.PRECISION_FP64, MIXED: _lib.PRECISION_MIXED}
+
+
+def _ptr(a: Optional[np.ndarray], typ=ctypes.c_double):
+    if a is None:
+        return ctypes.cast(None, ctypes.POINTER(typ))
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(typ))
+
+
+class Solver:
+    """One problem on the current HIP device.
+
+    precision: "fp64" (the plain drop-in) or "mixed" (FP32 sweeps, FP64
+    moments, sources and convergence tests).
+    """
+
+    def __init__(self, problem: Problem, precision: str = FP64, tile=(0, 0, 0),
+                 waves_per_tile: int = 0, first_sweep_fp64: bool = False):
+        problem.validate()
+        self.problem = p = problem.contiguous()
+        self.lib = _lib.load()
+        s = _lib.Problem()
+        s.nx, s.ny, s.nz = p.nx, p.ny, p.nz
+        s.lx, s.ly, s.lz = p.lx, p.ly, p.lz
+        s.ng, s.nang, s.nmom, s.nmat = p.ng, p.nang, p.nmom, p.nmat
+        s.mat = _ptr(p.mat, ctypes.c_int32)
+        s.sigt = _ptr(p.sigt)
+        s.slgg = _ptr(p.slgg)
+        s.qi = _ptr(p.qi)
+        s.sigt_cell = _ptr(p.sigt_cell)
+        s.epsi, s.iitm, s.oitm, s.fixup = p.epsi, p.iitm, p.oitm, int(bool(p.fixup))
+        o = _lib.SolveOptions()
+        o.precision = _PREC[precision]
+        o.tile[0], o.tile[1], o.tile[2] = tile
+        o.waves_per_tile = waves_per_tile
+        o.first_sweep_fp64 = int(bool(first_sweep_fp64))
+        self.precision = precision
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.snap_hip_solver_create(ctypes.byref(s), ctypes.byref(o), ctypes.byref(h)),
+                   "snap_hip_solver_create")
+        self._h = h
+
+    def run(self) -> dict:
+        st = _lib.SolveStats()
+        _lib.check(self.lib.snap_hip_solver_run(self._h, ctypes.byref(st)), "snap_hip_solver_run")
+        return st.as_dict()
+
+    def flux(self) -> Tuple[np.ndarray, Optional[np.ndarray]]:
+        p = self.problem
+        f0 = np.zeros((p.ng, p.nz, p.ny, p.nx))
+        fm = np.zeros((p.ng, p.nz, p.ny, p.nx, p.cmom - 1)) if p.cmom > 1 else None
+        _lib.check(self.lib.snap_hip_solver_get_flux(self._h, _ptr(f0), _ptr(fm)),
+                   "snap_hip_solver_get_flux")
+        return f0, fm
+
+    def time_sweeps(self, count: int, precision: Optional[str] = None) -> dict:
+        ms = ctypes.c_double()
+        ml = ctypes.c_double()
+        nl = ctypes.c_int()
+        prec = _PREC[precision or self.precision]
+        _lib.check(self.lib.snap_hip_solver_time_sweeps(self._h, count, prec, ctypes.byref(ms),
+                                                         ctypes.byref(ml), ctypes.byref(nl)),
+                   "snap_hip_solver_time_sweeps")
+        return {"ms_per_sweep": ms.value, "ms_per_launch": ml.value, "launches_per_sweep": nl.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self.lib.snap_hip_solver_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve(problem: Problem, precision: str = FP64, **kw):
+    """Solve on the GPU; returns (flux0 [ng,nz,ny,nx], fluxm or None, stats)."""
+    with Solver(problem, precision=precision, **kw) as s:
+        stats = s.run()
+        f0, fm = s.flux()
+    return f0, fm, stats

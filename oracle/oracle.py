@@ -1,0 +1,221 @@
+"""ctypes front end of the CPU oracle (libsnap_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the oracle is the checker for the HIP library.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+Each wrapper names the reference task body its C function restates.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsnap_oracle.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+class OracleProblem(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("lz", ctypes.c_double),
+        ("ng", ctypes.c_int), ("nang", ctypes.c_int), ("nmom", ctypes.c_int),
+        ("nmat", ctypes.c_int),
+        ("mat", _ip), ("sigt", _dp), ("slgg", _dp), ("qi", _dp), ("sigt_cell", _dp),
+        ("epsi", ctypes.c_double), ("iitm", ctypes.c_int), ("oitm", ctypes.c_int),
+        ("fixup", ctypes.c_int),
+    ]
+
+
+class OracleStats(ctypes.Structure):
+    _fields_ = [
+        ("inner_total", ctypes.c_int), ("outer_total", ctypes.c_int),
+        ("converged", ctypes.c_int), ("fixup_cells", ctypes.c_int),
+        ("sweep_seconds", ctypes.c_double), ("solve_seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+
+def build() -> None:
+    """Compile the oracle with its committed Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.snap_oracle_quadrature.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        lib.snap_oracle_quadrature.restype = ctypes.c_int
+        lib.snap_oracle_expand_xs.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, _dp]
+        lib.snap_oracle_expand_scat_xs.argtypes = [ctypes.c_int] * 4 + [_ip, _dp, _dp]
+        lib.snap_oracle_geometry_param.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
+            ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
+        lib.snap_oracle_outer_source.argtypes = [ctypes.c_int] * 4 + [_ip] + [_dp] * 6
+        lib.snap_oracle_inner_source.argtypes = [ctypes.c_int] * 3 + [_dp] * 6
+        lib.snap_oracle_converged.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double]
+        lib.snap_oracle_converged.restype = ctypes.c_int
+        lib.snap_oracle_sweep.argtypes = (
+            [ctypes.c_int] * 6 + [ctypes.c_double] * 3 + [_dp] * 8 + [ctypes.c_int] + [_dp] * 8)
+        lib.snap_oracle_sweep.restype = ctypes.c_int
+        lib.snap_oracle_solve.argtypes = [ctypes.POINTER(OracleProblem), _dp, _dp,
+                                          ctypes.POINTER(OracleStats), ctypes.c_int]
+        lib.snap_oracle_solve.restype = ctypes.c_int
+        lib.snap_oracle_time_sweeps.argtypes = [ctypes.POINTER(OracleProblem), ctypes.c_int,
+                                                ctypes.c_int]
+        lib.snap_oracle_time_sweeps.restype = ctypes.c_double
+        _lib = lib
+    return _lib
+
+
+def _p(a: Optional[np.ndarray], ptr=_dp):
+    if a is None:
+        return ctypes.cast(None, ptr)
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ptr)
+
+
+def quadrature(nang: int, nmom: int):
+    """compute_derived_globals (snap.cc:1171-1342): mu, eta, xi, w, ec[8][cmom][nang]."""
+    lib = load()
+    cmom = nmom * nmom
+    mu, eta, xi, w = (np.zeros(nang) for _ in range(4))
+    ec = np.zeros((8, cmom, nang))
+    rc = lib.snap_oracle_quadrature(nang, nmom, _p(mu), _p(eta), _p(xi), _p(w), _p(ec))
+    if rc:
+        raise ValueError("bad quadrature request")
+    return mu, eta, xi, w, ec
+
+
+def spacing(p):
+    """hi, hj, hk = 2/dx (snap.cc:1178-1180)."""
+    return 2.0 / (p.lx / p.nx), 2.0 / (p.ly / p.ny), 2.0 / (p.lz / p.nz)
+
+
+def expand_xs(mat: np.ndarray, sig: np.ndarray) -> np.ndarray:
+    lib = load()
+    ng = sig.shape[1]
+    xs = np.zeros((ng,) + mat.shape)
+    lib.snap_oracle_expand_xs(mat.size, ng, _p(mat, _ip), _p(np.ascontiguousarray(sig)), _p(xs))
+    return xs
+
+
+def expand_scat_xs(mat: np.ndarray, slgg: np.ndarray) -> np.ndarray:
+    lib = load()
+    ng, nmat, _, nmom = slgg.shape
+    sxs = np.zeros((ng,) + mat.shape + (nmom,))
+    lib.snap_oracle_expand_scat_xs(mat.size, ng, nmat, nmom, _p(mat, _ip), _p(slgg), _p(sxs))
+    return sxs
+
+
+def geometry_param(t_xs: np.ndarray, nang: int, hijk, mu, eta, xi, vdelt=None) -> np.ndarray:
+    lib = load()
+    ng = t_xs.shape[0]
+    dinv = np.zeros(t_xs.shape + (nang,))
+    lib.snap_oracle_geometry_param(t_xs[0].size, ng, nang, _p(t_xs), _p(vdelt), hijk[0], hijk[1],
+                                   hijk[2], _p(mu), _p(eta), _p(xi), _p(dinv))
+    return dinv
+
+
+def outer_source(mat, qi, slgg, flux0, fluxm=None):
+    lib = load()
+    ng, nmat, _, nmom = slgg.shape
+    cmom = nmom * nmom
+    ncells = mat.size
+    q2grp0 = np.zeros_like(flux0)
+    q2grpm = np.zeros(flux0.shape + (cmom - 1,)) if cmom > 1 else None
+    if cmom > 1 and fluxm is None:
+        fluxm = np.zeros(flux0.shape + (cmom - 1,))
+    lib.snap_oracle_outer_source(ncells, ng, nmat, nmom, _p(mat, _ip), _p(qi), _p(slgg),
+                                 _p(flux0), _p(fluxm), _p(q2grp0), _p(q2grpm))
+    return q2grp0, q2grpm
+
+
+def inner_source(s_xs, flux0, q2grp0, fluxm=None, q2grpm=None):
+    lib = load()
+    ng = flux0.shape[0]
+    nmom = s_xs.shape[-1]
+    cmom = nmom * nmom
+    qtot = np.zeros(flux0.shape + (cmom,))
+    lib.snap_oracle_inner_source(flux0[0].size, ng, nmom, _p(s_xs), _p(flux0), _p(q2grp0),
+                                 _p(fluxm), _p(q2grpm), _p(qtot))
+    return qtot
+
+
+def converged(flux0, flux0p, epsi) -> bool:
+    lib = load()
+    return bool(lib.snap_oracle_converged(flux0.size, _p(flux0), _p(flux0p), epsi))
+
+
+def sweep(nx, ny, nz, nang, cmom, corner, hijk, quad, qtot_g, dinv_g, t_xs_g, fixup=True,
+          ghost_in=(None, None, None), want_ghost_out=False):
+    """One (corner, group) MiniKBA sweep (sweep.cc:242-616).  Returns
+    (part_flux [nz,ny,nx], part_fluxm [nz,ny,nx,cmom-1] or None, fixup_cells, ghost_out)."""
+    lib = load()
+    mu, eta, xi, w, ec = quad
+    pf = np.zeros((nz, ny, nx))
+    pm = np.zeros((nz, ny, nx, cmom - 1)) if cmom > 1 else None
+    outs = (np.zeros((nz, ny, nang)), np.zeros((nz, nx, nang)), np.zeros((ny, nx, nang))) \
+        if want_ghost_out else (None, None, None)
+    n = lib.snap_oracle_sweep(nx, ny, nz, nang, cmom, corner, hijk[0], hijk[1], hijk[2],
+                              _p(mu), _p(eta), _p(xi), _p(w), _p(ec), _p(qtot_g), _p(dinv_g),
+                              _p(t_xs_g), int(bool(fixup)), _p(ghost_in[0]), _p(ghost_in[1]),
+                              _p(ghost_in[2]), _p(outs[0]), _p(outs[1]), _p(outs[2]),
+                              _p(pf), _p(pm))
+    return pf, pm, n, outs
+
+
+def _problem_struct(p, keep):
+    q = p.contiguous()
+    keep.append(q)
+    s = OracleProblem()
+    s.nx, s.ny, s.nz = q.nx, q.ny, q.nz
+    s.lx, s.ly, s.lz = q.lx, q.ly, q.lz
+    s.ng, s.nang, s.nmom, s.nmat = q.ng, q.nang, q.nmom, q.nmat
+    s.mat = _p(q.mat, _ip)
+    s.sigt = _p(q.sigt)
+    s.slgg = _p(q.slgg)
+    s.qi = _p(q.qi)
+    s.sigt_cell = _p(q.sigt_cell)
+    s.epsi, s.iitm, s.oitm, s.fixup = q.epsi, q.iitm, q.oitm, int(bool(q.fixup))
+    return s
+
+
+def solve(p, nthreads: int = 0):
+    """transport_solve (snap.cc:468-540).  Returns (flux0 [ng,nz,ny,nx],
+    fluxm [ng,nz,ny,nx,cmom-1] or None, stats dict)."""
+    lib = load()
+    p.validate()
+    keep = []
+    s = _problem_struct(p, keep)
+    flux0 = np.zeros((p.ng, p.nz, p.ny, p.nx))
+    fluxm = np.zeros((p.ng, p.nz, p.ny, p.nx, p.cmom - 1)) if p.cmom > 1 else None
+    st = OracleStats()
+    rc = lib.snap_oracle_solve(ctypes.byref(s), _p(flux0), _p(fluxm), ctypes.byref(st), nthreads)
+    if rc:
+        raise RuntimeError("oracle solve failed")
+    return flux0, fluxm, st.as_dict()
+
+
+def time_sweeps(p, ninner: int = 1, nthreads: int = 0) -> float:
+    """Seconds for `ninner` inner iterations' sweeps (all octants and groups)."""
+    lib = load()
+    keep = []
+    s = _problem_struct(p, keep)
+    t = lib.snap_oracle_time_sweeps(ctypes.byref(s), ninner, nthreads)
+    if t < 0:
+        raise RuntimeError("oracle timing failed")
+    return t

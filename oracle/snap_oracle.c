@@ -1,0 +1,570 @@
+/*
+ * snap_oracle.c -- CPU oracle for the Legion-SNAP mini-KBA steady-state solve.
+ *
+ * TEST INFRASTRUCTURE ONLY (see snap_oracle.h).  Every function restates one
+ * reference task body; the citation in front of each names the file:lines it
+ * follows.  Compiled with -ffp-contract=off so that every multiply and add
+ * rounds separately, as the reference's SSE/AVX intrinsics do.
+ *
+ * Deliberate, documented departures from the reference (each is a reference
+ * defect listed with its citation):
+ *  - ec is stored with octant stride cmom*nang (snap.cc:1320-1333 writes cmom
+ *    moments into an nmom*nang stride);
+ *  - the source expansion and moment reduction run over l < cmom
+ *    (sweep.cc:361 loops forever; sweep.cc:1135 and outer.cc:205 use nmom);
+ *  - all flux moments are re-zeroed each sweep (snap.cc:494 zeroes flux0 only);
+ *  - octant contributions are added in octant order 0..7, one deterministic
+ *    instance of the reference's unordered SumReduction (snap.cc:1832-1880).
+ */
+#include "snap_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static double now_seconds(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* compute_derived_globals, snap.cc:1171-1342, num_dims == 3.
+ * mu accumulates by dm (1210-1212), eta counts down (1214-1216),
+ * xi = sqrt(1 - (mu^2 + eta^2)) (1219-1222), w = 0.125/nang (1233-1234),
+ * ec[oct][mom][a] = (is*mu)^(2l-1) * (ks*xi*js*eta)^m (1312-1336). */
+int snap_oracle_quadrature(int nang, int nmom, double *mu, double *eta,
+                           double *xi, double *w, double *ec) {
+  if (nang < 1 || nmom < 1 || nmom > 4) return -1;
+  const int cmom = nmom * nmom;
+  const double dm = 1.0 / (double)nang;
+  mu[0] = 0.5 * dm;
+  for (int i = 1; i < nang; i++) mu[i] = mu[i - 1] + dm;
+  eta[0] = 1.0 - 0.5 * dm;
+  for (int i = 1; i < nang; i++) eta[i] = eta[i - 1] - dm;
+  for (int i = 0; i < nang; i++) {
+    const double t = mu[i] * mu[i] + eta[i] * eta[i];
+    xi[i] = sqrt(1.0 - t);
+  }
+  for (int i = 0; i < nang; i++) w[i] = 0.125 / (double)nang;
+  memset(ec, 0, sizeof(double) * (size_t)8 * cmom * nang);
+  for (int kd = 0; kd < 2; kd++) {
+    const int ks = kd ? 1 : -1;
+    for (int jd = 0; jd < 2; jd++) {
+      const int js = jd ? 1 : -1;
+      for (int id = 0; id < 2; id++) {
+        const int is = id ? 1 : -1;
+        const int oct = 4 * kd + 2 * jd + id;
+        double *e = ec + (size_t)oct * cmom * nang;
+        for (int a = 0; a < nang; a++) e[a] = 1.0;
+        int moment = 1;
+        for (int l = 1; l < nmom; l++) {
+          for (int m = 0; m < 2 * (l + 1) - 1; m++) {
+            for (int a = 0; a < nang; a++)
+              e[moment * nang + a] = pow(is * mu[a], 2 * (l + 1) - 3) *
+                                     pow(ks * xi[a] * js * eta[a], m);
+            moment++;
+          }
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+/* ExpandCrossSection::cpu_implementation, expxs.cc:85-115. */
+void snap_oracle_expand_xs(int ncells, int ng, const int32_t *mat,
+                           const double *sig, double *xs) {
+  for (int g = 0; g < ng; g++)
+    for (int c = 0; c < ncells; c++) xs[(size_t)g * ncells + c] = sig[(size_t)mat[c] * ng + g];
+}
+
+/* ExpandScatteringCrossSection::cpu_implementation, expxs.cc:220-250:
+ * s_xs[g][c] = slgg[g][mat[c]][g] (the in-group element, every order l). */
+void snap_oracle_expand_scat_xs(int ncells, int ng, int nmat, int nmom,
+                                const int32_t *mat, const double *slgg,
+                                double *s_xs) {
+  for (int g = 0; g < ng; g++)
+    for (int c = 0; c < ncells; c++) {
+      const double *src = slgg + (((size_t)g * nmat + mat[c]) * ng + g) * nmom;
+      double *dst = s_xs + ((size_t)g * ncells + c) * nmom;
+      for (int l = 0; l < nmom; l++) dst[l] = src[l];
+    }
+}
+
+/* CalculateGeometryParam::cpu_implementation, expxs.cc:358-392:
+ * dinv = 1/(xs + vdelt + hi*mu + hj*eta + hk*xi), added left to right. */
+void snap_oracle_geometry_param(int ncells, int ng, int nang, const double *t_xs,
+                                const double *vdelt, double hi, double hj,
+                                double hk, const double *mu, const double *eta,
+                                const double *xi, double *dinv) {
+  for (int g = 0; g < ng; g++) {
+    const double vd = vdelt ? vdelt[g] : 0.0;
+    for (int c = 0; c < ncells; c++) {
+      const double xs = t_xs[(size_t)g * ncells + c];
+      double *d = dinv + ((size_t)g * ncells + c) * nang;
+      for (int a = 0; a < nang; a++)
+        d[a] = 1.0 / (xs + vd + hi * mu[a] + hj * eta[a] + hk * xi[a]);
+    }
+  }
+}
+
+/* Legendre order of flux moment index m >= 1 (lma = {1,3,5,7}, snap.cc:1307). */
+static int moment_order(int m) {
+  int l = 1, start = 1;
+  while (m >= start + (2 * l + 1)) {
+    start += 2 * l + 1;
+    l++;
+  }
+  return l;
+}
+
+/* CalcOuterSource::cpu_implementation, outer.cc:107-217.
+ * q2grp0[g1][c] = qi[g1][c] + sum_{g2 != g1, ascending} slgg[g1][mat][g2][0]*flux0[g2][c]
+ * q2grpm[g1][c][m-1] = sum_{g2 != g1} slgg[g1][mat][g2][L(m)]*fluxm[g2][c][m-1],
+ * for all m = 1..cmom-1 (the reference stops at nmom-1, outer.cc:205). */
+void snap_oracle_outer_source(int ncells, int ng, int nmat, int nmom,
+                              const int32_t *mat, const double *qi,
+                              const double *slgg, const double *flux0,
+                              const double *fluxm, double *q2grp0,
+                              double *q2grpm) {
+  const int cmom = nmom * nmom;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < ncells; c++) {
+    const int mt = mat[c];
+    for (int g1 = 0; g1 < ng; g1++) {
+      double qo0 = qi[(size_t)g1 * ncells + c];
+      for (int g2 = 0; g2 < ng; g2++) {
+        if (g1 == g2) continue;
+        const double cs = slgg[(((size_t)g1 * nmat + mt) * ng + g2) * nmom];
+        qo0 += cs * flux0[(size_t)g2 * ncells + c];
+      }
+      q2grp0[(size_t)g1 * ncells + c] = qo0;
+    }
+    if (cmom > 1) {
+      for (int g1 = 0; g1 < ng; g1++) {
+        double *qom = q2grpm + ((size_t)g1 * ncells + c) * (cmom - 1);
+        for (int m = 1; m < cmom; m++) {
+          const int l = moment_order(m);
+          double acc = 0.0;
+          for (int g2 = 0; g2 < ng; g2++) {
+            if (g1 == g2) continue;
+            const double cs = slgg[(((size_t)g1 * nmat + mt) * ng + g2) * nmom + l];
+            acc += cs * fluxm[((size_t)g2 * ncells + c) * (cmom - 1) + (m - 1)];
+          }
+          qom[m - 1] = acc;
+        }
+      }
+    }
+  }
+}
+
+/* CalcInnerSource::cpu_implementation, inner.cc:106-160:
+ * qtot[0] = q2grp0 + flux0*s_xs[0]; qtot[m] = q2grpm[m-1] + fluxm[m-1]*s_xs[L(m)]. */
+void snap_oracle_inner_source(int ncells, int ng, int nmom, const double *s_xs,
+                              const double *flux0, const double *q2grp0,
+                              const double *fluxm, const double *q2grpm,
+                              double *qtot) {
+  const int cmom = nmom * nmom;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < ncells; c++) {
+    for (int g = 0; g < ng; g++) {
+      const size_t gc = (size_t)g * ncells + c;
+      const double *sx = s_xs + gc * nmom;
+      double *q = qtot + gc * cmom;
+      q[0] = q2grp0[gc] + flux0[gc] * sx[0];
+      for (int m = 1; m < cmom; m++)
+        q[m] = q2grpm[gc * (cmom - 1) + m - 1] +
+               fluxm[gc * (cmom - 1) + m - 1] * sx[moment_order(m)];
+    }
+  }
+}
+
+/* TestInnerConvergence inner.cc:283-324 and TestOuterConvergence
+ * outer.cc:354-403: df = |flux0/flux0p - 1|, or |flux0| when |flux0p| < 1e-12;
+ * not converged as soon as df > epsi (NaN compares false: converged). */
+int snap_oracle_converged(int n, const double *flux0, const double *flux0p,
+                          double epsi) {
+  const double tolr = 1.0e-12;
+  for (int i = 0; i < n; i++) {
+    double fp = flux0p[i];
+    double df = 1.0;
+    if (fabs(fp) < tolr) {
+      fp = 1.0;
+      df = 0.0;
+    }
+    df = fabs((flux0[i] / fp) - df);
+    if (df > epsi) return 0;
+  }
+  return 1;
+}
+
+/* MiniKBATask::cpu_implementation, sweep.cc:242-616 (generic variant),
+ * time-independent (vdelt == 0), non-MMS. */
+int snap_oracle_sweep(int nx, int ny, int nz, int nang, int cmom, int corner,
+                      double hi, double hj, double hk, const double *mu,
+                      const double *eta, const double *xi, const double *w,
+                      const double *ec, const double *qtot_g,
+                      const double *dinv_g, const double *t_xs_g, int fixup,
+                      const double *ghost_x_in, const double *ghost_y_in,
+                      const double *ghost_z_in, double *ghost_x_out,
+                      double *ghost_y_out, double *ghost_z_out,
+                      double *part_flux, double *part_fluxm) {
+  const int sxp = (corner & 1) != 0, syp = (corner & 2) != 0, szp = (corner & 4) != 0;
+  const size_t ab = (size_t)nang;
+  double *buf = (double *)malloc(sizeof(double) * ab * 13);
+  double *psi = buf, *pc = buf + ab, *psii = buf + 2 * ab, *psij = buf + 3 * ab,
+         *psik = buf + 4 * ab, *hv_x = buf + 5 * ab, *hv_y = buf + 6 * ab,
+         *hv_z = buf + 7 * ab, *fx_hv_x = buf + 8 * ab, *fx_hv_y = buf + 9 * ab,
+         *fx_hv_z = buf + 10 * ab, *tmp = buf + 11 * ab;
+  double *yflux_pencil = (double *)malloc(sizeof(double) * ab * nx);
+  double *zflux_plane = (double *)malloc(sizeof(double) * ab * nx * ny);
+  const double *ecb = ec + (size_t)corner * cmom * nang;
+  const double tolr = 1.0e-12;
+  int fixed_cells = 0;
+  (void)tmp;
+
+  for (int z = 0; z < nz; z++) {
+    for (int y = 0; y < ny; y++) {
+      for (int x = 0; x < nx; x++) {
+        const int px = sxp ? x : nx - 1 - x;
+        const int py = syp ? y : ny - 1 - y;
+        const int pz = szp ? z : nz - 1 - z;
+        const size_t c = ((size_t)pz * ny + py) * nx + px;
+        /* angular source (sweep.cc:331-345) */
+        const double *quad = qtot_g + c * cmom;
+        for (int a = 0; a < nang; a++) psi[a] = quad[0];
+        for (int l = 1; l < cmom; l++)
+          for (int a = 0; a < nang; a++) psi[a] += ecb[l * nang + a] * quad[l];
+        for (int a = 0; a < nang; a++) pc[a] = psi[a];
+        /* x inflow (sweep.cc:356-364) */
+        if (x == 0) {
+          if (ghost_x_in)
+            memcpy(psii, ghost_x_in + ((size_t)pz * ny + py) * ab, sizeof(double) * ab);
+          else
+            memset(psii, 0, sizeof(double) * ab);
+        }
+        for (int a = 0; a < nang; a++) pc[a] += psii[a] * mu[a] * hi;
+        /* y inflow (sweep.cc:365-376) */
+        if (y == 0) {
+          if (ghost_y_in)
+            memcpy(psij, ghost_y_in + ((size_t)pz * nx + px) * ab, sizeof(double) * ab);
+          else
+            memset(psij, 0, sizeof(double) * ab);
+        } else {
+          memcpy(psij, yflux_pencil + (size_t)x * ab, sizeof(double) * ab);
+        }
+        for (int a = 0; a < nang; a++) pc[a] += psij[a] * eta[a] * hj;
+        /* z inflow (sweep.cc:377-388) */
+        if (z == 0) {
+          if (ghost_z_in)
+            memcpy(psik, ghost_z_in + ((size_t)py * nx + px) * ab, sizeof(double) * ab);
+          else
+            memset(psik, 0, sizeof(double) * ab);
+        } else {
+          memcpy(psik, zflux_plane + ((size_t)y * nx + x) * ab, sizeof(double) * ab);
+        }
+        for (int a = 0; a < nang; a++) pc[a] += psik[a] * xi[a] * hk;
+        /* multiply by dinv (sweep.cc:397-400) */
+        const double *dinv = dinv_g + c * ab;
+        for (int a = 0; a < nang; a++) pc[a] *= dinv[a];
+
+        if (fixup) {
+          /* negative flux fixup, sweep.cc:402-505 (vdelt == 0 branch) */
+          unsigned old_negative_fluxes = 0;
+          int ran = 0;
+          for (int a = 0; a < nang; a++) hv_x[a] = 1.0;
+          for (int a = 0; a < nang; a++) hv_y[a] = 1.0;
+          for (int a = 0; a < nang; a++) hv_z[a] = 1.0;
+          const double t_xs = t_xs_g[c];
+          while (1) {
+            unsigned negative_fluxes = 0;
+            for (int a = 0; a < nang; a++) {
+              fx_hv_x[a] = 2.0 * pc[a] - psii[a];
+              if (fx_hv_x[a] < 0.0) {
+                hv_x[a] = 0.0;
+                negative_fluxes++;
+              }
+            }
+            for (int a = 0; a < nang; a++) {
+              fx_hv_y[a] = 2.0 * pc[a] - psij[a];
+              if (fx_hv_y[a] < 0.0) {
+                hv_y[a] = 0.0;
+                negative_fluxes++;
+              }
+            }
+            for (int a = 0; a < nang; a++) {
+              fx_hv_z[a] = 2.0 * pc[a] - psik[a];
+              if (fx_hv_z[a] < 0.0) {
+                hv_z[a] = 0.0;
+                negative_fluxes++;
+              }
+            }
+            if (negative_fluxes == old_negative_fluxes) break;
+            old_negative_fluxes = negative_fluxes;
+            ran = 1;
+            for (int a = 0; a < nang; a++) {
+              pc[a] = psi[a] + 0.5 * (psii[a] * mu[a] * hi * (1.0 + hv_x[a]) +
+                                      psij[a] * eta[a] * hj * (1.0 + hv_y[a]) +
+                                      psik[a] * xi[a] * hk * (1.0 + hv_z[a]));
+              double den = (pc[a] <= 0.0) ? 0.0
+                                          : (t_xs + mu[a] * hi * hv_x[a] +
+                                             eta[a] * hj * hv_y[a] +
+                                             xi[a] * hk * hv_z[a]);
+              if (den < tolr)
+                pc[a] = 0.0;
+              else
+                pc[a] /= den;
+            }
+          }
+          fixed_cells += ran;
+          for (int a = 0; a < nang; a++) psii[a] = fx_hv_x[a] * hv_x[a];
+          for (int a = 0; a < nang; a++) psij[a] = fx_hv_y[a] * hv_y[a];
+          for (int a = 0; a < nang; a++) psik[a] = fx_hv_z[a] * hv_z[a];
+        } else {
+          /* sweep.cc:521-534 */
+          for (int a = 0; a < nang; a++) psii[a] = 2.0 * pc[a] - psii[a];
+          for (int a = 0; a < nang; a++) psij[a] = 2.0 * pc[a] - psij[a];
+          for (int a = 0; a < nang; a++) psik[a] = 2.0 * pc[a] - psik[a];
+        }
+        /* outflow faces (sweep.cc:536-566) */
+        if (x == nx - 1 && ghost_x_out)
+          memcpy(ghost_x_out + ((size_t)pz * ny + py) * ab, psii, sizeof(double) * ab);
+        if (y == ny - 1) {
+          if (ghost_y_out)
+            memcpy(ghost_y_out + ((size_t)pz * nx + px) * ab, psij, sizeof(double) * ab);
+        } else {
+          memcpy(yflux_pencil + (size_t)x * ab, psij, sizeof(double) * ab);
+        }
+        if (z == nz - 1) {
+          if (ghost_z_out)
+            memcpy(ghost_z_out + ((size_t)py * nx + px) * ab, psik, sizeof(double) * ab);
+        } else {
+          memcpy(zflux_plane + ((size_t)y * nx + x) * ab, psik, sizeof(double) * ab);
+        }
+        /* moment reductions (sweep.cc:568-591) */
+        double total = 0.0;
+        for (int a = 0; a < nang; a++) {
+          psi[a] = w[a] * pc[a];
+          total += psi[a];
+        }
+        part_flux[c] = total;
+        if (cmom > 1) {
+          for (int l = 1; l < cmom; l++) {
+            const double *e = ecb + (size_t)l * nang;
+            total = 0.0;
+            for (int a = 0; a < nang; a++) total += e[a] * psi[a];
+            part_fluxm[c * (cmom - 1) + (l - 1)] = total;
+          }
+        }
+      }
+    }
+  }
+  free(zflux_plane);
+  free(yflux_pencil);
+  free(buf);
+  return fixed_cells;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Solve loop (snap.cc:225-568, steady state, one time step)                 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int nx, ny, nz, ncells, ng, nang, nmom, cmom;
+  double hi, hj, hk;
+  double *mu, *eta, *xi, *w, *ec;
+  double *t_xs, *s_xs, *dinv;
+  double *part_flux, *part_fluxm; /* [8][ng][cells] and [8][ng][cells][cmom-1] */
+} oracle_state_t;
+
+static int state_init(oracle_state_t *s, const oracle_problem_t *p) {
+  memset(s, 0, sizeof(*s));
+  s->nx = p->nx;
+  s->ny = p->ny;
+  s->nz = p->nz;
+  s->ncells = p->nx * p->ny * p->nz;
+  s->ng = p->ng;
+  s->nang = p->nang;
+  s->nmom = p->nmom;
+  s->cmom = p->nmom * p->nmom;
+  /* snap.cc:1178-1180 */
+  s->hi = 2.0 / (p->lx / (double)p->nx);
+  s->hj = 2.0 / (p->ly / (double)p->ny);
+  s->hk = 2.0 / (p->lz / (double)p->nz);
+  const size_t na = (size_t)p->nang;
+  s->mu = (double *)malloc(sizeof(double) * na);
+  s->eta = (double *)malloc(sizeof(double) * na);
+  s->xi = (double *)malloc(sizeof(double) * na);
+  s->w = (double *)malloc(sizeof(double) * na);
+  s->ec = (double *)malloc(sizeof(double) * 8 * s->cmom * na);
+  if (snap_oracle_quadrature(p->nang, p->nmom, s->mu, s->eta, s->xi, s->w, s->ec)) return -1;
+  const size_t gc = (size_t)s->ng * s->ncells;
+  s->t_xs = (double *)malloc(sizeof(double) * gc);
+  s->s_xs = (double *)malloc(sizeof(double) * gc * s->nmom);
+  s->dinv = (double *)malloc(sizeof(double) * gc * na);
+  s->part_flux = (double *)malloc(sizeof(double) * 8 * gc);
+  s->part_fluxm = s->cmom > 1 ? (double *)malloc(sizeof(double) * 8 * gc * (s->cmom - 1)) : NULL;
+  if (!s->t_xs || !s->s_xs || !s->dinv || !s->part_flux) return -1;
+  /* ExpandCrossSection / ExpandScatteringCrossSection / CalculateGeometryParam
+   * (snap.cc:419-450); a per-cell total cross section replaces the gather. */
+  if (p->sigt_cell)
+    memcpy(s->t_xs, p->sigt_cell, sizeof(double) * gc);
+  else
+    snap_oracle_expand_xs(s->ncells, s->ng, p->mat, p->sigt, s->t_xs);
+  snap_oracle_expand_scat_xs(s->ncells, s->ng, p->nmat, s->nmom, p->mat, p->slgg, s->s_xs);
+  snap_oracle_geometry_param(s->ncells, s->ng, p->nang, s->t_xs, NULL, s->hi, s->hj, s->hk,
+                             s->mu, s->eta, s->xi, s->dinv);
+  return 0;
+}
+
+static void state_free(oracle_state_t *s) {
+  free(s->mu);
+  free(s->eta);
+  free(s->xi);
+  free(s->w);
+  free(s->ec);
+  free(s->t_xs);
+  free(s->s_xs);
+  free(s->dinv);
+  free(s->part_flux);
+  free(s->part_fluxm);
+}
+
+/* perform_sweeps, snap.cc:849-887: 8 corners x all groups, independent tasks.
+ * Each task writes its own partial; partials are summed in octant order. */
+static int perform_sweeps(oracle_state_t *s, int fixup, const double *qtot,
+                          double *flux0, double *fluxm) {
+  const int ng = s->ng, nc = s->ncells, cm = s->cmom;
+  const size_t gc = (size_t)ng * nc;
+  int fixed = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : fixed)
+  for (int task = 0; task < 8 * ng; task++) {
+    const int corner = task / ng, g = task % ng;
+    fixed += snap_oracle_sweep(
+        s->nx, s->ny, s->nz, s->nang, cm, corner, s->hi, s->hj, s->hk, s->mu, s->eta, s->xi,
+        s->w, s->ec, qtot + (size_t)g * nc * cm, s->dinv + (size_t)g * nc * s->nang,
+        s->t_xs + (size_t)g * nc, fixup, NULL, NULL, NULL, NULL, NULL, NULL,
+        s->part_flux + (size_t)corner * gc + (size_t)g * nc,
+        cm > 1 ? s->part_fluxm + ((size_t)corner * gc + (size_t)g * nc) * (cm - 1) : NULL);
+  }
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < gc; i++) {
+    double acc = 0.0;
+    for (int o = 0; o < 8; o++) acc += s->part_flux[(size_t)o * gc + i];
+    flux0[i] = acc;
+  }
+  if (cm > 1) {
+    const size_t gm = gc * (cm - 1);
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < gm; i++) {
+      double acc = 0.0;
+      for (int o = 0; o < 8; o++) acc += s->part_fluxm[(size_t)o * gm + i];
+      fluxm[i] = acc;
+    }
+  }
+  return fixed;
+}
+
+int snap_oracle_solve(const oracle_problem_t *p, double *flux0, double *fluxm,
+                      oracle_stats_t *stats, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  oracle_state_t s;
+  oracle_stats_t st;
+  memset(&st, 0, sizeof(st));
+  if (state_init(&s, p)) {
+    state_free(&s);
+    return -1;
+  }
+  const int ng = s.ng, nc = s.ncells, cm = s.cmom;
+  const size_t gc = (size_t)ng * nc;
+  double *flux0po = (double *)calloc(gc, sizeof(double));
+  double *flux0pi = (double *)calloc(gc, sizeof(double));
+  double *q2grp0 = (double *)calloc(gc, sizeof(double));
+  double *qtot = (double *)calloc(gc * cm, sizeof(double));
+  double *q2grpm = cm > 1 ? (double *)calloc(gc * (cm - 1), sizeof(double)) : NULL;
+  double *fluxm_local = NULL;
+  if (cm > 1 && !fluxm) fluxm = fluxm_local = (double *)calloc(gc * (cm - 1), sizeof(double));
+  memset(flux0, 0, sizeof(double) * gc);
+  if (cm > 1) memset(fluxm, 0, sizeof(double) * gc * (cm - 1));
+
+  const double t0 = now_seconds();
+  int inner_converged = 0;
+  for (int otno = 0; otno < p->oitm; otno++) {
+    /* CalcOuterSource, then save_fluxes flux0 -> flux0po (snap.cc:477-481) */
+    snap_oracle_outer_source(nc, ng, p->nmat, s.nmom, p->mat, p->qi, p->slgg, flux0, fluxm,
+                             q2grp0, q2grpm);
+    memcpy(flux0po, flux0, sizeof(double) * gc);
+    st.outer_total++;
+    inner_converged = 0;
+    for (int inno = 0; inno < p->iitm; inno++) {
+      /* snap.cc:490-503 */
+      snap_oracle_inner_source(nc, ng, s.nmom, s.s_xs, flux0, q2grp0, fluxm, q2grpm, qtot);
+      memcpy(flux0pi, flux0, sizeof(double) * gc);
+      const double ts = now_seconds();
+      st.fixup_cells += perform_sweeps(&s, p->fixup, qtot, flux0, fluxm);
+      st.sweep_seconds += now_seconds() - ts;
+      st.inner_total++;
+      inner_converged = snap_oracle_converged((int)gc, flux0, flux0pi, p->epsi);
+      if (inner_converged) break;
+    }
+    /* snap.cc:519-538: the outer test is skipped on the first outer and
+     * fails whenever the last inner did not converge (outer.cc:362-365). */
+    if (otno == 0) continue;
+    const int outer_converged =
+        inner_converged && snap_oracle_converged((int)gc, flux0, flux0po, 100.0 * p->epsi);
+    if (outer_converged) {
+      st.converged = 1;
+      break;
+    }
+  }
+  st.solve_seconds = now_seconds() - t0;
+  if (stats) *stats = st;
+  free(flux0po);
+  free(flux0pi);
+  free(q2grp0);
+  free(qtot);
+  free(q2grpm);
+  free(fluxm_local);
+  state_free(&s);
+  return 0;
+}
+
+double snap_oracle_time_sweeps(const oracle_problem_t *p, int ninner, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  oracle_state_t s;
+  if (state_init(&s, p)) {
+    state_free(&s);
+    return -1.0;
+  }
+  const int ng = s.ng, nc = s.ncells, cm = s.cmom;
+  const size_t gc = (size_t)ng * nc;
+  double *flux0 = (double *)calloc(gc, sizeof(double));
+  double *fluxm = cm > 1 ? (double *)calloc(gc * (cm - 1), sizeof(double)) : NULL;
+  double *qtot = (double *)calloc(gc * cm, sizeof(double));
+  /* the first inner's source: q2grp0 = qi, qtot[0] = qi (flux is zero) */
+  for (int g = 0; g < ng; g++)
+    for (int c = 0; c < nc; c++) qtot[((size_t)g * nc + c) * cm] = p->qi[(size_t)g * nc + c];
+  double total = 0.0;
+  for (int it = 0; it < ninner; it++) {
+    const double ts = now_seconds();
+    perform_sweeps(&s, p->fixup, qtot, flux0, fluxm);
+    total += now_seconds() - ts;
+  }
+  free(flux0);
+  free(fluxm);
+  free(qtot);
+  state_free(&s);
+  return total;
+}

@@ -1,0 +1,116 @@
+/*
+ * snap_oracle.h -- CPU oracle for the Legion-SNAP mini-KBA steady-state solve.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a Legion-free FP64 restatement of the
+ * reference's CPU task bodies and of its transport_solve control flow.  It is
+ * the checker for the HIP library: only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it.  The product path never calls it.
+ *
+ * Data layout (dense, x fastest, as the reference's SOA layout
+ * snap.cc:1456-1475 with one field per energy group):
+ *   cell index c = (z*ny + y)*nx + x
+ *   per-group scalar fields      f[g][c]
+ *   per-group angle records      f[g][c][nang]           (dinv)
+ *   per-group moment records     f[g][c][cmom]           (qtot)
+ *   per-group flux moments       f[g][c][cmom-1]         (fluxm, q2grpm)
+ *   in-group scattering          s_xs[g][c][nmom]        (by Legendre order)
+ *   material tables              sigt[mat][g]
+ *   group-to-group scattering    slgg[g_to][mat][g_from][nmom]  (snap.cc:596-708)
+ *   expansion coefficients       ec[oct][cmom][nang]     (octant stride cmom*nang)
+ */
+#ifndef SNAP_ORACLE_H
+#define SNAP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int nx, ny, nz;
+  double lx, ly, lz;
+  int ng, nang, nmom, nmat;
+  const int32_t *mat;       /* [cells] 0-based material index */
+  const double *sigt;       /* [nmat][ng] */
+  const double *slgg;       /* [ng][nmat][ng][nmom] */
+  const double *qi;         /* [ng][cells] fixed isotropic source */
+  const double *sigt_cell;  /* optional [ng][cells]: replaces sigt[mat[c]][g] */
+  double epsi;
+  int iitm, oitm;
+  int fixup;
+} oracle_problem_t;
+
+typedef struct {
+  int inner_total;          /* inner iterations executed (sweeps of all octants and groups) */
+  int outer_total;          /* outer iterations executed */
+  int converged;            /* 1 if the outer test passed before oitm */
+  int fixup_cells;          /* (cell,octant,group) sweeps whose fixup loop changed pc, summed over the solve */
+  double sweep_seconds;     /* wall time spent in sweeps */
+  double solve_seconds;     /* wall time of the whole outer/inner loop */
+} oracle_stats_t;
+
+/* compute_derived_globals, snap.cc:1171-1342 (3-D branch). */
+int snap_oracle_quadrature(int nang, int nmom, double *mu, double *eta,
+                           double *xi, double *w, double *ec);
+
+/* ExpandCrossSection, expxs.cc:85-115 */
+void snap_oracle_expand_xs(int ncells, int ng, const int32_t *mat,
+                           const double *sig, double *xs);
+/* ExpandScatteringCrossSection, expxs.cc:220-250 */
+void snap_oracle_expand_scat_xs(int ncells, int ng, int nmat, int nmom,
+                                const int32_t *mat, const double *slgg,
+                                double *s_xs);
+/* CalculateGeometryParam, expxs.cc:358-392 */
+void snap_oracle_geometry_param(int ncells, int ng, int nang, const double *t_xs,
+                                const double *vdelt, double hi, double hj,
+                                double hk, const double *mu, const double *eta,
+                                const double *xi, double *dinv);
+/* CalcOuterSource, outer.cc:107-217 (all cmom-1 moments). */
+void snap_oracle_outer_source(int ncells, int ng, int nmat, int nmom,
+                              const int32_t *mat, const double *qi,
+                              const double *slgg, const double *flux0,
+                              const double *fluxm, double *q2grp0,
+                              double *q2grpm);
+/* CalcInnerSource, inner.cc:106-160 */
+void snap_oracle_inner_source(int ncells, int ng, int nmom, const double *s_xs,
+                              const double *flux0, const double *q2grp0,
+                              const double *fluxm, const double *q2grpm,
+                              double *qtot);
+/* TestInnerConvergence inner.cc:283-324 (scale 1) / TestOuterConvergence
+ * outer.cc:354-403 (scale 100).  Returns 1 when converged. */
+int snap_oracle_converged(int n, const double *flux0, const double *flux0p,
+                          double epsi);
+
+/* One (octant, group) mini-KBA sweep, sweep.cc:242-616, vacuum boundaries
+ * unless face inflows are given.  Writes the octant's contribution to the
+ * scalar flux (part_flux[c]) and to the flux moments (part_fluxm[c][cmom-1]).
+ * ghost_{x,y,z}_in: optional [ny*nz|nx*nz|nx*ny][nang] inflow faces indexed in
+ * the reference's ghost-point order (ghostx (y,z), ghosty (x,z), ghostz (x,y));
+ * ghost_*_out: optional outflow faces in the same order.
+ * Returns the number of cells whose fixup loop ran at least once. */
+int snap_oracle_sweep(int nx, int ny, int nz, int nang, int cmom, int corner,
+                      double hi, double hj, double hk, const double *mu,
+                      const double *eta, const double *xi, const double *w,
+                      const double *ec, const double *qtot_g,
+                      const double *dinv_g, const double *t_xs_g, int fixup,
+                      const double *ghost_x_in, const double *ghost_y_in,
+                      const double *ghost_z_in, double *ghost_x_out,
+                      double *ghost_y_out, double *ghost_z_out,
+                      double *part_flux, double *part_fluxm);
+
+/* transport_solve steady-state loop, snap.cc:468-540.  flux0 [ng][cells],
+ * fluxm [ng][cells][cmom-1] (may be NULL when nmom == 1).  nthreads <= 0
+ * means all OpenMP threads. */
+int snap_oracle_solve(const oracle_problem_t *p, double *flux0, double *fluxm,
+                      oracle_stats_t *stats, int nthreads);
+
+/* Time `ninner` inner iterations' sweeps (all octants and groups) of problem
+ * p starting from a zero flux: the CPU baseline sample. Returns seconds. */
+double snap_oracle_time_sweeps(const oracle_problem_t *p, int ninner,
+                               int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
