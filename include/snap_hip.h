@@ -41,8 +41,10 @@ extern "C" {
 #define SNAP_ERR_NO_CONTEXT (-4)   /* initialize_gpu_context not called on this device */
 #define SNAP_ERR_NO_MEMORY (-5)    /* device allocation failed */
 
-#define SNAP_PRECISION_FP64 0      /* FP64 sweep: the plain drop-in */
-#define SNAP_PRECISION_MIXED 1     /* FP32 sweep, FP64 moments/source/convergence */
+#define SNAP_PRECISION_FP64 0        /* FP64 sweep with fused multiply-adds */
+#define SNAP_PRECISION_MIXED 1       /* FP32 sweep, FP64 moments/source/convergence */
+#define SNAP_PRECISION_FP64_EXACT 2  /* FP64 sweep in the reference CPU operation order,
+                                        every multiply and add rounded separately */
 
 typedef void *snap_stream_t;
 
@@ -218,11 +220,16 @@ typedef struct {
   int fixup;
 } snap_problem_t;
 
+#define SNAP_DINV_DEFAULT 0     /* FP64: stream dinv; MIXED: recompute */
+#define SNAP_DINV_STREAM 1      /* sweep reads the dinv array (reference data flow) */
+#define SNAP_DINV_RECOMPUTE 2   /* sweep forms dinv from t_xs per cell and angle */
+
 typedef struct {
-  int precision;            /* SNAP_PRECISION_FP64 or SNAP_PRECISION_MIXED */
+  int precision;            /* SNAP_PRECISION_FP64, _FP64_EXACT or _MIXED */
   int tile[3];              /* sweep tile edges (x,y,z); 0 = choose */
   int waves_per_tile;       /* waves per tile workgroup; 0 = choose */
   int first_sweep_fp64;     /* MIXED: run the first inner's sweep in FP64 */
+  int dinv_mode;            /* SNAP_DINV_DEFAULT, _STREAM or _RECOMPUTE */
 } snap_solve_options_t;
 
 typedef struct {
@@ -248,9 +255,10 @@ int snap_hip_solver_run(snap_solver_t *solver, snap_solve_stats_t *stats);
 /* Copy the scalar flux [ng][cells] and flux moments [ng][cells][cmom-1]
  * (may be NULL) to host memory. */
 int snap_hip_solver_get_flux(snap_solver_t *solver, double *flux0, double *fluxm);
-/* Time `count` inner sweeps (all octants and groups) on the source left by
- * the last run (or the first inner's source).  Writes the mean GPU time of one
- * inner sweep and the mean duration of one launch of the sweep kernel. */
+/* Time `count` inner sweeps (all octants and groups) in the given precision
+ * on the source left by the last run.  Writes the mean GPU time of one inner
+ * sweep and the mean duration of one launch of the sweep kernel.  MIXED needs
+ * a solver created with precision MIXED. */
 int snap_hip_solver_time_sweeps(snap_solver_t *solver, int count, int precision,
                                 double *ms_per_sweep, double *ms_per_launch,
                                 int *launches_per_sweep);

@@ -139,13 +139,15 @@ struct SweepPlan {
   int nTx = 0, nTy = 0, nTz = 0, ntiles = 0;
   int nang = 0, cmom = 0, waves = 0;
   int cpw = 0;                 // cells per 64-lane wavefront
+  int sb = 0;                  // cells per per-wave moment batch
+  int maxdiag = 0;             // most cells on one cell diagonal of a tile
   int nplanes = 0;             // tile diagonals
   int ncellplanes = 0;         // cell diagonals inside one tile
   std::vector<int> plane_off;  // host: offset of each tile diagonal in plane_tiles
   int *plane_tiles = nullptr;  // device: packed tile coordinates, by diagonal
   int *cell_off = nullptr;     // device: offsets of cell diagonals in cells
   int16_t *cells = nullptr;    // device: packed (i | j<<5 | k<<10), by diagonal
-  size_t lds_bytes(int real_bytes) const;
+  size_t lds_bytes(int real_bytes, bool recompute_dinv) const;
   size_t face_doubles_per_sweep() const;  // x + y + z faces, all tiles
 };
 int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int TZ, int waves);
@@ -160,13 +162,19 @@ struct SweepLaunch {
   int corner0;
   int fixup;
   double hi, hj, hk;
-  Acc3 qtot, dinv, txs;        // inputs, group stride in sg (dinv/txs FP32 when fp32)
-  int sink_atomic;             // 1: atomic add into flux/fluxm; 0: write partials
-  Acc3 flux, fluxm;
+  bool general;      // true: accessors, ghosts, atomic sink (one sweep per call)
+  // dense inputs and partial outputs (general == false)
+  const double *q;             // [ng_job][cells][cmom]
+  const void *dinv_dense;      // [ng_job][cells][nang] of the sweep's real type
+  const void *txs_dense;       // [ng_job][cells] of the sweep's real type
   double *part_flux, *part_fluxm;  // [8][ng_job][cells], [8][ng_job][cells][cmom-1]
+  // accessors (general == true)
+  Acc3 qtot, dinv, txs, flux, fluxm;
   Acc2 gx, gy, gz;             // ghost faces in physical coordinates (base NULL = vacuum)
   void *faces;                 // [nsweeps][face_doubles_per_sweep] elements of Real
   bool fp32;                   // FP32 sweep arithmetic and faces
+  bool exact;                  // FP64 in the reference's operation order, no fused ops
+  bool recompute_dinv;         // form dinv from t_xs in the kernel instead of reading it
 };
 // Enqueue every tile diagonal of the sweep (plan->nplanes launches).  When
 // `events` is given (2*nplanes events) each launch is bracketed by a pair.

@@ -80,7 +80,8 @@ struct snap_solver {
     return SNAP_OK;
   }
 
-  int sweep(bool fp32, hipEvent_t *events) {
+  int sweep(int precision, hipEvent_t *events) {
+    const bool fp32 = precision == SNAP_PRECISION_MIXED;
     SweepLaunch L;
     memset(&L, 0, sizeof(L));
     L.plan = &plan;
@@ -92,14 +93,17 @@ struct snap_solver {
     L.hi = hi;
     L.hj = hj;
     L.hk = hk;
-    L.qtot = g_acc(qtot, cmom, 8);
-    L.dinv = fp32 ? g_acc(dinv32, nang, 4) : g_acc(dinv, nang, 8);
-    L.txs = fp32 ? g_acc(txs32, 1, 4) : g_acc(txs, 1, 8);
-    L.sink_atomic = 0;
+    L.general = false;
+    L.q = qtot.as<double>();
+    L.dinv_dense = fp32 ? dinv32.p : dinv.p;
+    L.txs_dense = fp32 ? txs32.p : txs.p;
     L.part_flux = part_flux.as<double>();
     L.part_fluxm = cmom > 1 ? part_fluxm.as<double>() : nullptr;
     L.faces = faces.p;
     L.fp32 = fp32;
+    L.exact = precision == SNAP_PRECISION_FP64_EXACT;
+    L.recompute_dinv = fp32 ? (opt.dinv_mode != SNAP_DINV_STREAM) : (opt.dinv_mode == SNAP_DINV_RECOMPUTE);
+    if (fp32 && !L.recompute_dinv && !dinv32.p) return fail(SNAP_ERR_INVALID, "no FP32 dinv");
     return sweep_enqueue(L, stream, events);
   }
 
@@ -166,7 +170,8 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
   std::unique_ptr<snap_solver> s(new snap_solver());
   s->prob = *p;
   if (o) s->opt = *o;
-  if (s->opt.precision != SNAP_PRECISION_FP64 && s->opt.precision != SNAP_PRECISION_MIXED)
+  if (s->opt.precision != SNAP_PRECISION_FP64 && s->opt.precision != SNAP_PRECISION_MIXED &&
+      s->opt.precision != SNAP_PRECISION_FP64_EXACT)
     return fail(SNAP_ERR_INVALID, "unknown precision %d", s->opt.precision);
   s->e = Extent3{p->nx, p->ny, p->nz};
   s->ng = p->ng;
@@ -288,7 +293,9 @@ extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) 
       SNAP_TRY(s->copy(s->flux0pi, s->flux0, sizeof(double) * gc));
       const bool fp32 = mixed && !(s->opt.first_sweep_fp64 && st.inner_total == 0);
       SNAP_HIP_TRY(hipEventRecord(s->ev[0], stream));
-      SNAP_TRY(s->sweep(fp32, nullptr));
+      SNAP_TRY(s->sweep(fp32 ? SNAP_PRECISION_MIXED
+                             : (mixed ? SNAP_PRECISION_FP64 : s->opt.precision),
+                        nullptr));
       SNAP_HIP_TRY(hipEventRecord(s->ev[1], stream));
       SNAP_TRY(s->reduce_octants());
       if (fp32)
@@ -339,11 +346,12 @@ extern "C" int snap_hip_solver_time_sweeps(snap_solver_t *s, int count, int prec
   if (!s || count < 1) return fail(SNAP_ERR_INVALID, "bad timing request");
   const bool fp32 = precision == SNAP_PRECISION_MIXED;
   if (fp32 && !s->have_fp32) return fail(SNAP_ERR_INVALID, "solver was created without mixed precision");
+  if (precision < 0 || precision > 2) return fail(SNAP_ERR_INVALID, "unknown precision %d", precision);
   const int np = s->plan.nplanes;
   std::vector<hipEvent_t> evs((size_t)2 * np * count);
   for (auto &x : evs) SNAP_HIP_TRY(hipEventCreate(&x));
   SNAP_HIP_TRY(hipEventRecord(s->ev[0], s->stream));
-  for (int i = 0; i < count; i++) SNAP_TRY(s->sweep(fp32, evs.data() + (size_t)2 * np * i));
+  for (int i = 0; i < count; i++) SNAP_TRY(s->sweep(precision, evs.data() + (size_t)2 * np * i));
   SNAP_HIP_TRY(hipEventRecord(s->ev[1], s->stream));
   SNAP_HIP_TRY(hipEventSynchronize(s->ev[1]));
   float total = 0.f;

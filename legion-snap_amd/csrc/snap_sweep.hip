@@ -37,13 +37,48 @@
 namespace snap {
 
 // ---------------------------------------------------------------------------
+// LDS layout of one tile workgroup (byte offsets), shared by host and device
+// ---------------------------------------------------------------------------
+struct SweepLds {
+  size_t ecs, ect, pwb, cix, qsl, cells, coff, total;  // byte offsets from the start of LDS
+};
+
+__host__ __device__ inline size_t a16(size_t b) { return (b + 15) & ~size_t(15); }
+
+constexpr int kMaxSteps = 4;
+
+// faces [x|y|z] (Real) | ec of the corner [cmom][nang] and transposed
+// [nang][cmom] | per-wave w*pc batches [W][SB][nang] | their cell indices
+// [W][SB] | per-wave source moments [W][kMaxSteps][CPW][cmom] | the tile's
+// cell list [ncells] and diagonal offsets [ncp+1]
+__host__ __device__ inline SweepLds sweep_lds(int TX, int TY, int TZ, int nang, int cmom, int W,
+                                              int SB, int ncells_tile, int ncp, int cpw, int rb) {
+  SweepLds L;
+  size_t o = a16((size_t)(TY * TZ + TX * TZ + TX * TY) * nang * rb);
+  L.ecs = o;
+  o += (size_t)cmom * nang * 8;
+  L.ect = o;
+  o += (size_t)cmom * nang * 8;
+  L.pwb = o;
+  o += (size_t)W * SB * nang * 8;
+  L.qsl = o;
+  o += (size_t)W * kMaxSteps * cpw * cmom * 8;
+  L.cix = o;
+  o = a16(o + (size_t)W * SB * 4);
+  L.cells = o;
+  o = a16(o + (size_t)ncells_tile * 2);
+  L.coff = o;
+  o = a16(o + (size_t)(ncp + 1) * 4);
+  L.total = o;
+  return L;
+}
+
+// ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
-static size_t align16(size_t b) { return (b + 15) & ~size_t(15); }
-
-size_t SweepPlan::lds_bytes(int real_bytes) const {
-  const size_t faces = (size_t)(TY * TZ + TX * TZ + TX * TY) * nang * real_bytes;
-  return align16(faces) + (size_t)cmom * nang * sizeof(double) + (size_t)waves * 64 * sizeof(double);
+size_t SweepPlan::lds_bytes(int real_bytes, bool recompute_dinv) const {
+  (void)recompute_dinv;
+  return sweep_lds(TX, TY, TZ, nang, cmom, waves, sb, TX * TY * TZ, ncellplanes, cpw, real_bytes).total;
 }
 
 size_t SweepPlan::face_doubles_per_sweep() const {
@@ -60,12 +95,13 @@ static int pick_cpw(int nang) {
 }
 
 void plan_default_tile(Extent3 e, int nang, int cmom, int *TX, int *TY, int *TZ, int *waves) {
-  // 8^3 tiles: 22 cell diagonals per tile, 1.5 face doubles of HBM traffic
-  // per cell-angle.  Clamp to the chunk.
+  // 8x8x4 tiles: 18 cell diagonals of at most 32 cells, two workgroups per
+  // CU with FP64 faces and double-buffered staging at 36 angles.  Clamp to
+  // the chunk.
   *TX = std::min(8, e.nx);
   *TY = std::min(8, e.ny);
-  *TZ = std::min(8, e.nz);
-  *waves = 4;
+  *TZ = std::min(4, e.nz);
+  *waves = 8;
   (void)nang;
   (void)cmom;
 }
@@ -101,10 +137,12 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
   p->cmom = cmom;
   p->waves = waves;
   p->cpw = pick_cpw(nang);
-  if (p->lds_bytes(8) > 160 * 1024)
-    return fail(SNAP_ERR_INVALID, "tile %dx%dx%d with %d angles needs %zu B of LDS (> 160 KiB)",
-                TX, TY, TZ, nang, p->lds_bytes(8));
-
+  // cells per moment batch: enough (cell, moment) sums to fill a wavefront
+  {
+    int sbm = std::min(64 / cmom, 8);
+    if (sbm < 1) sbm = 1;
+    p->sb = std::max(p->cpw, (sbm / p->cpw) * p->cpw);
+  }
   // tile diagonals in sweep order
   p->nplanes = p->nTx + p->nTy + p->nTz - 2;
   std::vector<int> tiles;
@@ -123,6 +161,7 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
   p->ncellplanes = TX + TY + TZ - 2;
   std::vector<int16_t> cells;
   std::vector<int> coff(p->ncellplanes + 1, 0);
+  p->maxdiag = 0;
   for (int d = 0; d < p->ncellplanes; d++) {
     coff[d] = (int)cells.size();
     for (int k = 0; k < TZ; k++)
@@ -130,8 +169,18 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
         const int i = d - j - k;
         if (i >= 0 && i < TX) cells.push_back((int16_t)(i | (j << 5) | (k << 10)));
       }
+    p->maxdiag = std::max(p->maxdiag, (int)cells.size() - coff[d]);
   }
   coff[p->ncellplanes] = (int)cells.size();
+  // every wavefront owns at most kMaxSteps cell groups of a diagonal
+  const int need_waves = (p->maxdiag + kMaxSteps * p->cpw - 1) / (kMaxSteps * p->cpw);
+  if (p->waves < need_waves) p->waves = need_waves;
+  if (p->waves > 16)
+    return fail(SNAP_ERR_INVALID, "tile %dx%dx%d needs %d waves per workgroup (> 16)", TX, TY, TZ,
+                p->waves);
+  if (p->lds_bytes(8, false) > 160 * 1024)
+    return fail(SNAP_ERR_INVALID, "tile %dx%dx%d with %d angles needs %zu B of LDS (> 160 KiB)",
+                TX, TY, TZ, nang, p->lds_bytes(8, false));
 
   SNAP_HIP_TRY(hipMalloc((void **)&p->plane_tiles, sizeof(int) * tiles.size()));
   SNAP_HIP_TRY(hipMalloc((void **)&p->cell_off, sizeof(int) * coff.size()));
@@ -148,20 +197,31 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
 // ---------------------------------------------------------------------------
 // kernel
 // ---------------------------------------------------------------------------
+// Arguments shared by both variants; the dense variant (the solver's own
+// [g][cell][record] layout) addresses everything from these base pointers.
 struct SweepArgs {
   Extent3 ext;
   int TX, TY, TZ, nTx, nTy, nTz, ntiles;
-  int nang, cmom, ncellplanes;
-  int ng_job, corner0, fixup, sink_atomic;
+  int nang, cmom, ncellplanes, ncells_tile, sb;
+  int ng_job, corner0, fixup, recompute_dinv;
   double hi, hj, hk;
   const double *mu, *eta, *xi, *w, *ec;
-  Acc3 qtot, dinv, txs, flux, fluxm;
-  double *part_flux, *part_fluxm;
-  Acc2 gx, gy, gz;
+  const double *q;        // dense: [ng_job][cells][cmom]
+  const void *dinv;       // dense: [ng_job][cells][nang] of Real
+  const void *txs;        // dense: [ng_job][cells] of Real
+  double *part_flux;      // [8][ng_job][cells]
+  double *part_fluxm;     // [8][ng_job][cells][cmom-1]
   void *faces;
   const int *plane_tiles;
   const int *cell_off;
   const int16_t *cells;
+};
+
+// Arguments of the general variant (the drop-in entry point): Realm affine
+// accessors, ghost faces, and flux accumulation by device atomics.
+struct SweepGeneral {
+  Acc3 qtot, dinv, txs, flux, fluxm;
+  Acc2 gx, gy, gz;
 };
 
 __device__ inline void wave_sync() {
@@ -170,17 +230,40 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename Real, int CPW>
-__global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs A, int plane_off) {
+// a*b + c: one rounding (fused) in the fast variants, two in the exact one
+template <bool EXACT>
+__device__ inline double madd(double a, double b, double c) {
+  if constexpr (EXACT)
+    return a * b + c;
+  else
+    return __builtin_fma(a, b, c);
+}
+__device__ inline float maddf(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Tile sweep.  Real: arithmetic and face type.  CPW: cells per wavefront.
+// EXACT: the reference's operation order with separate roundings (bit-equal
+// to the CPU oracle); otherwise fused multiply-adds.  GEN: general accessors,
+// ghost faces and atomic accumulation (the drop-in entry point).
+template <typename Real, int CPW, bool EXACT, bool GEN>
+__global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs A, SweepGeneral G, int plane_off) {
   constexpr bool kFP64 = std::is_same<Real, double>::value;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int TX = A.TX, TY = A.TY, TZ = A.TZ, nang = A.nang, cmom = A.cmom;
+  const int TX = A.TX, TY = A.TY, TZ = A.TZ, nang = A.nang, cmom = A.cmom, SB = A.sb;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
+  const bool recompute = A.recompute_dinv != 0;
+  const int ncp = A.ncellplanes;
+  const SweepLds lay = sweep_lds(TX, TY, TZ, nang, cmom, W, SB, A.ncells_tile, ncp, CPW, (int)sizeof(Real));
   const int fxn = TY * TZ * nang, fyn = TX * TZ * nang, fzn = TX * TY * nang;
   Real *fx = (Real *)smem_raw;
   Real *fy = fx + fxn;
   Real *fz = fy + fyn;
-  double *ecs = (double *)(smem_raw + ((sizeof(Real) * (size_t)(fxn + fyn + fzn) + 15) & ~size_t(15)));
-  double *scratch = ecs + cmom * nang;
+  double *ecs = (double *)(smem_raw + lay.ecs);   // [cmom][nang]
+  double *ect = (double *)(smem_raw + lay.ect);   // [nang][cmom]
+  double *pwb = (double *)(smem_raw + lay.pwb) + wave * SB * nang;  // this wave's w*pc batch
+  int *cix = (int *)(smem_raw + lay.cix) + wave * SB;               // batch cell indices
+  double *qsl = (double *)(smem_raw + lay.qsl) + wave * kMaxSteps * CPW * cmom;  // source moments
+  int16_t *tcells = (int16_t *)(smem_raw + lay.cells);              // cell list of the tile
+  int *toff = (int *)(smem_raw + lay.coff);                         // its diagonal offsets
 
   const int s = blockIdx.y;
   const int corner = A.corner0 + s / A.ng_job;
@@ -192,125 +275,240 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs A, int plane_off
   const int ex = min(TX, nx - x0), ey = min(TY, ny - y0), ez = min(TZ, nz - z0);
   const bool sxp = (corner & 1) != 0, syp = (corner & 2) != 0, szp = (corner & 4) != 0;
   const int tile_id = (tz * A.nTy + ty) * A.nTx + tx;
-  const size_t per_tile = (size_t)(fxn + fyn + fzn);
+  const int per_tile = fxn + fyn + fzn;
   Real *faces_s = (Real *)A.faces + (size_t)s * A.ntiles * per_tile;
+  const int ncells = nx * ny * nz;
+  // dense per-sweep bases
+  const double *qg = A.q + (size_t)g * ncells * cmom;
+  const Real *dg = (const Real *)A.dinv + (size_t)g * ncells * nang;
+  const Real *tg = (const Real *)A.txs + (size_t)g * ncells;
+  double *pfg = A.part_flux + ((size_t)corner * A.ng_job + g) * ncells;
+  double *pmg = A.part_fluxm + ((size_t)corner * A.ng_job + g) * ncells * (cmom - 1);
 
-  // ---- tile inflow faces: from the upstream tile, the ghost array, or vacuum
-  for (int idx = threadIdx.x; idx < fxn; idx += blockDim.x) {
-    const int a = idx % nang, slot = idx / nang, j = slot % TY, k = slot / TY;
-    Real v = Real(0);
-    if (j < ey && k < ez) {
-      if (tx > 0) {
-        v = faces_s[(size_t)(tile_id - 1) * per_tile + idx];
-      } else if (A.gx.base) {
-        const int py = syp ? y0 + j : ny - 1 - (y0 + j), pz = szp ? z0 + k : nz - 1 - (z0 + k);
-        v = (Real)((const double *)A.gx.at(py, pz, 0))[a];
-      }
-    }
-    fx[idx] = v;
-  }
-  for (int idx = threadIdx.x; idx < fyn; idx += blockDim.x) {
-    const int a = idx % nang, slot = idx / nang, i = slot % TX, k = slot / TX;
-    Real v = Real(0);
-    if (i < ex && k < ez) {
-      if (ty > 0) {
-        v = faces_s[(size_t)(tile_id - A.nTx) * per_tile + fxn + idx];
-      } else if (A.gy.base) {
-        const int px = sxp ? x0 + i : nx - 1 - (x0 + i), pz = szp ? z0 + k : nz - 1 - (z0 + k);
-        v = (Real)((const double *)A.gy.at(px, pz, 0))[a];
-      }
-    }
-    fy[idx] = v;
-  }
-  for (int idx = threadIdx.x; idx < fzn; idx += blockDim.x) {
-    const int a = idx % nang, slot = idx / nang, i = slot % TX, j = slot / TX;
-    Real v = Real(0);
-    if (i < ex && j < ey) {
-      if (tz > 0) {
-        v = faces_s[(size_t)(tile_id - A.nTx * A.nTy) * per_tile + fxn + fyn + idx];
-      } else if (A.gz.base) {
-        const int px = sxp ? x0 + i : nx - 1 - (x0 + i), py = syp ? y0 + j : ny - 1 - (y0 + j);
-        v = (Real)((const double *)A.gz.at(px, py, 0))[a];
-      }
-    }
-    fz[idx] = v;
-  }
-  for (int idx = threadIdx.x; idx < cmom * nang; idx += blockDim.x)
-    ecs[idx] = A.ec[(size_t)corner * cmom * nang + idx];
-  __syncthreads();
-
-  // ---- per-lane constants
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
+  // ---- per-lane constants: lane = cell slot * nang + angle
   const int cl = (CPW == 1) ? 0 : lane / nang;
   const int a = lane - cl * nang;
   const bool lane_ok = (cl < CPW) && (a < nang);
   const unsigned long long seg =
       lane_ok ? (((nang >= 64) ? ~0ull : ((1ull << nang) - 1ull)) << (cl * nang)) : 0ull;
   const int aa = lane_ok ? a : 0;
+  const int per_step = W * CPW;
+
+  // ---- tables and inflow faces (from the upstream tile, the ghost array,
+  // or vacuum) into LDS
+  for (int idx = threadIdx.x; idx < A.ncells_tile; idx += blockDim.x) tcells[idx] = A.cells[idx];
+  for (int idx = threadIdx.x; idx <= ncp; idx += blockDim.x) toff[idx] = A.cell_off[idx];
+  for (int idx = threadIdx.x; idx < cmom * nang; idx += blockDim.x) {
+    const double v = A.ec[(size_t)corner * cmom * nang + idx];
+    const int l = idx / nang, b = idx - l * nang;
+    ecs[idx] = v;
+    ect[b * cmom + l] = v;
+  }
+  for (int idx = threadIdx.x; idx < fxn; idx += blockDim.x) {
+    const int a2 = idx % nang, slot = idx / nang, j = slot % TY, k = slot / TY;
+    Real v = Real(0);
+    if (j < ey && k < ez) {
+      if (tx > 0) {
+        v = faces_s[(size_t)(tile_id - 1) * per_tile + idx];
+      } else if (GEN && G.gx.base) {
+        const int py = syp ? y0 + j : ny - 1 - (y0 + j), pz = szp ? z0 + k : nz - 1 - (z0 + k);
+        v = (Real)((const double *)G.gx.at(py, pz, 0))[a2];
+      }
+    }
+    fx[idx] = v;
+  }
+  for (int idx = threadIdx.x; idx < fyn; idx += blockDim.x) {
+    const int a2 = idx % nang, slot = idx / nang, i = slot % TX, k = slot / TX;
+    Real v = Real(0);
+    if (i < ex && k < ez) {
+      if (ty > 0) {
+        v = faces_s[(size_t)(tile_id - A.nTx) * per_tile + fxn + idx];
+      } else if (GEN && G.gy.base) {
+        const int px = sxp ? x0 + i : nx - 1 - (x0 + i), pz = szp ? z0 + k : nz - 1 - (z0 + k);
+        v = (Real)((const double *)G.gy.at(px, pz, 0))[a2];
+      }
+    }
+    fy[idx] = v;
+  }
+  for (int idx = threadIdx.x; idx < fzn; idx += blockDim.x) {
+    const int a2 = idx % nang, slot = idx / nang, i = slot % TX, j = slot / TX;
+    Real v = Real(0);
+    if (i < ex && j < ey) {
+      if (tz > 0) {
+        v = faces_s[(size_t)(tile_id - A.nTx * A.nTy) * per_tile + fxn + fyn + idx];
+      } else if (GEN && G.gz.base) {
+        const int px = sxp ? x0 + i : nx - 1 - (x0 + i), py = syp ? y0 + j : ny - 1 - (y0 + j);
+        v = (Real)((const double *)G.gz.at(px, py, 0))[a2];
+      }
+    }
+    fz[idx] = v;
+  }
+
   const double mu_d = A.mu[aa], eta_d = A.eta[aa], xi_d = A.xi[aa], w_d = A.w[aa];
-  // FP64 keeps (psi*mu)*hi as the reference writes it; FP32 folds mu*hi in FP64.
+  // EXACT keeps (psi*mu)*hi as the reference writes it; the others fold mu*hi.
   const Real mu_a = (Real)mu_d, eta_a = (Real)eta_d, xi_a = (Real)xi_d;
   const Real hi = (Real)A.hi, hj = (Real)A.hj, hk = (Real)A.hk;
   const Real muh = (Real)(mu_d * A.hi), etah = (Real)(eta_d * A.hj), xih = (Real)(xi_d * A.hk);
+  // dinv recomputed as CalculateGeometryParam forms it (expxs.cc:378-381)
+  const double g_hm = A.hi * mu_d, g_he = A.hj * eta_d, g_hx = A.hk * xi_d;
+  const float g_sum32 = (float)(g_hm + g_he + g_hx);
   const Real tolr = (Real)1.0e-12;
-  const int64_t cells_total = (int64_t)nx * ny * nz;
   const int nm1 = cmom - 1;
+  const bool q_in_lanes = cmom <= nang;
+  __syncthreads();
 
-  for (int d = 0; d < A.ncellplanes; ++d) {
-    const int c0 = A.cell_off[d], c1 = A.cell_off[d + 1];
-    for (int base = c0; base < c1; base += W * CPW) {
-      const int my = base + wave * CPW + cl;
-      bool valid = lane_ok && my < c1;
-      int i = 0, j = 0, k = 0;
-      if (valid) {
-        const int pk = A.cells[my];
-        i = pk & 31;
-        j = (pk >> 5) & 31;
-        k = (pk >> 10) & 31;
-        valid = (i < ex) && (j < ey) && (k < ez);
-      }
-      double pcw = 0.0;
-      if (valid) {
+  // ---- register prefetch of this wave's cells on the next diagonal: dinv
+  // (lane a holds angle a), the source moments (lane l of the cell's segment
+  // holds moment l when cmom <= nang) and t_xs.
+  double n_dv[kMaxSteps], n_q[kMaxSteps], n_tx[kMaxSteps];
+  int n_cell[kMaxSteps];
+  auto cell_of = [&](int idx, int &i, int &j, int &k) -> bool {
+    const int pk = tcells[idx];
+    i = pk & 31;
+    j = (pk >> 5) & 31;
+    k = (pk >> 10) & 31;
+    return (i < ex) && (j < ey) && (k < ez);
+  };
+  auto prefetch = [&](int dd) {
+    const int c0 = toff[dd], c1 = toff[dd + 1];
+#pragma unroll
+    for (int st = 0; st < kMaxSteps; st++) {
+      const int my = c0 + st * per_step + wave * CPW + cl;
+      int i, j, k;
+      n_dv[st] = 0.0;
+      n_q[st] = 0.0;
+      n_tx[st] = 0.0;
+      n_cell[st] = -1;
+      if (lane_ok && my < c1 && cell_of(my, i, j, k)) {
         const int X = x0 + i, Y = y0 + j, Z = z0 + k;
         const int px = sxp ? X : nx - 1 - X, py = syp ? Y : ny - 1 - Y, pz = szp ? Z : nz - 1 - Z;
+        const int cell = (pz * ny + py) * nx + px;
+        n_cell[st] = cell;
+        if constexpr (GEN) {
+          if (!recompute) n_dv[st] = (double)((const Real *)G.dinv.at(px, py, pz, 0))[a];
+          if (q_in_lanes && a < cmom) n_q[st] = ((const double *)G.qtot.at(px, py, pz, 0))[a];
+          n_tx[st] = (double)*(const Real *)G.txs.at(px, py, pz, 0);
+        } else {
+          if (!recompute) n_dv[st] = (double)dg[(size_t)cell * nang + a];
+          if (q_in_lanes && a < cmom) n_q[st] = qg[(size_t)cell * cmom + a];
+          n_tx[st] = (double)tg[cell];
+        }
+      }
+    }
+  };
+
+  int nbuf = 0;  // cells in this wave's moment batch (wave-uniform)
+  auto flush = [&]() {
+    // per-cell flux moments over angles in ascending angle order
+    // (sweep.cc:568-591): phi_l = sum_a ec[l][a] * (w[a]*pc[a])
+    wave_sync();
+    for (int o = lane; o < nbuf * cmom; o += 64) {
+      const int kk = o / cmom, l = o - kk * cmom;
+      const int cell = cix[kk];
+      if (cell < 0) continue;
+      const double *v = pwb + kk * nang;
+      double sum = 0.0;
+      for (int b = 0; b < nang; b++) sum = madd<EXACT>(ect[b * cmom + l], v[b], sum);
+      if constexpr (GEN) {
+        const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
+        double *dst = (l == 0) ? (double *)G.flux.at(px, py, pz, 0)
+                               : (double *)G.fluxm.at(px, py, pz, 0) + (l - 1);
+        atomicAdd(dst, sum);
+      } else {
+        if (l == 0)
+          pfg[cell] = sum;
+        else
+          pmg[(size_t)cell * nm1 + (l - 1)] = sum;
+      }
+    }
+    wave_sync();
+    nbuf = 0;
+  };
+
+  prefetch(0);
+  for (int d = 0; d < ncp; ++d) {
+    const int c0 = toff[d], c1 = toff[d + 1];
+    const int nsteps = (c1 - c0 + per_step - 1) / per_step;
+    double c_dv[kMaxSteps], c_tx[kMaxSteps];
+    int c_cell[kMaxSteps];
+    // source moments of this diagonal's cells to the wave's LDS slots
+#pragma unroll
+    for (int st = 0; st < kMaxSteps; st++) {
+      c_dv[st] = n_dv[st];
+      c_tx[st] = n_tx[st];
+      c_cell[st] = n_cell[st];
+      if (q_in_lanes && lane_ok && a < cmom) qsl[(st * CPW + cl) * cmom + a] = n_q[st];
+    }
+    if (d + 1 < ncp) prefetch(d + 1);
+    wave_sync();
+#pragma unroll
+    for (int st = 0; st < kMaxSteps; ++st) {
+      if (st >= nsteps) break;
+      const int my = c0 + st * per_step + wave * CPW + cl;
+      const int cell = c_cell[st];
+      const bool valid = cell >= 0;
+      double pcw = 0.0;
+      if (valid) {
+        const int pk = tcells[my];
+        const int i = pk & 31, j = (pk >> 5) & 31, k = (pk >> 10) & 31;
         // angular source (sweep.cc:331-345)
-        const double *q = (const double *)A.qtot.at(px, py, pz, g);
+        const double *q;
+        if (q_in_lanes) {
+          q = qsl + (st * CPW + cl) * cmom;
+        } else if constexpr (GEN) {
+          const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
+          q = (const double *)G.qtot.at(px, py, pz, 0);
+        } else {
+          q = qg + (size_t)cell * cmom;
+        }
         Real psi;
         if constexpr (kFP64) {
           double ps = q[0];
-          for (int l = 1; l < cmom; l++) ps += ecs[l * nang + a] * q[l];
+          for (int l = 1; l < cmom; l++) ps = madd<EXACT>(ecs[l * nang + a], q[l], ps);
           psi = ps;
         } else {
-          float ps = (float)q[0];
-          for (int l = 1; l < cmom; l++) ps = __builtin_fmaf((float)ecs[l * nang + a], (float)q[l], ps);
-          psi = ps;
+          float pf = (float)q[0];
+          for (int l = 1; l < cmom; l++) pf = maddf((float)ecs[l * nang + a], (float)q[l], pf);
+          psi = pf;
+        }
+        const Real txs = (Real)c_tx[st];
+        Real dv;
+        if (recompute) {
+          if constexpr (kFP64)
+            dv = 1.0 / ((((double)txs + g_hm) + g_he) + g_hx);
+          else
+            dv = 1.0f / ((float)txs + g_sum32);
+        } else {
+          dv = (Real)c_dv[st];
         }
         Real *sx = fx + (k * TY + j) * nang + a;
         Real *sy = fy + (k * TX + i) * nang + a;
         Real *sz = fz + (j * TX + i) * nang + a;
         const Real psii = *sx, psij = *sy, psik = *sz;
-        const Real dv = ((const Real *)A.dinv.at(px, py, pz, g))[a];
         // initial diamond-difference solution (sweep.cc:352-400)
         Real pc = psi;
-        if constexpr (kFP64) {
+        if constexpr (kFP64 && EXACT) {
           pc += psii * mu_a * hi;
           pc += psij * eta_a * hj;
           pc += psik * xi_a * hk;
+        } else if constexpr (kFP64) {
+          pc = __builtin_fma(psii, muh, pc);
+          pc = __builtin_fma(psij, etah, pc);
+          pc = __builtin_fma(psik, xih, pc);
         } else {
-          pc = __builtin_fmaf(psii, muh, pc);
-          pc = __builtin_fmaf(psij, etah, pc);
-          pc = __builtin_fmaf(psik, xih, pc);
+          pc = maddf(psii, muh, pc);
+          pc = maddf(psij, etah, pc);
+          pc = maddf(psik, xih, pc);
         }
         pc *= dv;
         Real ox, oy, oz;
         if (A.fixup) {
-          // negative-flux fixup (sweep.cc:402-505; gpu_sweep.cu:1556-1630)
-          const Real txs = *(const Real *)A.txs.at(px, py, pz, g);
+          // negative-flux fixup (sweep.cc:402-505; gpu_sweep.cu:1556-1630).
+          // Each pass that continues clears at least one of the 3*nang
+          // flags or ends the loop, so 3*nang+2 passes always suffice.
           Real hvx = Real(1), hvy = Real(1), hvz = Real(1);
           Real fxx = Real(0), fxy = Real(0), fxz = Real(0);
           unsigned old_neg = 0;
-          // each pass that continues clears at least one of the 3*nang flags
-          // or stops, so 3*nang+2 passes always suffice
           for (int it = 0; it < 3 * nang + 2; ++it) {
             fxx = Real(2) * pc - psii;
             fxy = Real(2) * pc - psij;
@@ -325,14 +523,15 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs A, int plane_off
             if (neg == old_neg) break;
             old_neg = neg;
             Real den;
-            if constexpr (kFP64) {
+            if constexpr (kFP64 && EXACT) {
               pc = psi + 0.5 * (psii * mu_a * hi * (1.0 + hvx) + psij * eta_a * hj * (1.0 + hvy) +
                                 psik * xi_a * hk * (1.0 + hvz));
               den = (pc <= 0.0) ? 0.0 : (txs + mu_a * hi * hvx + eta_a * hj * hvy + xi_a * hk * hvz);
             } else {
-              pc = psi + 0.5f * (psii * muh * (1.0f + hvx) + psij * etah * (1.0f + hvy) +
-                                 psik * xih * (1.0f + hvz));
-              den = (pc <= 0.0f) ? 0.0f : (txs + muh * hvx + etah * hvy + xih * hvz);
+              const Real t = psii * muh * (Real(1) + hvx) + psij * etah * (Real(1) + hvy) +
+                             psik * xih * (Real(1) + hvz);
+              pc = psi + Real(0.5) * t;
+              den = (pc <= Real(0)) ? Real(0) : (txs + muh * hvx + etah * hvy + xih * hvz);
             }
             pc = (den < tolr) ? Real(0) : pc / den;
           }
@@ -349,70 +548,48 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs A, int plane_off
         *sz = oz;
         pcw = w_d * (double)pc;
       }
-      // ---- per-cell moment sums over angles, ascending angle order
-      scratch[wave * 64 + lane] = pcw;
-      wave_sync();
-      for (int o = lane; o < CPW * cmom; o += 64) {
-        const int c2 = o / cmom, l = o - c2 * cmom;
-        const int my2 = base + wave * CPW + c2;
-        if (my2 >= c1) continue;
-        const int pk = A.cells[my2];
-        const int i2 = pk & 31, j2 = (pk >> 5) & 31, k2 = (pk >> 10) & 31;
-        if (i2 >= ex || j2 >= ey || k2 >= ez) continue;
-        const int X = x0 + i2, Y = y0 + j2, Z = z0 + k2;
-        const int px = sxp ? X : nx - 1 - X, py = syp ? Y : ny - 1 - Y, pz = szp ? Z : nz - 1 - Z;
-        const double *e = ecs + l * nang;
-        const double *v = scratch + wave * 64 + c2 * nang;
-        double sum = 0.0;
-        for (int b = 0; b < nang; b++) sum += e[b] * v[b];
-        if (A.sink_atomic) {
-          double *dst = (l == 0) ? (double *)A.flux.at(px, py, pz, g)
-                                 : (double *)A.fluxm.at(px, py, pz, g) + (l - 1);
-          atomicAdd(dst, sum);
-        } else {
-          const int64_t cell = ((int64_t)pz * ny + py) * nx + px;
-          const int64_t row = ((int64_t)corner * A.ng_job + g) * cells_total + cell;
-          if (l == 0)
-            A.part_flux[row] = sum;
-          else
-            A.part_fluxm[row * nm1 + (l - 1)] = sum;
-        }
+      // ---- batch this step's cells for the moment sums
+      if (lane_ok) {
+        pwb[(nbuf + cl) * nang + a] = pcw;
+        if (a == 0) cix[nbuf + cl] = (my < c1) ? cell : -1;
       }
-      wave_sync();
+      nbuf += CPW;
+      if (nbuf + CPW > SB) flush();
     }
     __syncthreads();
   }
+  if (nbuf > 0) flush();
 
   // ---- tile outflow faces: to the face buffer, and at the chunk's far side
   // to the ghost arrays
   Real *out = faces_s + (size_t)tile_id * per_tile;
   for (int idx = threadIdx.x; idx < fxn; idx += blockDim.x) {
     out[idx] = fx[idx];
-    if (tx == A.nTx - 1 && A.gx.base) {
+    if (GEN && tx == A.nTx - 1 && G.gx.base) {
       const int a2 = idx % nang, slot = idx / nang, j = slot % TY, k = slot / TY;
       if (j < ey && k < ez) {
         const int py = syp ? y0 + j : ny - 1 - (y0 + j), pz = szp ? z0 + k : nz - 1 - (z0 + k);
-        ((double *)A.gx.at(py, pz, 0))[a2] = (double)fx[idx];
+        ((double *)G.gx.at(py, pz, 0))[a2] = (double)fx[idx];
       }
     }
   }
   for (int idx = threadIdx.x; idx < fyn; idx += blockDim.x) {
     out[fxn + idx] = fy[idx];
-    if (ty == A.nTy - 1 && A.gy.base) {
+    if (GEN && ty == A.nTy - 1 && G.gy.base) {
       const int a2 = idx % nang, slot = idx / nang, i = slot % TX, k = slot / TX;
       if (i < ex && k < ez) {
         const int px = sxp ? x0 + i : nx - 1 - (x0 + i), pz = szp ? z0 + k : nz - 1 - (z0 + k);
-        ((double *)A.gy.at(px, pz, 0))[a2] = (double)fy[idx];
+        ((double *)G.gy.at(px, pz, 0))[a2] = (double)fy[idx];
       }
     }
   }
   for (int idx = threadIdx.x; idx < fzn; idx += blockDim.x) {
     out[fxn + fyn + idx] = fz[idx];
-    if (tz == A.nTz - 1 && A.gz.base) {
+    if (GEN && tz == A.nTz - 1 && G.gz.base) {
       const int a2 = idx % nang, slot = idx / nang, i = slot % TX, j = slot / TX;
       if (i < ex && j < ey) {
         const int px = sxp ? x0 + i : nx - 1 - (x0 + i), py = syp ? y0 + j : ny - 1 - (y0 + j);
-        ((double *)A.gz.at(px, py, 0))[a2] = (double)fz[idx];
+        ((double *)G.gz.at(px, py, 0))[a2] = (double)fz[idx];
       }
     }
   }
@@ -421,18 +598,18 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs A, int plane_off
 // ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
-typedef void (*SweepKernel)(SweepArgs, int);
+typedef void (*SweepKernel)(SweepArgs, SweepGeneral, int);
 
-template <typename Real>
+template <typename Real, bool EXACT, bool GEN>
 static SweepKernel kernel_for(int cpw) {
   switch (cpw) {
-    case 1: return k_sweep_tiles<Real, 1>;
-    case 2: return k_sweep_tiles<Real, 2>;
-    case 3: return k_sweep_tiles<Real, 3>;
-    case 4: return k_sweep_tiles<Real, 4>;
-    case 5: return k_sweep_tiles<Real, 5>;
-    case 6: return k_sweep_tiles<Real, 6>;
-    case 8: return k_sweep_tiles<Real, 8>;
+    case 1: return k_sweep_tiles<Real, 1, EXACT, GEN>;
+    case 2: return k_sweep_tiles<Real, 2, EXACT, GEN>;
+    case 3: return k_sweep_tiles<Real, 3, EXACT, GEN>;
+    case 4: return k_sweep_tiles<Real, 4, EXACT, GEN>;
+    case 5: return k_sweep_tiles<Real, 5, EXACT, GEN>;
+    case 6: return k_sweep_tiles<Real, 6, EXACT, GEN>;
+    case 8: return k_sweep_tiles<Real, 8, EXACT, GEN>;
     default: return nullptr;
   }
 }
@@ -455,8 +632,14 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
   if (L.nsweeps < 1 || L.nsweeps > 65535) return fail(SNAP_ERR_INVALID, "bad sweep count");
   if (L.quad->nang != p.nang || L.quad->cmom != p.cmom)
     return fail(SNAP_ERR_INVALID, "quadrature does not match the sweep plan");
+  if (L.general && (L.nsweeps != 1 || L.ng_job != 1))
+    return fail(SNAP_ERR_INVALID, "the general sweep handles one (corner, group) per call");
+  if ((int64_t)p.ext.nx * p.ext.ny * p.ext.nz >= (int64_t)1 << 31)
+    return fail(SNAP_ERR_UNSUPPORTED, "chunks of 2^31 cells or more");
   SweepArgs A;
   memset(&A, 0, sizeof(A));
+  SweepGeneral G;
+  memset(&G, 0, sizeof(G));
   A.ext = p.ext;
   A.TX = p.TX;
   A.TY = p.TY;
@@ -468,10 +651,12 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
   A.nang = p.nang;
   A.cmom = p.cmom;
   A.ncellplanes = p.ncellplanes;
+  A.sb = p.sb;
+  A.ncells_tile = p.TX * p.TY * p.TZ;
+  A.recompute_dinv = L.recompute_dinv ? 1 : 0;
   A.ng_job = L.ng_job;
   A.corner0 = L.corner0;
   A.fixup = L.fixup;
-  A.sink_atomic = L.sink_atomic;
   A.hi = L.hi;
   A.hj = L.hj;
   A.hk = L.hk;
@@ -480,29 +665,45 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
   A.xi = L.quad->xi;
   A.w = L.quad->w;
   A.ec = L.quad->ec;
-  A.qtot = L.qtot;
-  A.dinv = L.dinv;
-  A.txs = L.txs;
-  A.flux = L.flux;
-  A.fluxm = L.fluxm;
+  A.q = L.q;
+  A.dinv = L.dinv_dense;
+  A.txs = L.txs_dense;
   A.part_flux = L.part_flux;
   A.part_fluxm = L.part_fluxm;
-  A.gx = L.gx;
-  A.gy = L.gy;
-  A.gz = L.gz;
   A.faces = L.faces;
   A.plane_tiles = p.plane_tiles;
   A.cell_off = p.cell_off;
   A.cells = p.cells;
-  const size_t lds = p.lds_bytes(L.fp32 ? 4 : 8);
-  SweepKernel k = L.fp32 ? kernel_for<float>(p.cpw) : kernel_for<double>(p.cpw);
+  if (L.general) {
+    G.qtot = L.qtot;
+    G.dinv = L.dinv;
+    G.txs = L.txs;
+    G.flux = L.flux;
+    G.fluxm = L.fluxm;
+    G.gx = L.gx;
+    G.gy = L.gy;
+    G.gz = L.gz;
+    if (!L.recompute_dinv && !G.dinv.base) return fail(SNAP_ERR_INVALID, "dinv is required");
+  } else {
+    if (!A.q || !A.txs || !A.part_flux || (p.cmom > 1 && !A.part_fluxm))
+      return fail(SNAP_ERR_INVALID, "dense sweep inputs missing");
+    if (!L.recompute_dinv && !A.dinv) return fail(SNAP_ERR_INVALID, "dinv is required");
+  }
+  const size_t lds = p.lds_bytes(L.fp32 ? 4 : 8, L.recompute_dinv);
+  SweepKernel k;
+  if (L.general)
+    k = L.fp32 ? kernel_for<float, false, true>(p.cpw)
+               : (L.exact ? kernel_for<double, true, true>(p.cpw) : kernel_for<double, false, true>(p.cpw));
+  else
+    k = L.fp32 ? kernel_for<float, false, false>(p.cpw)
+               : (L.exact ? kernel_for<double, true, false>(p.cpw) : kernel_for<double, false, false>(p.cpw));
   if (!k) return fail(SNAP_ERR_INVALID, "no sweep kernel for %d cells per wave", p.cpw);
   SNAP_TRY(ensure_lds_attr(k, lds));
   const dim3 block(p.waves * 64);
   for (int D = 0; D < p.nplanes; D++) {
     const int count = p.plane_off[D + 1] - p.plane_off[D];
     if (events) SNAP_HIP_TRY(hipEventRecord(events[2 * D], st));
-    hipLaunchKernelGGL(k, dim3(count, L.nsweeps), block, lds, st, A, p.plane_off[D]);
+    hipLaunchKernelGGL(k, dim3(count, L.nsweeps), block, lds, st, A, G, p.plane_off[D]);
     if (events) SNAP_HIP_TRY(hipEventRecord(events[2 * D + 1], st));
   }
   SNAP_HIP_TRY(hipGetLastError());
@@ -655,7 +856,7 @@ extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream
   SNAP_TRY(single_acc3(a.flux, b, &L.flux, "flux"));
   SNAP_TRY(single_acc3(a.fluxm, b, &L.fluxm, "fluxm", c.quad.cmom == 1));
   if (c.quad.cmom > 1 && !L.fluxm.base) return fail(SNAP_ERR_INVALID, "fluxm is required when cmom > 1");
-  L.sink_atomic = 1;
+  L.general = true;
   // ghost points are absolute (y,z) / (x,z) / (x,y): fold the origin in
   auto ghost = [](const snap_accessor2_t &gacc, int64_t o0, int64_t o1) {
     Acc2 r;
@@ -670,5 +871,6 @@ extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream
   L.gz = ghost(a.ghostz, a.origin[0], a.origin[1]);
   L.faces = wk->faces;
   L.fp32 = false;
+  L.exact = true;
   return sweep_enqueue(L, st);
 }

@@ -19,6 +19,10 @@ SNAP_ERR_NO_CONTEXT = -4
 SNAP_ERR_NO_MEMORY = -5
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+PRECISION_FP64_EXACT = 2
+DINV_DEFAULT = 0
+DINV_STREAM = 1
+DINV_RECOMPUTE = 2
 
 _i64 = ctypes.c_int64
 _vp = ctypes.c_void_p
@@ -75,6 +79,7 @@ class SolveOptions(ctypes.Structure):
     _fields_ = [
         ("precision", ctypes.c_int), ("tile", ctypes.c_int * 3),
         ("waves_per_tile", ctypes.c_int), ("first_sweep_fp64", ctypes.c_int),
+        ("dinv_mode", ctypes.c_int),
     ]
 
 

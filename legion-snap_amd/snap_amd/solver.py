@@ -13,9 +13,11 @@ import numpy as np
 from . import _lib
 from .problems import Problem
 
-FP64 = "fp64"
-MIXED = "mixed"
-_PREC = {FP64: _lib.PRECISION_FP64, MIXED: _lib.PRECISION_MIXED}
+FP64 = "fp64"              # FP64 sweep, fused multiply-adds
+FP64_EXACT = "fp64_exact"  # FP64 sweep in the reference CPU operation order
+MIXED = "mixed"            # FP32 sweep, FP64 moments, sources and tests
+_PREC = {FP64: _lib.PRECISION_FP64, MIXED: _lib.PRECISION_MIXED,
+         FP64_EXACT: _lib.PRECISION_FP64_EXACT}
 
 
 def _ptr(a: Optional[np.ndarray], typ=ctypes.c_double):
@@ -28,12 +30,14 @@ def _ptr(a: Optional[np.ndarray], typ=ctypes.c_double):
 class Solver:
     """One problem on the current HIP device.
 
-    precision: "fp64" (the plain drop-in) or "mixed" (FP32 sweeps, FP64
-    moments, sources and convergence tests).
+    precision: "fp64" (FP64 sweeps with fused multiply-adds), "fp64_exact"
+    (FP64 sweeps in the reference CPU operation order, bit-equal to the CPU
+    arithmetic) or "mixed" (FP32 sweeps, FP64 moments, sources and tests).
     """
 
     def __init__(self, problem: Problem, precision: str = FP64, tile=(0, 0, 0),
-                 waves_per_tile: int = 0, first_sweep_fp64: bool = False):
+                 waves_per_tile: int = 0, first_sweep_fp64: bool = False,
+                 dinv: str = "default"):
         problem.validate()
         self.problem = p = problem.contiguous()
         self.lib = _lib.load()
@@ -52,6 +56,8 @@ class Solver:
         o.tile[0], o.tile[1], o.tile[2] = tile
         o.waves_per_tile = waves_per_tile
         o.first_sweep_fp64 = int(bool(first_sweep_fp64))
+        o.dinv_mode = {"default": _lib.DINV_DEFAULT, "stream": _lib.DINV_STREAM,
+                       "recompute": _lib.DINV_RECOMPUTE}[dinv]
         self.precision = precision
         h = ctypes.c_void_p()
         _lib.check(self.lib.snap_hip_solver_create(ctypes.byref(s), ctypes.byref(o), ctypes.byref(h)),
