@@ -1,0 +1,176 @@
+"""Benchmark: mini-KBA sweep grind time and solve time on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload (one "step" = one full steady-state solve, outer/inner iterations to
+convergence): 64^3 cells on the unit cube, 32 groups, 36 angles per octant
+(288 directions), P3 scattering (16 flux moments), two materials with the
+unit source in the central cube, vacuum boundaries, epsi 1e-4.  Inputs are
+resident in HBM before the timed region.
+
+Prints one JSON line.  `value` is the FP64 sweep grind time
+t_sweep / (cells * 8*nang * ng * inner_iterations) in ns (lower is better).
+`roofline` prices the sweep kernel's algorithmic bytes per unit of the
+reference data flow, B64 = 8 + (24*cmom + 8)/nang, against 8 TB/s, with the
+kernel's mean launch duration from HIP events around each launch.
+`cpu_baseline` times the CPU oracle (a restatement of the reference's CPU
+sweep) over one inner iteration's sweeps of the same problem.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "legion-snap_amd"))
+sys.path.insert(0, REPO)
+
+METRIC = "sweep grind time ns/(cell·angle·group) + solve s, FP64 vs mixed @1e-6 rel"
+HBM_PEAK_GBS = 8000.0
+
+
+def bytes_per_unit(nang: int, cmom: int, real_bytes: int) -> float:
+    """Algorithmic HBM bytes per cell x direction x group of the reference
+    data flow: dinv streamed once per octant; source moments, t_xs and the
+    flux-moment read-modify-write amortised over the octant's angles."""
+    return real_bytes + (24.0 * cmom + 8.0) / nang
+
+
+def cpu_baseline(problem, budget_s: float = 25.0):
+    """Time the CPU oracle on one inner iteration's sweeps (all octants and
+    groups) of `problem` using every host core, or of a reduced group count
+    when the full sweep would exceed the budget (the grind time is per unit)."""
+    from oracle import oracle
+    import dataclasses
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores) or cores)))
+    # probe on 2 groups, then size the sample to the budget
+    probe = dataclasses.replace(problem, ng=2, sigt=problem.sigt[:, :2], siga=problem.siga[:, :2],
+                                sigs=problem.sigs[:, :2], slgg=problem.slgg[:2, :, :2, :],
+                                qi=problem.qi[:2])
+    t = oracle.time_sweeps(probe, 1, cores)
+    per_group = t / 2.0
+    ng = int(max(2, min(problem.ng, budget_s / max(per_group, 1e-9))))
+    sample = dataclasses.replace(problem, ng=ng, sigt=problem.sigt[:, :ng], siga=problem.siga[:, :ng],
+                                 sigs=problem.sigs[:, :ng], slgg=problem.slgg[:ng, :, :ng, :],
+                                 qi=problem.qi[:ng])
+    t = oracle.time_sweeps(sample, 1, cores)
+    units = sample.ncells * 8 * sample.nang * ng
+    return {
+        "value": t * 1e9 / units,
+        "unit": "ns/(cell·angle·group)",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"one inner iteration's sweeps (8 octants x {ng} of {problem.ng} groups) of the "
+                   f"{problem.nx}^3-cell, {problem.nang}-angle, P{problem.nmom - 1} workload, "
+                   f"{units:.3g} cell-angle-groups in {t:.1f} s"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--size", type=int, default=64, help="grid edge (default: the 64^3 workload)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    from snap_amd import problems
+    from snap_amd.solver import Solver
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+    p = problems.config1(n=args.size)
+    units = p.units_per_sweep
+    details = {}
+
+    # ---- FP64 (fused multiply-adds, dinv streamed: the plain drop-in data flow)
+    with Solver(p, precision="fp64") as s:
+        for _ in range(args.warmup):
+            s.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = [s.run() for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        f64, fm64 = s.flux()
+        ts = s.time_sweeps(3)
+    inner = stats[-1]["inner_total"]
+    sweep_ms = float(np.mean([st["sweep_ms"] for st in stats]))
+    solve_ms = float(np.mean([st["solve_ms"] for st in stats]))
+    grind = sweep_ms * 1e6 / (units * inner)
+    b64 = bytes_per_unit(p.nang, p.cmom, 8)
+    launches = ts["launches_per_sweep"]
+    achieved = (units / launches) * b64 / (ts["ms_per_launch"] * 1e-3) / 1e9
+    details["fp64"] = {"grind_ns": grind, "sweep_ms_per_inner": sweep_ms / inner,
+                       "solve_s": solve_ms / 1e3, "inner": inner, "outer": stats[-1]["outer_total"],
+                       "wall_s_per_step": wall / args.steps, "launch_ms": ts["ms_per_launch"],
+                       "launches_per_sweep": launches, "bytes_per_unit": b64}
+
+    # ---- FP64 in the reference's operation order (bit-equal to the oracle)
+    with Solver(p, precision="fp64_exact") as s:
+        st = s.run()
+        fex, _ = s.flux()
+    details["fp64_exact"] = {"grind_ns": st["sweep_ms"] * 1e6 / (units * st["inner_total"]),
+                             "solve_s": st["solve_ms"] / 1e3, "inner": st["inner_total"],
+                             "maxrel_vs_fp64_fused": float(np.max(np.abs(fex - f64) / np.abs(fex)))}
+
+    # ---- mixed precision (FP32 sweep, FP64 moments / sources / tests)
+    with Solver(p, precision="mixed") as s:
+        for _ in range(args.warmup):
+            s.run()
+        st = s.run()
+        fmx, _ = s.flux()
+    d = np.abs(fmx - fex)
+    details["mixed"] = {"grind_ns": st["sweep_ms"] * 1e6 / (units * st["inner_total"]),
+                        "solve_s": st["solve_ms"] / 1e3, "inner": st["inner_total"],
+                        "outer": st["outer_total"],
+                        "maxrel_vs_fp64": float(np.max(d / np.abs(fex))),
+                        "l2rel_vs_fp64": float(np.linalg.norm(d) / np.linalg.norm(fex)),
+                        "speedup_vs_fp64_grind": grind / (st["sweep_ms"] * 1e6 / (units * st["inner_total"]))}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(p)
+        details["fp64_speedup_vs_cpu"] = cpu["value"] / grind
+
+    line = {
+        "metric": METRIC,
+        "value": grind,
+        "unit": "ns/(cell·angle·group)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": solve_ms,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": (f"config 1: {p.nx}^3 cells, {p.ng} groups, {p.nang} angles/octant, "
+                                f"P{p.nmom - 1} ({p.cmom} moments), two materials, FP64 solve to "
+                                f"convergence (one step = one solve)"),
+                   "cells": p.ncells, "groups": p.ng, "angles_per_octant": p.nang, "nmom": p.nmom},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+        "details": details,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,67 @@
+"""The C ABI library: built, loadable, exports every entry point declared in
+include/snap_hip.h, and rejects invalid arguments before touching a GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from snap_amd import _lib
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "snap_hip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(snap_hip_\w+)\s*\(", text)))
+
+
+def test_library_loads():
+    lib = _lib.load()
+    assert lib.snap_hip_version().decode().startswith("snap_hip")
+
+
+def test_every_declared_symbol_is_exported():
+    lib = _lib.load()
+    syms = declared_symbols()
+    assert len(syms) >= 18
+    for name in syms:
+        assert hasattr(lib, name), name
+    assert set(syms) == set(_lib.SIGNATURES), "ctypes table out of sync with the header"
+
+
+def test_struct_sizes_match_header_layout():
+    # natural alignment of the ABI structs (x86-64 SysV)
+    assert ctypes.sizeof(_lib.Accessor3) == 32
+    assert ctypes.sizeof(_lib.Accessor2) == 24
+    assert ctypes.sizeof(_lib.Rect3) == 48
+
+
+def test_solver_rejects_null_problem():
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    rc = lib.snap_hip_solver_create(None, None, ctypes.byref(h))
+    assert rc == _lib.SNAP_ERR_INVALID
+    assert b"NULL" in lib.snap_hip_last_error()
+
+
+def test_sweep_rejects_unsupported_modes():
+    lib = _lib.load()
+    a = _lib.SweepArgs()
+    a.mms_source = 1
+    assert lib.snap_hip_run_gpu_sweep(ctypes.byref(a), None) == _lib.SNAP_ERR_UNSUPPORTED
+    a.mms_source = 0
+    a.vdelt = 0.5
+    assert lib.snap_hip_run_gpu_sweep(ctypes.byref(a), None) == _lib.SNAP_ERR_UNSUPPORTED
+    a.vdelt = 0.0
+    a.corner = 3
+    a.stride_x_positive = 0  # corner 3 sweeps +x
+    assert lib.snap_hip_run_gpu_sweep(ctypes.byref(a), None) == _lib.SNAP_ERR_INVALID
+
+
+def test_check_raises_with_message():
+    with pytest.raises(_lib.SnapError) as e:
+        _lib.check(_lib.SNAP_ERR_INVALID, "x")
+    assert "x failed" in str(e.value)

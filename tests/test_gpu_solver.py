@@ -1,0 +1,124 @@
+"""GPU solve (libsnap_hip's Legion-free transport_solve) against the CPU oracle.
+
+Bars, from the requirement on this path:
+* FP64 in the reference's operation order ("fp64_exact"): bit-equal scalar
+  flux and flux moments, identical inner/outer counts;
+* FP64 with fused multiply-adds ("fp64"): <= 1e-12 relative, identical counts;
+* mixed precision (FP32 sweep, FP64 moments/sources/tests): max relative
+  error <= 5e-6 and L2 relative error <= 1e-6 against the FP64 oracle,
+  identical counts on these cases (the measured errors are printed).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from snap_amd import problems
+from snap_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+CASES = {
+    "config0": lambda: problems.config0(),
+    "config1_small": lambda: problems.config1(n=16, ng=8, nang=12, nmom=4),
+    "config1_ragged": lambda: problems.config1(n=20, ng=6, nang=36, nmom=2),
+    "config3_small": lambda: problems.config3(n=16, ng=4, nang=24),
+    "p2_nang10": lambda: problems.config1(n=24, ng=4, nang=10, nmom=3),
+    "odd_shape": lambda: problems.config1(n=9, ng=3, nang=7, nmom=1),
+}
+
+_oracle_cache = {}
+
+
+def oracle_solution(oracle_mod, name):
+    if name not in _oracle_cache:
+        _oracle_cache[name] = oracle_mod.solve(CASES[name]())
+    return _oracle_cache[name]
+
+
+def gpu_solve(p, precision, **kw):
+    with Solver(p, precision=precision, **kw) as s:
+        st = s.run()
+        f, fm = s.flux()
+    return f, fm, st
+
+
+def rel_errors(a, b):
+    d = np.abs(a - b)
+    return float(np.max(d / np.maximum(np.abs(b), 1e-300))), float(np.linalg.norm(d) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fp64_exact_is_bit_equal(gpu, oracle_mod, name):
+    fo, fmo, sto = oracle_solution(oracle_mod, name)
+    f, fm, st = gpu_solve(CASES[name](), "fp64_exact")
+    print(name, "fp64_exact", st)
+    assert (st["inner_total"], st["outer_total"], st["converged"]) == \
+        (sto["inner_total"], sto["outer_total"], sto["converged"])
+    assert np.array_equal(f, fo)
+    if fmo is not None:
+        assert np.array_equal(fm, fmo)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fp64_fused_within_1e12(gpu, oracle_mod, name):
+    fo, fmo, sto = oracle_solution(oracle_mod, name)
+    f, fm, st = gpu_solve(CASES[name](), "fp64")
+    mx, l2 = rel_errors(f, fo)
+    print(name, "fp64", f"maxrel={mx:.3e} l2rel={l2:.3e}")
+    assert (st["inner_total"], st["outer_total"]) == (sto["inner_total"], sto["outer_total"])
+    assert mx <= 1e-12
+    if fmo is not None:
+        assert np.max(np.abs(fm - fmo)) <= 1e-12 * np.max(np.abs(fmo))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_mixed_precision_error(gpu, oracle_mod, name):
+    fo, fmo, sto = oracle_solution(oracle_mod, name)
+    f, fm, st = gpu_solve(CASES[name](), "mixed")
+    mx, l2 = rel_errors(f, fo)
+    print(name, "mixed", f"maxrel={mx:.3e} l2rel={l2:.3e}", st)
+    assert st["sweeps_fp32"] == st["inner_total"]
+    assert (st["inner_total"], st["outer_total"]) == (sto["inner_total"], sto["outer_total"])
+    assert mx <= 5e-6 and l2 <= 1e-6
+
+
+def test_dinv_modes_agree(gpu):
+    # forming dinv in the kernel is the same arithmetic as the geometry kernel
+    p = CASES["config1_small"]()
+    f1, _, _ = gpu_solve(p, "fp64_exact", dinv="stream")
+    f2, _, _ = gpu_solve(p, "fp64_exact", dinv="recompute")
+    assert np.array_equal(f1, f2)
+
+
+@pytest.mark.parametrize("tile,waves", [((4, 4, 4), 4), ((8, 4, 2), 2), ((5, 3, 7), 4),
+                                        ((16, 16, 1), 8)])
+def test_tile_shapes_do_not_change_results(gpu, oracle_mod, tile, waves):
+    fo, _, _ = oracle_solution(oracle_mod, "config1_ragged")
+    f, _, _ = gpu_solve(CASES["config1_ragged"](), "fp64_exact", tile=tile, waves_per_tile=waves)
+    assert np.array_equal(f, fo)
+
+
+def test_golden_fixtures(gpu):
+    g = np.load(os.path.join(GOLDEN, "config0.npz"), allow_pickle=False)
+    f, _, st = gpu_solve(problems.config0(), "fp64_exact")
+    assert np.array_equal(f, g["flux0"]) and st["inner_total"] == int(g["inner_total"])
+    g = np.load(os.path.join(GOLDEN, "config1_small.npz"), allow_pickle=False)
+    f, fm, st = gpu_solve(CASES["config1_small"](), "fp64_exact")
+    idx = g["cells"]
+    assert np.array_equal(f.reshape(f.shape[0], -1)[:, idx], g["flux0_samples"])
+    assert np.array_equal(fm.reshape(fm.shape[0], -1, fm.shape[-1])[:, idx, :], g["fluxm_samples"])
+
+
+def test_solver_rejects_bad_input(gpu):
+    from snap_amd import _lib
+    p = problems.config0(n=4, ng=2, nang=4)
+    p.mat[0, 0, 0] = 5
+    with pytest.raises(ValueError):
+        Solver(p)
+    p = problems.config0(n=4, ng=2, nang=80)  # more angles than the tiled sweep handles
+    with pytest.raises(_lib.SnapError) as e:
+        Solver(p)
+    assert e.value.code == _lib.SNAP_ERR_UNSUPPORTED

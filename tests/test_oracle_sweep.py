@@ -1,0 +1,155 @@
+"""The oracle's mini-KBA sweep (sweep.cc:242-616) pinned without retyping it:
+
+* one cell under vacuum inflow: pc = q * dinv;
+* a pencil along x with a given inflow: the diamond-difference recursion
+  psi_out = r psi_in + s has the geometric-series closed form;
+* a dense linear solve of the whole diamond-difference system (cell balance
+  plus the closure psi_out = 2 pc - psi_in on every face), independent of
+  the sweep order;
+* exact discrete particle balance over the chunk, with and without fixup;
+* the fixup keeps every outflow non-negative where the plain sweep does not.
+"""
+import numpy as np
+import pytest
+
+
+def _setup(oracle_mod, nx, ny, nz, nang, lx=1.0, ly=1.0, lz=1.0, seed=1, sig=(0.5, 2.0),
+           q=(0.2, 1.0)):
+    rng = np.random.default_rng(seed)
+    quad = oracle_mod.quadrature(nang, 1)
+    mu, eta, xi, w, ec = quad
+    h = (2.0 / (lx / nx), 2.0 / (ly / ny), 2.0 / (lz / nz))
+    txs = rng.uniform(*sig, size=(nz, ny, nx))
+    qq = rng.uniform(*q, size=(nz, ny, nx))
+    dinv = oracle_mod.geometry_param(txs[None], nang, h, mu, eta, xi)[0]
+    return quad, h, np.ascontiguousarray(txs), np.ascontiguousarray(qq[..., None]), dinv
+
+
+def _dense_dd(txs, q, h, mu, eta, xi, corner):
+    """Solve the diamond-difference equations of one angle as one linear system."""
+    nz, ny, nx = txs.shape
+    sx, sy, sz = (corner & 1) != 0, (corner & 2) != 0, (corner & 4) != 0
+    ncell = nx * ny * nz
+    idx = lambda x, y, z: (z * ny + y) * nx + x  # noqa: E731
+    n = 4 * ncell
+    A = np.zeros((n, n))
+    b = np.zeros(n)
+    D = txs + mu * h[0] + eta * h[1] + xi * h[2]
+    coef = (mu * h[0], eta * h[1], xi * h[2])
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                c = idx(x, y, z)
+                r = 4 * c
+                A[r, r] = D[z, y, x]
+                b[r] = q[z, y, x, 0]
+                ups = [(x - 1 if sx else x + 1, y, z), (x, y - 1 if sy else y + 1, z),
+                       (x, y, z - 1 if sz else z + 1)]
+                for ax in range(3):
+                    ux, uy, uz = ups[ax]
+                    row = r + 1 + ax
+                    A[row, row] = 1.0          # outflow
+                    A[row, r] = -2.0           # -2 pc
+                    if 0 <= ux < nx and 0 <= uy < ny and 0 <= uz < nz:
+                        u = 4 * idx(ux, uy, uz) + 1 + ax
+                        A[row, u] += 1.0       # + inflow
+                        A[r, u] -= coef[ax]    # balance: - mu h inflow
+    sol = np.linalg.solve(A, b)
+    return sol[0::4].reshape(nz, ny, nx)
+
+
+@pytest.mark.parametrize("corner", range(8))
+def test_single_cell_vacuum(oracle_mod, corner):
+    quad, h, txs, q, dinv = _setup(oracle_mod, 1, 1, 1, 3)
+    pf, _, nfix, _ = oracle_mod.sweep(1, 1, 1, 3, 1, corner, h, quad, q, dinv, txs, fixup=True)
+    w = quad[3]
+    assert nfix == 0
+    assert np.allclose(pf[0, 0, 0], np.sum(w * q[0, 0, 0, 0] * dinv[0, 0, 0]), rtol=1e-15)
+
+
+@pytest.mark.parametrize("corner", [0, 1, 6, 7])
+def test_pencil_closed_form(oracle_mod, corner):
+    N = 8
+    quad, h, txs, q, dinv = _setup(oracle_mod, N, 1, 1, 1, lx=1.0, ly=100.0, lz=100.0)
+    mu, eta, xi, w, ec = quad
+    sig = 1.0
+    txs[:] = sig
+    q[:] = 0.3
+    dinv = oracle_mod.geometry_param(txs[None], 1, h, mu, eta, xi)[0]
+    psi0 = 2.0
+    gx = np.full((1, 1, 1), psi0)
+    pf, _, nfix, outs = oracle_mod.sweep(N, 1, 1, 1, 1, corner, h, quad, q, dinv, txs, fixup=True,
+                                         ghost_in=(gx, None, None), want_ghost_out=True)
+    Dv = sig + mu[0] * h[0] + eta[0] * h[1] + xi[0] * h[2]
+    r = 2.0 * mu[0] * h[0] / Dv - 1.0
+    s = 2.0 * 0.3 / Dv
+    assert r > 0 and nfix == 0
+    k = np.arange(N)
+    psi_in = r ** k * psi0 + s * (1.0 - r ** k) / (1.0 - r)
+    pc = (0.3 + mu[0] * h[0] * psi_in) / Dv
+    order = k if (corner & 1) else k[::-1]
+    got = pf[0, 0, :][order] / w[0]
+    assert np.allclose(got, pc, rtol=1e-13, atol=0)
+    psi_out = r ** N * psi0 + s * (1.0 - r ** N) / (1.0 - r)
+    assert np.allclose(outs[0][0, 0, 0], psi_out, rtol=1e-13)
+
+
+@pytest.mark.parametrize("corner", [0, 3, 5, 7])
+def test_dense_linear_solve(oracle_mod, corner):
+    nx, ny, nz, nang = 3, 2, 2, 2
+    quad, h, txs, q, dinv = _setup(oracle_mod, nx, ny, nz, nang, seed=corner + 10)
+    mu, eta, xi, w, ec = quad
+    pf, _, _, _ = oracle_mod.sweep(nx, ny, nz, nang, 1, corner, h, quad, q, dinv, txs, fixup=False)
+    ref = np.zeros_like(pf)
+    for a in range(nang):
+        ref += w[a] * _dense_dd(txs, q, h, mu[a], eta[a], xi[a], corner)
+    assert np.allclose(pf, ref, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("fixup", [False, True])
+@pytest.mark.parametrize("corner", [0, 7])
+def test_discrete_balance(oracle_mod, fixup, corner):
+    nx, ny, nz, nang = 6, 5, 4, 5
+    quad, h, txs, q, dinv = _setup(oracle_mod, nx, ny, nz, nang, seed=4 + corner)
+    mu, eta, xi, w, ec = quad
+    pf, _, _, outs = oracle_mod.sweep(nx, ny, nz, nang, 1, corner, h, quad, q, dinv, txs,
+                                      fixup=fixup, want_ghost_out=True)
+    lhs = w.sum() * q.sum() - np.sum(txs * pf)
+    rhs = (np.sum(outs[0].sum(axis=(0, 1)) * w * mu * h[0] / 2.0)
+           + np.sum(outs[1].sum(axis=(0, 1)) * w * eta * h[1] / 2.0)
+           + np.sum(outs[2].sum(axis=(0, 1)) * w * xi * h[2] / 2.0))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_fixup_keeps_outflows_nonnegative(oracle_mod):
+    # optically thick cells next to a vacuum boundary drive the plain
+    # diamond difference negative; the set-to-zero fixup clamps every outflow
+    nx = ny = nz = 4
+    quad, h, txs, q, dinv = _setup(oracle_mod, nx, ny, nz, 6, lx=40.0, ly=40.0, lz=40.0,
+                                   sig=(5.0, 5.0), q=(0.0, 1.0), seed=9)
+    q[1:, :, :, 0] = 0.0
+    _, _, _, plain = oracle_mod.sweep(nx, ny, nz, 6, 1, 7, h, quad, q, dinv, txs, fixup=False,
+                                      want_ghost_out=True)
+    pf, _, nfix, fixed = oracle_mod.sweep(nx, ny, nz, 6, 1, 7, h, quad, q, dinv, txs, fixup=True,
+                                          want_ghost_out=True)
+    assert min(o.min() for o in plain) < 0.0
+    assert nfix > 0
+    assert min(o.min() for o in fixed) >= 0.0
+    assert pf.min() >= 0.0
+
+
+def test_moments_are_weighted_angular_sums(oracle_mod):
+    # fluxm[l-1] = sum_a ec[l][a] w[a] pc[a]; with a source that has no
+    # higher moments and one angle per octant, moment l is ec[l] * phi0
+    nx, ny, nz, nang, nmom = 3, 3, 3, 1, 2
+    quad = oracle_mod.quadrature(nang, nmom)
+    mu, eta, xi, w, ec = quad
+    h = (6.0, 6.0, 6.0)
+    txs = np.full((nz, ny, nx), 1.0)
+    dinv = oracle_mod.geometry_param(txs[None], nang, h, mu, eta, xi)[0]
+    qt = np.zeros((nz, ny, nx, 4))
+    qt[..., 0] = 1.0
+    for corner in range(8):
+        pf, pm, _, _ = oracle_mod.sweep(nx, ny, nz, nang, 4, corner, h, quad, qt, dinv, txs)
+        for l in range(1, 4):
+            assert np.allclose(pm[..., l - 1], ec[corner, l, 0] * pf, rtol=1e-14, atol=1e-17)
