@@ -32,6 +32,17 @@ sys.path.insert(0, REPO)
 
 METRIC = "sweep grind time ns/(cell·angle·group) + solve s, FP64 vs mixed @1e-6 rel"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "sweep_config1_fp64_traffic.json")
+
+
+def measured_traffic(size: int):
+    """HBM bytes per launch of the FP64 sweep kernel from the committed
+    FETCH_SIZE / WRITE_SIZE profile of this workload (calibrated against a
+    plain copy), or None when no profile covers it."""
+    if size != 64 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        return float(json.load(f)["bytes_per_launch"])
 
 
 def bytes_per_unit(nang: int, cmom: int, real_bytes: int) -> float:
@@ -79,6 +90,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--size", type=int, default=64, help="grid edge (default: the 64^3 workload)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one 1 GiB device-to-device copy and exit (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -92,6 +106,15 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         return 2
+    if args.copy_probe:
+        n = 1 << 27  # 1 GiB of doubles
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8}))
+        return 0
     p = problems.config1(n=args.size)
     units = p.units_per_sweep
     details = {}
@@ -164,7 +187,7 @@ def main() -> int:
                                 f"convergence (one step = one solve)"),
                    "cells": p.ncells, "groups": p.ng, "angles_per_octant": p.nang, "nmom": p.nmom},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.size)},
         "cpu_baseline": cpu,
         "details": details,
     }
