@@ -230,6 +230,7 @@ typedef struct {
   int waves_per_tile;       /* waves per tile workgroup; 0 = choose */
   int first_sweep_fp64;     /* MIXED: run the first inner's sweep in FP64 */
   int dinv_mode;            /* SNAP_DINV_DEFAULT, _STREAM or _RECOMPUTE */
+  int angles_per_lane;      /* sweep angles per GPU lane, 1 or 2; 0 = choose */
 } snap_solve_options_t;
 
 typedef struct {
@@ -256,12 +257,16 @@ int snap_hip_solver_run(snap_solver_t *solver, snap_solve_stats_t *stats);
  * (may be NULL) to host memory. */
 int snap_hip_solver_get_flux(snap_solver_t *solver, double *flux0, double *fluxm);
 /* Time `count` inner sweeps (all octants and groups) in the given precision
- * on the source left by the last run.  Writes the mean GPU time of one inner
+ * on the source left by the last run, after one untimed sweep.  Writes the mean GPU time of one inner
  * sweep and the mean duration of one launch of the sweep kernel.  MIXED needs
  * a solver created with precision MIXED. */
 int snap_hip_solver_time_sweeps(snap_solver_t *solver, int count, int precision,
                                 double *ms_per_sweep, double *ms_per_launch,
                                 int *launches_per_sweep);
+/* The sweep plan the solver chose: plan[0..2] tile edges (x,y,z), plan[3]
+ * waves per tile workgroup, plan[4] angles per lane, plan[5] cells per
+ * wavefront. */
+int snap_hip_solver_plan(snap_solver_t *solver, int32_t plan[6]);
 void snap_hip_solver_destroy(snap_solver_t *solver);
 
 #ifdef __cplusplus

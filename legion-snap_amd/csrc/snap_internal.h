@@ -138,6 +138,8 @@ struct SweepPlan {
   int TX = 0, TY = 0, TZ = 0;
   int nTx = 0, nTy = 0, nTz = 0, ntiles = 0;
   int nang = 0, cmom = 0, waves = 0;
+  int apl = 1;                 // angles per lane (1 or 2)
+  int lpc = 0;                 // lanes per cell, ceil(nang / apl)
   int cpw = 0;                 // cells per 64-lane wavefront
   int sb = 0;                  // cells per per-wave moment batch
   int maxdiag = 0;             // most cells on one cell diagonal of a tile
@@ -150,9 +152,14 @@ struct SweepPlan {
   size_t lds_bytes(int real_bytes, bool recompute_dinv) const;
   size_t face_doubles_per_sweep() const;  // x + y + z faces, all tiles
 };
-int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int TZ, int waves);
+// apl = 0 picks the angles-per-lane count that fills the wavefront best.
+int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int TZ, int waves,
+                int apl = 0);
 void plan_free(SweepPlan *p);
-void plan_default_tile(Extent3 e, int nang, int cmom, int *TX, int *TY, int *TZ, int *waves);
+// Fill in the tile edges, waves per workgroup and angles per lane left at 0
+// for a sweep whose faces are real_bytes wide (8 FP64, 4 FP32).
+void plan_choose(Extent3 e, int nang, int cmom, int real_bytes, int *TX, int *TY, int *TZ,
+                 int *waves, int *apl);
 
 struct SweepLaunch {
   const SweepPlan *plan;

@@ -192,15 +192,10 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
   SNAP_TRY(quad_compute(p->nang, p->nmom, mu, eta, xi, w, ec));
   SNAP_TRY(quad_upload(&s->quad, ec.data(), mu.data(), eta.data(), xi.data(), w.data(), p->nang, p->nmom));
   int TX = s->opt.tile[0], TY = s->opt.tile[1], TZ = s->opt.tile[2], waves = s->opt.waves_per_tile;
-  {
-    int dx, dy, dz, dw;
-    plan_default_tile(s->e, s->nang, s->cmom, &dx, &dy, &dz, &dw);
-    if (TX <= 0) TX = dx;
-    if (TY <= 0) TY = dy;
-    if (TZ <= 0) TZ = dz;
-    if (waves <= 0) waves = dw;
-  }
-  SNAP_TRY(plan_create(&s->plan, s->e, s->nang, s->cmom, TX, TY, TZ, waves));
+  int apl = s->opt.angles_per_lane;
+  plan_choose(s->e, s->nang, s->cmom, s->opt.precision == SNAP_PRECISION_MIXED ? 4 : 8, &TX, &TY,
+              &TZ, &waves, &apl);
+  SNAP_TRY(plan_create(&s->plan, s->e, s->nang, s->cmom, TX, TY, TZ, waves, apl));
 
   SNAP_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   for (auto &x : s->ev) SNAP_HIP_TRY(hipEventCreate(&x));
@@ -350,6 +345,7 @@ extern "C" int snap_hip_solver_time_sweeps(snap_solver_t *s, int count, int prec
   const int np = s->plan.nplanes;
   std::vector<hipEvent_t> evs((size_t)2 * np * count);
   for (auto &x : evs) SNAP_HIP_TRY(hipEventCreate(&x));
+  SNAP_TRY(s->sweep(precision, nullptr));  // first launches load the code object
   SNAP_HIP_TRY(hipEventRecord(s->ev[0], s->stream));
   for (int i = 0; i < count; i++) SNAP_TRY(s->sweep(precision, evs.data() + (size_t)2 * np * i));
   SNAP_HIP_TRY(hipEventRecord(s->ev[1], s->stream));
@@ -366,6 +362,17 @@ extern "C" int snap_hip_solver_time_sweeps(snap_solver_t *s, int count, int prec
   if (ms_per_sweep) *ms_per_sweep = total / count;
   if (ms_per_launch) *ms_per_launch = launch_sum / (double)(np * count);
   if (launches_per_sweep) *launches_per_sweep = np;
+  return SNAP_OK;
+}
+
+extern "C" int snap_hip_solver_plan(snap_solver_t *s, int32_t plan[6]) {
+  if (!s || !plan) return fail(SNAP_ERR_INVALID, "null argument");
+  plan[0] = s->plan.TX;
+  plan[1] = s->plan.TY;
+  plan[2] = s->plan.TZ;
+  plan[3] = s->plan.waves;
+  plan[4] = s->plan.apl;
+  plan[5] = s->plan.cpw;
   return SNAP_OK;
 }
 

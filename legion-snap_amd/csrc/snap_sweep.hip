@@ -16,17 +16,19 @@
 // diagonal and overwrites it with its own outflow), with one workgroup barrier
 // per cell diagonal.  Tile faces go to HBM only at tile entry and exit.
 //
-// Lanes carry angles: a 64-lane wavefront holds CPW = 64/nang cells of the
-// current diagonal, lane = cell_slot*nang + angle.  The fixup's negative-flux
-// count is cell-global over angles (sweep.cc:437-505), so each pass counts with
-// three wave ballots masked to the cell's lane segment.  Flux moments are
-// reduced per cell through LDS in ascending angle order, as the CPU variant
-// sums them (sweep.cc:568-591).
+// Lanes carry angles: each lane holds APL (1 or 2) consecutive angles of one
+// cell, a cell spans LPC = ceil(nang/APL) lanes and a 64-lane wavefront holds
+// CPW = 64/LPC cells of the current diagonal.  The fixup's negative-flux count
+// is cell-global over angles (sweep.cc:437-505), so each pass counts with wave
+// ballots masked to the cell's lane segment.  Flux moments are reduced per
+// cell through LDS in ascending angle order, as the CPU variant sums them
+// (sweep.cc:568-591).
 //
-// The FP64 instantiation keeps the reference's operation order everywhere and
-// the library is built with -ffp-contract=off, so its results are the CPU
-// arithmetic bit for bit.  The FP32 instantiation (mixed precision) runs the
-// per-angle arithmetic and faces in float and the moment sums in double.
+// The EXACT FP64 instantiation keeps the reference's operation order
+// everywhere and the library is built with -ffp-contract=off, so its results
+// are the CPU arithmetic bit for bit.  The fused FP64 instantiation uses
+// multiply-adds; the FP32 one (mixed precision) runs the per-angle arithmetic
+// and faces in float and the moment sums in double.
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -45,28 +47,41 @@ struct SweepLds {
 
 __host__ __device__ inline size_t a16(size_t b) { return (b + 15) & ~size_t(15); }
 
-constexpr int kMaxSteps = 4;
+// Most cell groups (steps) one wavefront owns on a cell diagonal; the planner
+// picks the wave count so that every diagonal fits.
+__host__ __device__ constexpr int max_steps(int apl) { return apl == 1 ? 4 : 2; }
+// Workgroup size bound: two angles per lane need more than the 128 registers
+// a 16-wave workgroup leaves each lane, so those workgroups stop at 8 waves.
+__host__ __device__ constexpr int max_threads(int apl) { return apl == 1 ? 1024 : 512; }
 
-// faces [x|y|z] (Real) | ec of the corner [cmom][nang] and transposed
-// [nang][cmom] | per-wave w*pc batches [W][SB][nang] | their cell indices
-// [W][SB] | per-wave source moments [W][kMaxSteps][CPW][cmom] | the tile's
-// cell list [ncells] and diagonal offsets [ncp+1]
+// Row strides of the two ec tables: the per-angle table is read by lane
+// (angle) with the moment as the row, the per-moment table by (cell, moment)
+// lanes with the angle as the row; fixed strides keep both reads at
+// immediate offsets.
+__host__ __device__ constexpr int ec_row(int apl) { return 64 * apl; }
+constexpr int kEctRow = 16;  // >= cmom (P3 at most)
+
+// faces [x|y|z] (Real) | ec of the corner [cmom][ec_row] and [nang][kEctRow]
+// | per-wave w*pc batches [W][SB][nang] | per-wave source moments
+// [W][steps][CPW][cmom] | batch cell indices [W][SB] | per tile cell
+// {chunk cell, face offsets} [ncells] (int4) | diagonal offsets [ncp+1]
 __host__ __device__ inline SweepLds sweep_lds(int TX, int TY, int TZ, int nang, int cmom, int W,
-                                              int SB, int ncells_tile, int ncp, int cpw, int rb) {
+                                              int SB, int ncells_tile, int ncp, int cpw, int apl,
+                                              int rb) {
   SweepLds L;
   size_t o = a16((size_t)(TY * TZ + TX * TZ + TX * TY) * nang * rb);
   L.ecs = o;
-  o += (size_t)cmom * nang * 8;
+  o += (size_t)cmom * ec_row(apl) * 8;
   L.ect = o;
-  o += (size_t)cmom * nang * 8;
+  o += (size_t)nang * kEctRow * 8;
   L.pwb = o;
   o += (size_t)W * SB * nang * 8;
   L.qsl = o;
-  o += (size_t)W * kMaxSteps * cpw * cmom * 8;
+  o += (size_t)W * max_steps(apl) * cpw * cmom * 8;
   L.cix = o;
   o = a16(o + (size_t)W * SB * 4);
   L.cells = o;
-  o = a16(o + (size_t)ncells_tile * 2);
+  o = a16(o + (size_t)ncells_tile * 16);
   L.coff = o;
   o = a16(o + (size_t)(ncp + 1) * 4);
   L.total = o;
@@ -78,7 +93,8 @@ __host__ __device__ inline SweepLds sweep_lds(int TX, int TY, int TZ, int nang, 
 // ---------------------------------------------------------------------------
 size_t SweepPlan::lds_bytes(int real_bytes, bool recompute_dinv) const {
   (void)recompute_dinv;
-  return sweep_lds(TX, TY, TZ, nang, cmom, waves, sb, TX * TY * TZ, ncellplanes, cpw, real_bytes).total;
+  return sweep_lds(TX, TY, TZ, nang, cmom, waves, sb, TX * TY * TZ, ncellplanes, cpw, apl,
+                   real_bytes).total;
 }
 
 size_t SweepPlan::face_doubles_per_sweep() const {
@@ -87,23 +103,81 @@ size_t SweepPlan::face_doubles_per_sweep() const {
 
 static const int kCpwChoices[] = {1, 2, 3, 4, 5, 6, 8};
 
-static int pick_cpw(int nang) {
-  int best = 1;
+static int pick_cpw(int lpc) {
+  int best = 0;
   for (int c : kCpwChoices)
-    if (c * nang <= 64) best = c;
+    if (c * lpc <= 64) best = c;
   return best;
 }
 
-void plan_default_tile(Extent3 e, int nang, int cmom, int *TX, int *TY, int *TZ, int *waves) {
-  // 8x8x4 tiles: 18 cell diagonals of at most 32 cells, two workgroups per
-  // CU with FP64 faces and double-buffered staging at 36 angles.  Clamp to
-  // the chunk.
-  *TX = std::min(8, e.nx);
-  *TY = std::min(8, e.ny);
-  *TZ = std::min(4, e.nz);
-  *waves = 8;
-  (void)nang;
-  (void)cmom;
+// angles per lane: the choice that uses the most of the wavefront's
+// angle slots (ties keep one angle per lane)
+static int pick_apl(int nang) {
+  int best = 0;
+  double best_use = -1.0;
+  for (int apl = 1; apl <= 2; apl++) {
+    const int lpc = (nang + apl - 1) / apl;
+    const int cpw = pick_cpw(lpc);
+    if (cpw == 0) continue;
+    const double use = double(cpw * nang) / double(64 * apl);
+    if (use > best_use + 1e-9) {
+      best_use = use;
+      best = apl;
+    }
+  }
+  return best;
+}
+
+// most cells on one cell diagonal i+j+k = d of a TX x TY x TZ tile
+static int tile_maxdiag(int TX, int TY, int TZ) {
+  int best = 0;
+  for (int d = 0; d < TX + TY + TZ - 2; d++) {
+    int n = 0;
+    for (int k = 0; k < TZ; k++)
+      for (int j = 0; j < TY; j++) {
+        const int i = d - j - k;
+        if (i >= 0 && i < TX) n++;
+      }
+    best = std::max(best, n);
+  }
+  return best;
+}
+
+void plan_choose(Extent3 e, int nang, int cmom, int real_bytes, int *TX, int *TY, int *TZ,
+                 int *waves, int *apl) {
+  // angles per lane: one at FP64 (two would exceed 128 registers per lane
+  // and halve the resident waves), the better-filled wavefront at FP32, two
+  // whenever one cell's angles exceed a wavefront
+  if (*apl <= 0) *apl = (nang > 64) ? 2 : (real_bytes == 4 ? pick_apl(nang) : 1);
+  if (*apl < 1 || *apl > 2) return;  // plan_create reports it
+  const int lpc = (nang + *apl - 1) / *apl;
+  const int cpw = std::max(1, pick_cpw(lpc));
+  const int ms = max_steps(*apl), wmax = max_threads(*apl) / 64;
+  if (*TX <= 0 || *TY <= 0 || *TZ <= 0) {
+    // largest candidate tile whose biggest cell diagonal the workgroup
+    // covers and whose LDS fits; measured best first (8x8x4 at FP64, 8x8x8
+    // with FP32 faces and two angles per lane)
+    static const int cand[][3] = {{8, 8, 8}, {8, 8, 4}, {8, 4, 4}, {4, 4, 4}, {4, 4, 2},
+                                  {4, 2, 2}, {2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
+    const int first = (real_bytes == 4 && *apl == 2) ? 0 : 1;
+    for (int c = first; c < (int)(sizeof(cand) / sizeof(cand[0])); c++) {
+      const int tx = std::min(cand[c][0], e.nx), ty = std::min(cand[c][1], e.ny),
+                tz = std::min(cand[c][2], e.nz);
+      const int need = (tile_maxdiag(tx, ty, tz) + ms * cpw - 1) / (ms * cpw);
+      const int w = std::max(need, *waves > 0 ? *waves : (*apl == 1 ? 8 : 6));
+      if (w > wmax) continue;
+      const int sb = cpw * std::min(std::max(1, 64 / (cmom * cpw)), std::max(1, 8 / cpw));
+      const size_t lds = sweep_lds(tx, ty, tz, nang, cmom, w, sb, tx * ty * tz, tx + ty + tz - 2,
+                                   cpw, *apl, 8).total;
+      if (lds > 160 * 1024) continue;
+      *TX = tx;
+      *TY = ty;
+      *TZ = tz;
+      break;
+    }
+    if (*TX <= 0) *TX = *TY = *TZ = 1;
+  }
+  if (*waves <= 0) *waves = (*apl == 1) ? 8 : 6;  // plan_create raises it to what the tile needs
 }
 
 void plan_free(SweepPlan *p) {
@@ -113,16 +187,20 @@ void plan_free(SweepPlan *p) {
   *p = SweepPlan();
 }
 
-int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int TZ, int waves) {
+int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int TZ, int waves,
+                int apl) {
   plan_free(p);
   if (e.nx < 1 || e.ny < 1 || e.nz < 1) return fail(SNAP_ERR_INVALID, "empty chunk");
-  if (nang < 1 || nang > 64)
-    return fail(SNAP_ERR_UNSUPPORTED, "the tiled sweep handles 1..64 angles per octant, got %d",
+  if (nang < 1 || nang > 128)
+    return fail(SNAP_ERR_UNSUPPORTED, "the tiled sweep handles 1..128 angles per octant, got %d",
                 nang);
   if (cmom < 1 || cmom > 16) return fail(SNAP_ERR_INVALID, "cmom must be 1..16, got %d", cmom);
   if (TX < 1 || TY < 1 || TZ < 1 || TX > 32 || TY > 32 || TZ > 32)
     return fail(SNAP_ERR_INVALID, "tile edges must be 1..32");
-  if (waves < 1 || waves > 16) return fail(SNAP_ERR_INVALID, "waves per tile must be 1..16");
+  if (waves < 0 || waves > 16) return fail(SNAP_ERR_INVALID, "waves per tile must be 1..16");
+  if (apl == 0) apl = pick_apl(nang);
+  if (apl < 1 || apl > 2 || (nang + apl - 1) / apl > 64)
+    return fail(SNAP_ERR_INVALID, "%d angles per lane cannot hold %d angles", apl, nang);
   p->ext = e;
   p->TX = TX;
   p->TY = TY;
@@ -135,13 +213,18 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
   p->ntiles = p->nTx * p->nTy * p->nTz;
   p->nang = nang;
   p->cmom = cmom;
+  // default: 8 waves (two workgroups per CU at 128 registers per lane); 6
+  // with two angles per lane, where ~150 registers leave 12 waves per CU
+  if (waves == 0) waves = (apl == 1) ? 8 : 6;
   p->waves = waves;
-  p->cpw = pick_cpw(nang);
+  p->apl = apl;
+  p->lpc = (nang + apl - 1) / apl;
+  p->cpw = pick_cpw(p->lpc);
   // cells per moment batch: enough (cell, moment) sums to fill a wavefront
   {
-    int sbm = std::min(64 / cmom, 8);
-    if (sbm < 1) sbm = 1;
-    p->sb = std::max(p->cpw, (sbm / p->cpw) * p->cpw);
+    int per = 64 / (cmom * p->cpw);
+    if (per < 1) per = 1;
+    p->sb = p->cpw * std::min(per, std::max(1, 8 / p->cpw));
   }
   // tile diagonals in sweep order
   p->nplanes = p->nTx + p->nTy + p->nTz - 2;
@@ -172,12 +255,17 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
     p->maxdiag = std::max(p->maxdiag, (int)cells.size() - coff[d]);
   }
   coff[p->ncellplanes] = (int)cells.size();
-  // every wavefront owns at most kMaxSteps cell groups of a diagonal
-  const int need_waves = (p->maxdiag + kMaxSteps * p->cpw - 1) / (kMaxSteps * p->cpw);
+  // every wavefront owns at most max_steps cell groups of a diagonal
+  const int ms = max_steps(apl);
+  const int need_waves = (p->maxdiag + ms * p->cpw - 1) / (ms * p->cpw);
   if (p->waves < need_waves) p->waves = need_waves;
-  if (p->waves > 16)
-    return fail(SNAP_ERR_INVALID, "tile %dx%dx%d needs %d waves per workgroup (> 16)", TX, TY, TZ,
-                p->waves);
+  if (p->waves * 64 > max_threads(apl)) {
+    if (waves * 64 <= max_threads(apl) && need_waves * 64 <= max_threads(apl))
+      p->waves = max_threads(apl) / 64;
+    else
+      return fail(SNAP_ERR_INVALID, "tile %dx%dx%d needs %d waves per workgroup (> %d)", TX, TY,
+                  TZ, std::max(waves, need_waves), max_threads(apl) / 64);
+  }
   if (p->lds_bytes(8, false) > 160 * 1024)
     return fail(SNAP_ERR_INVALID, "tile %dx%dx%d with %d angles needs %zu B of LDS (> 160 KiB)",
                 TX, TY, TZ, nang, p->lds_bytes(8, false));
@@ -202,7 +290,7 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
 struct SweepArgs {
   Extent3 ext;
   int TX, TY, TZ, nTx, nTy, nTz, ntiles;
-  int nang, cmom, ncellplanes, ncells_tile, sb;
+  int nang, cmom, ncellplanes, ncells_tile, sb, lpc;
   int ng_job, corner0, fixup, recompute_dinv;
   double hi, hj, hk;
   const double *mu, *eta, *xi, *w, *ec;
@@ -240,30 +328,61 @@ __device__ inline double madd(double a, double b, double c) {
 }
 __device__ inline float maddf(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// Number of set bits of the ballots b[] inside the lane's cell segment,
+// summed: the cell-global count of negative outflows.  With one cell per
+// wavefront only that cell's lanes can vote; otherwise each segment is
+// counted with scalar operations and the lane picks its own.
+template <int CPW, int NB>
+__device__ inline unsigned seg_count(const unsigned long long (&b)[NB], unsigned long long m0,
+                                     int LPC, int cl) {
+  if constexpr (CPW == 1) {
+    unsigned n = 0;
+#pragma unroll
+    for (int k = 0; k < NB; k++) n += (unsigned)__popcll(b[k]);
+    return n;
+  } else {
+    unsigned res = 0;
+#pragma unroll
+    for (int cs = 0; cs < CPW; cs++) {
+      const unsigned long long m = m0 << (cs * LPC);
+      unsigned n = 0;
+#pragma unroll
+      for (int k = 0; k < NB; k++) n += (unsigned)__popcll(b[k] & m);
+      res = (cl == cs) ? n : res;
+    }
+    return res;
+  }
+}
+
 // Tile sweep.  Real: arithmetic and face type.  CPW: cells per wavefront.
-// EXACT: the reference's operation order with separate roundings (bit-equal
-// to the CPU oracle); otherwise fused multiply-adds.  GEN: general accessors,
+// APL: angles per lane.  EXACT: the reference's operation order with separate
+// roundings (bit-equal to the CPU oracle); otherwise fused multiply-adds.
+// REC: form dinv from t_xs instead of reading it.  GEN: general accessors,
 // ghost faces and atomic accumulation (the drop-in entry point).
-template <typename Real, int CPW, bool EXACT, bool GEN>
-__global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs A, SweepGeneral G, int plane_off) {
+template <typename Real, int CPW, int APL, bool EXACT, bool REC, bool GEN>
+__global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, SweepGeneral G, int plane_off) {
   constexpr bool kFP64 = std::is_same<Real, double>::value;
+  constexpr int MS = max_steps(APL);  // cell groups per wave and diagonal
+  constexpr int QS = APL;             // source moments held per lane
+  constexpr int ECR = ec_row(APL);    // row stride of the ec table
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int TX = A.TX, TY = A.TY, TZ = A.TZ, nang = A.nang, cmom = A.cmom, SB = A.sb;
+  const int LPC = A.lpc;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
-  const bool recompute = A.recompute_dinv != 0;
   const int ncp = A.ncellplanes;
-  const SweepLds lay = sweep_lds(TX, TY, TZ, nang, cmom, W, SB, A.ncells_tile, ncp, CPW, (int)sizeof(Real));
+  const SweepLds lay =
+      sweep_lds(TX, TY, TZ, nang, cmom, W, SB, A.ncells_tile, ncp, CPW, APL, (int)sizeof(Real));
   const int fxn = TY * TZ * nang, fyn = TX * TZ * nang, fzn = TX * TY * nang;
   Real *fx = (Real *)smem_raw;
   Real *fy = fx + fxn;
   Real *fz = fy + fyn;
-  double *ecs = (double *)(smem_raw + lay.ecs);   // [cmom][nang]
-  double *ect = (double *)(smem_raw + lay.ect);   // [nang][cmom]
+  double *ecs = (double *)(smem_raw + lay.ecs);   // [cmom][ECR]: ec[l][a] at l*ECR + a
+  double *ect = (double *)(smem_raw + lay.ect);   // [nang][kEctRow]: ec[l][a] at a*kEctRow + l
   double *pwb = (double *)(smem_raw + lay.pwb) + wave * SB * nang;  // this wave's w*pc batch
   int *cix = (int *)(smem_raw + lay.cix) + wave * SB;               // batch cell indices
-  double *qsl = (double *)(smem_raw + lay.qsl) + wave * kMaxSteps * CPW * cmom;  // source moments
-  int16_t *tcells = (int16_t *)(smem_raw + lay.cells);              // cell list of the tile
-  int *toff = (int *)(smem_raw + lay.coff);                         // its diagonal offsets
+  double *qsl = (double *)(smem_raw + lay.qsl) + wave * MS * CPW * cmom;  // source moments
+  int4 *tcr = (int4 *)(smem_raw + lay.cells);  // per tile cell: chunk cell (or -1), face offsets
+  int *toff = (int *)(smem_raw + lay.coff);    // offsets of the cell diagonals in tcr
 
   const int s = blockIdx.y;
   const int corner = A.corner0 + s / A.ng_job;
@@ -285,114 +404,173 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs A, SweepGeneral 
   double *pfg = A.part_flux + ((size_t)corner * A.ng_job + g) * ncells;
   double *pmg = A.part_fluxm + ((size_t)corner * A.ng_job + g) * ncells * (cmom - 1);
 
-  // ---- per-lane constants: lane = cell slot * nang + angle
-  const int cl = (CPW == 1) ? 0 : lane / nang;
-  const int a = lane - cl * nang;
-  const bool lane_ok = (cl < CPW) && (a < nang);
-  const unsigned long long seg =
-      lane_ok ? (((nang >= 64) ? ~0ull : ((1ull << nang) - 1ull)) << (cl * nang)) : 0ull;
-  const int aa = lane_ok ? a : 0;
+  // ---- per-lane constants: lane = cell slot * LPC + la, angles la + r*LPC
+  const int cl = (CPW == 1) ? 0 : lane / LPC;
+  const int la = lane - cl * LPC;
+  const bool lane_ok = cl < CPW;
+  const unsigned long long seg0 = (LPC >= 64) ? ~0ull : ((1ull << LPC) - 1ull);
+  int an[APL];     // angle of slot r (clamped into range)
+  bool aok[APL];   // slot r holds a real angle of a real cell slot
+#pragma unroll
+  for (int r = 0; r < APL; r++) {
+    const int a = la + r * LPC;
+    aok[r] = lane_ok && a < nang;
+    an[r] = aok[r] ? a : 0;
+  }
   const int per_step = W * CPW;
 
   // ---- tables and inflow faces (from the upstream tile, the ghost array,
   // or vacuum) into LDS
-  for (int idx = threadIdx.x; idx < A.ncells_tile; idx += blockDim.x) tcells[idx] = A.cells[idx];
+  for (int idx = threadIdx.x; idx < A.ncells_tile; idx += blockDim.x) {
+    const int pk = A.cells[idx];
+    const int i = pk & 31, j = (pk >> 5) & 31, k = (pk >> 10) & 31;
+    int cell = -1;
+    if (i < ex && j < ey && k < ez) {
+      const int X = x0 + i, Y = y0 + j, Z = z0 + k;
+      const int px = sxp ? X : nx - 1 - X, py = syp ? Y : ny - 1 - Y, pz = szp ? Z : nz - 1 - Z;
+      cell = (pz * ny + py) * nx + px;
+    }
+    tcr[idx] = make_int4(cell, (k * TY + j) * nang, (k * TX + i) * nang, (j * TX + i) * nang);
+  }
   for (int idx = threadIdx.x; idx <= ncp; idx += blockDim.x) toff[idx] = A.cell_off[idx];
   for (int idx = threadIdx.x; idx < cmom * nang; idx += blockDim.x) {
     const double v = A.ec[(size_t)corner * cmom * nang + idx];
     const int l = idx / nang, b = idx - l * nang;
-    ecs[idx] = v;
-    ect[b * cmom + l] = v;
+    ecs[l * ECR + b] = v;
+    ect[b * kEctRow + l] = v;
   }
-  for (int idx = threadIdx.x; idx < fxn; idx += blockDim.x) {
-    const int a2 = idx % nang, slot = idx / nang, j = slot % TY, k = slot / TY;
-    Real v = Real(0);
-    if (j < ey && k < ez) {
-      if (tx > 0) {
-        v = faces_s[(size_t)(tile_id - 1) * per_tile + idx];
-      } else if (GEN && G.gx.base) {
-        const int py = syp ? y0 + j : ny - 1 - (y0 + j), pz = szp ? z0 + k : nz - 1 - (z0 + k);
-        v = (Real)((const double *)G.gx.at(py, pz, 0))[a2];
+  // Inflow slots of cells outside a ragged chunk edge are copied like the
+  // others; no cell reads them.
+  {
+    const Real *up = faces_s + (size_t)(tile_id - 1) * per_tile;
+    if (tx > 0) {
+      for (int idx = threadIdx.x; idx < fxn; idx += blockDim.x) fx[idx] = up[idx];
+    } else if (GEN && G.gx.base) {
+      for (int idx = threadIdx.x; idx < fxn; idx += blockDim.x) {
+        const int a2 = idx % nang, slot = idx / nang, j = slot % TY, k = slot / TY;
+        Real v = Real(0);
+        if (j < ey && k < ez) {
+          const int py = syp ? y0 + j : ny - 1 - (y0 + j), pz = szp ? z0 + k : nz - 1 - (z0 + k);
+          v = (Real)((const double *)G.gx.at(py, pz, 0))[a2];
+        }
+        fx[idx] = v;
       }
+    } else {
+      for (int idx = threadIdx.x; idx < fxn; idx += blockDim.x) fx[idx] = Real(0);
     }
-    fx[idx] = v;
   }
-  for (int idx = threadIdx.x; idx < fyn; idx += blockDim.x) {
-    const int a2 = idx % nang, slot = idx / nang, i = slot % TX, k = slot / TX;
-    Real v = Real(0);
-    if (i < ex && k < ez) {
-      if (ty > 0) {
-        v = faces_s[(size_t)(tile_id - A.nTx) * per_tile + fxn + idx];
-      } else if (GEN && G.gy.base) {
-        const int px = sxp ? x0 + i : nx - 1 - (x0 + i), pz = szp ? z0 + k : nz - 1 - (z0 + k);
-        v = (Real)((const double *)G.gy.at(px, pz, 0))[a2];
+  {
+    const Real *up = faces_s + (size_t)(tile_id - A.nTx) * per_tile + fxn;
+    if (ty > 0) {
+      for (int idx = threadIdx.x; idx < fyn; idx += blockDim.x) fy[idx] = up[idx];
+    } else if (GEN && G.gy.base) {
+      for (int idx = threadIdx.x; idx < fyn; idx += blockDim.x) {
+        const int a2 = idx % nang, slot = idx / nang, i = slot % TX, k = slot / TX;
+        Real v = Real(0);
+        if (i < ex && k < ez) {
+          const int px = sxp ? x0 + i : nx - 1 - (x0 + i), pz = szp ? z0 + k : nz - 1 - (z0 + k);
+          v = (Real)((const double *)G.gy.at(px, pz, 0))[a2];
+        }
+        fy[idx] = v;
       }
+    } else {
+      for (int idx = threadIdx.x; idx < fyn; idx += blockDim.x) fy[idx] = Real(0);
     }
-    fy[idx] = v;
   }
-  for (int idx = threadIdx.x; idx < fzn; idx += blockDim.x) {
-    const int a2 = idx % nang, slot = idx / nang, i = slot % TX, j = slot / TX;
-    Real v = Real(0);
-    if (i < ex && j < ey) {
-      if (tz > 0) {
-        v = faces_s[(size_t)(tile_id - A.nTx * A.nTy) * per_tile + fxn + fyn + idx];
-      } else if (GEN && G.gz.base) {
-        const int px = sxp ? x0 + i : nx - 1 - (x0 + i), py = syp ? y0 + j : ny - 1 - (y0 + j);
-        v = (Real)((const double *)G.gz.at(px, py, 0))[a2];
+  {
+    const Real *up = faces_s + (size_t)(tile_id - A.nTx * A.nTy) * per_tile + fxn + fyn;
+    if (tz > 0) {
+      for (int idx = threadIdx.x; idx < fzn; idx += blockDim.x) fz[idx] = up[idx];
+    } else if (GEN && G.gz.base) {
+      for (int idx = threadIdx.x; idx < fzn; idx += blockDim.x) {
+        const int a2 = idx % nang, slot = idx / nang, i = slot % TX, j = slot / TX;
+        Real v = Real(0);
+        if (i < ex && j < ey) {
+          const int px = sxp ? x0 + i : nx - 1 - (x0 + i), py = syp ? y0 + j : ny - 1 - (y0 + j);
+          v = (Real)((const double *)G.gz.at(px, py, 0))[a2];
+        }
+        fz[idx] = v;
       }
+    } else {
+      for (int idx = threadIdx.x; idx < fzn; idx += blockDim.x) fz[idx] = Real(0);
     }
-    fz[idx] = v;
   }
 
-  const double mu_d = A.mu[aa], eta_d = A.eta[aa], xi_d = A.xi[aa], w_d = A.w[aa];
-  // EXACT keeps (psi*mu)*hi as the reference writes it; the others fold mu*hi.
-  const Real mu_a = (Real)mu_d, eta_a = (Real)eta_d, xi_a = (Real)xi_d;
+  // per-angle constants.  EXACT keeps (psi*mu)*hi as the reference writes
+  // it (cx,cy,cz = mu,eta,xi); the others fold them (cx = mu*hi, ...).
+  // hi*mu is the same double either way, so the recomputed dinv is formed
+  // from these too.
+  Real cx[APL], cy[APL], cz[APL];
+  double w_d[APL];
+  float g_sum32[APL];
+#pragma unroll
+  for (int r = 0; r < APL; r++) {
+    const double mu_d = A.mu[an[r]], eta_d = A.eta[an[r]], xi_d = A.xi[an[r]];
+    w_d[r] = A.w[an[r]];
+    if constexpr (EXACT) {
+      cx[r] = (Real)mu_d;
+      cy[r] = (Real)eta_d;
+      cz[r] = (Real)xi_d;
+    } else {
+      cx[r] = (Real)(mu_d * A.hi);
+      cy[r] = (Real)(eta_d * A.hj);
+      cz[r] = (Real)(xi_d * A.hk);
+    }
+    g_sum32[r] = (float)((A.hi * mu_d + A.hj * eta_d) + A.hk * xi_d);
+  }
   const Real hi = (Real)A.hi, hj = (Real)A.hj, hk = (Real)A.hk;
-  const Real muh = (Real)(mu_d * A.hi), etah = (Real)(eta_d * A.hj), xih = (Real)(xi_d * A.hk);
-  // dinv recomputed as CalculateGeometryParam forms it (expxs.cc:378-381)
-  const double g_hm = A.hi * mu_d, g_he = A.hj * eta_d, g_hx = A.hk * xi_d;
-  const float g_sum32 = (float)(g_hm + g_he + g_hx);
   const Real tolr = (Real)1.0e-12;
   const int nm1 = cmom - 1;
-  const bool q_in_lanes = cmom <= nang;
+  // source moments travel in lanes (lane la of a cell's segment holds moments
+  // la + m*LPC) when they fit; otherwise the wave copies them to LDS itself
+  const bool q_in_lanes = cmom <= QS * LPC;
+  // moment sums: lane -> (batched cell fk, moment fl) for the first 64 sums
+  const int fk = lane / cmom, fl = lane - fk * cmom;
   __syncthreads();
 
-  // ---- register prefetch of this wave's cells on the next diagonal: dinv
-  // (lane a holds angle a), the source moments (lane l of the cell's segment
-  // holds moment l when cmom <= nang) and t_xs.
-  double n_dv[kMaxSteps], n_q[kMaxSteps], n_tx[kMaxSteps];
-  int n_cell[kMaxSteps];
-  auto cell_of = [&](int idx, int &i, int &j, int &k) -> bool {
-    const int pk = tcells[idx];
-    i = pk & 31;
-    j = (pk >> 5) & 31;
-    k = (pk >> 10) & 31;
-    return (i < ex) && (j < ey) && (k < ez);
+  // ---- register prefetch of this wave's cells on the next diagonal: dinv of
+  // the lane's angles, its source moments and t_xs.
+  double n_dv[MS][APL], n_q[MS][QS], n_tx[MS];
+  int n_cell[MS];
+  auto load_cell = [&](int cell, int st) {
+    if constexpr (GEN) {
+      const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
+      const Real *dv = (const Real *)G.dinv.at(px, py, pz, 0);
+      const double *qc = (const double *)G.qtot.at(px, py, pz, 0);
+#pragma unroll
+      for (int r = 0; r < APL; r++) n_dv[st][r] = (double)dv[an[r]];
+#pragma unroll
+      for (int m = 0; m < QS; m++)
+        if (q_in_lanes && la + m * LPC < cmom) n_q[st][m] = qc[la + m * LPC];
+      n_tx[st] = (double)*(const Real *)G.txs.at(px, py, pz, 0);
+    } else {
+      if constexpr (!REC) {
+        const Real *dv = dg + (size_t)cell * nang;
+#pragma unroll
+        for (int r = 0; r < APL; r++) n_dv[st][r] = (double)dv[an[r]];
+      }
+      const double *qc = qg + (size_t)cell * cmom;
+#pragma unroll
+      for (int m = 0; m < QS; m++)
+        if (q_in_lanes && la + m * LPC < cmom) n_q[st][m] = qc[la + m * LPC];
+      n_tx[st] = (double)tg[cell];
+    }
   };
   auto prefetch = [&](int dd) {
     const int c0 = toff[dd], c1 = toff[dd + 1];
 #pragma unroll
-    for (int st = 0; st < kMaxSteps; st++) {
+    for (int st = 0; st < MS; st++) {
       const int my = c0 + st * per_step + wave * CPW + cl;
-      int i, j, k;
-      n_dv[st] = 0.0;
-      n_q[st] = 0.0;
+#pragma unroll
+      for (int r = 0; r < APL; r++) n_dv[st][r] = 0.0;
+#pragma unroll
+      for (int m = 0; m < QS; m++) n_q[st][m] = 0.0;
       n_tx[st] = 0.0;
       n_cell[st] = -1;
-      if (lane_ok && my < c1 && cell_of(my, i, j, k)) {
-        const int X = x0 + i, Y = y0 + j, Z = z0 + k;
-        const int px = sxp ? X : nx - 1 - X, py = syp ? Y : ny - 1 - Y, pz = szp ? Z : nz - 1 - Z;
-        const int cell = (pz * ny + py) * nx + px;
+      if (lane_ok && my < c1) {
+        const int cell = tcr[my].x;
         n_cell[st] = cell;
-        if constexpr (GEN) {
-          if (!recompute) n_dv[st] = (double)((const Real *)G.dinv.at(px, py, pz, 0))[a];
-          if (q_in_lanes && a < cmom) n_q[st] = ((const double *)G.qtot.at(px, py, pz, 0))[a];
-          n_tx[st] = (double)*(const Real *)G.txs.at(px, py, pz, 0);
-        } else {
-          if (!recompute) n_dv[st] = (double)dg[(size_t)cell * nang + a];
-          if (q_in_lanes && a < cmom) n_q[st] = qg[(size_t)cell * cmom + a];
-          n_tx[st] = (double)tg[cell];
-        }
+        if (cell >= 0) load_cell(cell, st);
       }
     }
   };
@@ -403,12 +581,27 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs A, SweepGeneral 
     // (sweep.cc:568-591): phi_l = sum_a ec[l][a] * (w[a]*pc[a])
     wave_sync();
     for (int o = lane; o < nbuf * cmom; o += 64) {
-      const int kk = o / cmom, l = o - kk * cmom;
+      int kk = fk, l = fl;
+      if (o >= 64) {
+        kk = o / cmom;
+        l = o - kk * cmom;
+      }
       const int cell = cix[kk];
       if (cell < 0) continue;
       const double *v = pwb + kk * nang;
+      const double *e = ect + l;
       double sum = 0.0;
-      for (int b = 0; b < nang; b++) sum = madd<EXACT>(ect[b * cmom + l], v[b], sum);
+      int b = 0;
+      for (; b + 4 <= nang; b += 4) {
+        const double v0 = v[b], v1 = v[b + 1], v2 = v[b + 2], v3 = v[b + 3];
+        const double e0 = e[b * kEctRow], e1 = e[(b + 1) * kEctRow], e2 = e[(b + 2) * kEctRow],
+                     e3 = e[(b + 3) * kEctRow];
+        sum = madd<EXACT>(e0, v0, sum);
+        sum = madd<EXACT>(e1, v1, sum);
+        sum = madd<EXACT>(e2, v2, sum);
+        sum = madd<EXACT>(e3, v3, sum);
+      }
+      for (; b < nang; b++) sum = madd<EXACT>(e[b * kEctRow], v[b], sum);
       if constexpr (GEN) {
         const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
         double *dst = (l == 0) ? (double *)G.flux.at(px, py, pz, 0)
@@ -429,129 +622,201 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs A, SweepGeneral 
   for (int d = 0; d < ncp; ++d) {
     const int c0 = toff[d], c1 = toff[d + 1];
     const int nsteps = (c1 - c0 + per_step - 1) / per_step;
-    double c_dv[kMaxSteps], c_tx[kMaxSteps];
-    int c_cell[kMaxSteps];
+    double c_dv[MS][APL], c_tx[MS];
+    int c_cell[MS];
     // source moments of this diagonal's cells to the wave's LDS slots
 #pragma unroll
-    for (int st = 0; st < kMaxSteps; st++) {
-      c_dv[st] = n_dv[st];
+    for (int st = 0; st < MS; st++) {
+#pragma unroll
+      for (int r = 0; r < APL; r++) c_dv[st][r] = n_dv[st][r];
       c_tx[st] = n_tx[st];
       c_cell[st] = n_cell[st];
-      if (q_in_lanes && lane_ok && a < cmom) qsl[(st * CPW + cl) * cmom + a] = n_q[st];
+#pragma unroll
+      for (int m = 0; m < QS; m++)
+        if (q_in_lanes && lane_ok && la + m * LPC < cmom)
+          qsl[(st * CPW + cl) * cmom + la + m * LPC] = n_q[st][m];
+    }
+    if (!q_in_lanes) {
+      // moments do not fit the cell's lanes: the wave copies its cells'
+      // records to its LDS slots directly (not prefetched)
+      wave_sync();
+      for (int o = lane; o < MS * CPW * cmom; o += 64) {
+        const int sc = o / cmom, l = o - sc * cmom;  // sc = st * CPW + slot
+        const int st = sc / CPW, slot = sc - st * CPW;
+        const int my = c0 + st * per_step + wave * CPW + slot;
+        double v = 0.0;
+        if (my < c1) {
+          const int cell = tcr[my].x;
+          if (cell >= 0) {
+            if constexpr (GEN) {
+              const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
+              v = ((const double *)G.qtot.at(px, py, pz, 0))[l];
+            } else {
+              v = qg[(size_t)cell * cmom + l];
+            }
+          }
+        }
+        qsl[sc * cmom + l] = v;
+      }
     }
     if (d + 1 < ncp) prefetch(d + 1);
     wave_sync();
 #pragma unroll
-    for (int st = 0; st < kMaxSteps; ++st) {
+    for (int st = 0; st < MS; ++st) {
       if (st >= nsteps) break;
       const int my = c0 + st * per_step + wave * CPW + cl;
       const int cell = c_cell[st];
-      const bool valid = cell >= 0;
-      double pcw = 0.0;
-      if (valid) {
-        const int pk = tcells[my];
-        const int i = pk & 31, j = (pk >> 5) & 31, k = (pk >> 10) & 31;
-        // angular source (sweep.cc:331-345)
-        const double *q;
-        if (q_in_lanes) {
-          q = qsl + (st * CPW + cl) * cmom;
-        } else if constexpr (GEN) {
-          const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
-          q = (const double *)G.qtot.at(px, py, pz, 0);
-        } else {
-          q = qg + (size_t)cell * cmom;
-        }
-        Real psi;
+      double pcw[APL];
+#pragma unroll
+      for (int r = 0; r < APL; r++) pcw[r] = 0.0;
+      if (cell >= 0) {
+        const int4 rec = tcr[my];
+        // angular source (sweep.cc:331-345), moments from the LDS slots
+        const double *q = qsl + (st * CPW + cl) * cmom;
+        Real psi[APL];
         if constexpr (kFP64) {
-          double ps = q[0];
-          for (int l = 1; l < cmom; l++) ps = madd<EXACT>(ecs[l * nang + a], q[l], ps);
-          psi = ps;
+          double ps[APL];
+#pragma unroll
+          for (int r = 0; r < APL; r++) ps[r] = q[0];
+          for (int l = 1; l < cmom; l++) {
+            const double ql = q[l];
+#pragma unroll
+            for (int r = 0; r < APL; r++) ps[r] = madd<EXACT>(ecs[l * ECR + an[r]], ql, ps[r]);
+          }
+#pragma unroll
+          for (int r = 0; r < APL; r++) psi[r] = ps[r];
         } else {
-          float pf = (float)q[0];
-          for (int l = 1; l < cmom; l++) pf = maddf((float)ecs[l * nang + a], (float)q[l], pf);
-          psi = pf;
+          float pf[APL];
+#pragma unroll
+          for (int r = 0; r < APL; r++) pf[r] = (float)q[0];
+          for (int l = 1; l < cmom; l++) {
+            const float ql = (float)q[l];
+#pragma unroll
+            for (int r = 0; r < APL; r++)
+              pf[r] = maddf((float)ecs[l * ECR + an[r]], ql, pf[r]);
+          }
+#pragma unroll
+          for (int r = 0; r < APL; r++) psi[r] = pf[r];
         }
         const Real txs = (Real)c_tx[st];
-        Real dv;
-        if (recompute) {
-          if constexpr (kFP64)
-            dv = 1.0 / ((((double)txs + g_hm) + g_he) + g_hx);
+        Real dv[APL];
+#pragma unroll
+        for (int r = 0; r < APL; r++) {
+          // dinv as CalculateGeometryParam forms it (expxs.cc:378-381)
+          if constexpr (REC && kFP64 && EXACT)
+            dv[r] = 1.0 / ((((double)txs + hi * cx[r]) + hj * cy[r]) + hk * cz[r]);
+          else if constexpr (REC && kFP64)
+            dv[r] = 1.0 / ((((double)txs + cx[r]) + cy[r]) + cz[r]);
+          else if constexpr (REC)
+            dv[r] = 1.0f / ((float)txs + g_sum32[r]);
           else
-            dv = 1.0f / ((float)txs + g_sum32);
-        } else {
-          dv = (Real)c_dv[st];
+            dv[r] = (Real)c_dv[st][r];
         }
-        Real *sx = fx + (k * TY + j) * nang + a;
-        Real *sy = fy + (k * TX + i) * nang + a;
-        Real *sz = fz + (j * TX + i) * nang + a;
-        const Real psii = *sx, psij = *sy, psik = *sz;
-        // initial diamond-difference solution (sweep.cc:352-400)
-        Real pc = psi;
-        if constexpr (kFP64 && EXACT) {
-          pc += psii * mu_a * hi;
-          pc += psij * eta_a * hj;
-          pc += psik * xi_a * hk;
-        } else if constexpr (kFP64) {
-          pc = __builtin_fma(psii, muh, pc);
-          pc = __builtin_fma(psij, etah, pc);
-          pc = __builtin_fma(psik, xih, pc);
-        } else {
-          pc = maddf(psii, muh, pc);
-          pc = maddf(psij, etah, pc);
-          pc = maddf(psik, xih, pc);
+        // inflow faces; slots past the last angle read angle 0's entry and
+        // are never stored
+        Real *sx = fx + rec.y;
+        Real *sy = fy + rec.z;
+        Real *sz = fz + rec.w;
+        Real psii[APL], psij[APL], psik[APL], pc[APL];
+#pragma unroll
+        for (int r = 0; r < APL; r++) {
+          psii[r] = sx[an[r]];
+          psij[r] = sy[an[r]];
+          psik[r] = sz[an[r]];
+          // initial diamond-difference solution (sweep.cc:352-400)
+          Real p = psi[r];
+          if constexpr (kFP64 && EXACT) {
+            p += psii[r] * cx[r] * hi;
+            p += psij[r] * cy[r] * hj;
+            p += psik[r] * cz[r] * hk;
+          } else if constexpr (kFP64) {
+            p = __builtin_fma(psii[r], cx[r], p);
+            p = __builtin_fma(psij[r], cy[r], p);
+            p = __builtin_fma(psik[r], cz[r], p);
+          } else {
+            p = maddf(psii[r], cx[r], p);
+            p = maddf(psij[r], cy[r], p);
+            p = maddf(psik[r], cz[r], p);
+          }
+          pc[r] = p * dv[r];
         }
-        pc *= dv;
-        Real ox, oy, oz;
+        // outflow factors: 1 while the face keeps its diamond-difference
+        // value, 0 once the fixup has set it to zero
+        bool hx[APL], hy[APL], hz[APL];
+#pragma unroll
+        for (int r = 0; r < APL; r++) hx[r] = hy[r] = hz[r] = true;
         if (A.fixup) {
           // negative-flux fixup (sweep.cc:402-505; gpu_sweep.cu:1556-1630).
-          // Each pass that continues clears at least one of the 3*nang
-          // flags or ends the loop, so 3*nang+2 passes always suffice.
-          Real hvx = Real(1), hvy = Real(1), hvz = Real(1);
-          Real fxx = Real(0), fxy = Real(0), fxz = Real(0);
-          unsigned old_neg = 0;
-          for (int it = 0; it < 3 * nang + 2; ++it) {
-            fxx = Real(2) * pc - psii;
-            fxy = Real(2) * pc - psij;
-            fxz = Real(2) * pc - psik;
-            const bool nxn = fxx < Real(0), nyn = fxy < Real(0), nzn = fxz < Real(0);
-            if (nxn) hvx = Real(0);
-            if (nyn) hvy = Real(0);
-            if (nzn) hvz = Real(0);
-            const unsigned neg = (unsigned)(__popcll(__ballot(nxn) & seg) +
-                                            __popcll(__ballot(nyn) & seg) +
-                                            __popcll(__ballot(nzn) & seg));
-            if (neg == old_neg) break;
-            old_neg = neg;
-            Real den;
-            if constexpr (kFP64 && EXACT) {
-              pc = psi + 0.5 * (psii * mu_a * hi * (1.0 + hvx) + psij * eta_a * hj * (1.0 + hvy) +
-                                psik * xi_a * hk * (1.0 + hvz));
-              den = (pc <= 0.0) ? 0.0 : (txs + mu_a * hi * hvx + eta_a * hj * hvy + xi_a * hk * hvz);
-            } else {
-              const Real t = psii * muh * (Real(1) + hvx) + psij * etah * (Real(1) + hvy) +
-                             psik * xih * (Real(1) + hvz);
-              pc = psi + Real(0.5) * t;
-              den = (pc <= Real(0)) ? Real(0) : (txs + muh * hvx + etah * hvy + xih * hvz);
+          // The count of negative outflows is cell-global over angles.  A
+          // pass that clears no new flag leaves the next pc equal to this
+          // one, so the pass after it repeats the count and ends the loop:
+          // with 3*nang flags, 3*nang+2 passes always reach the break, and
+          // the outflows are 2*pc - psi_in of the final pc.
+          auto count_negative = [&]() -> unsigned {
+            unsigned long long bal[3 * APL];
+#pragma unroll
+            for (int r = 0; r < APL; r++) {
+              const Real p2 = Real(2) * pc[r];
+              const bool nxn = aok[r] && p2 - psii[r] < Real(0);
+              const bool nyn = aok[r] && p2 - psij[r] < Real(0);
+              const bool nzn = aok[r] && p2 - psik[r] < Real(0);
+              hx[r] = hx[r] && !nxn;
+              hy[r] = hy[r] && !nyn;
+              hz[r] = hz[r] && !nzn;
+              bal[3 * r] = __builtin_amdgcn_ballot_w64(nxn);
+              bal[3 * r + 1] = __builtin_amdgcn_ballot_w64(nyn);
+              bal[3 * r + 2] = __builtin_amdgcn_ballot_w64(nzn);
             }
-            pc = (den < tolr) ? Real(0) : pc / den;
+            return seg_count<CPW>(bal, seg0, LPC, cl);
+          };
+          unsigned old_neg = count_negative();
+          if (old_neg != 0) {
+            for (int it = 1; it < 3 * nang + 2; ++it) {
+#pragma unroll
+              for (int r = 0; r < APL; r++) {
+                const Real vx = hx[r] ? Real(1) : Real(0), vy = hy[r] ? Real(1) : Real(0),
+                           vz = hz[r] ? Real(1) : Real(0);
+                Real den, p;
+                if constexpr (kFP64 && EXACT) {
+                  p = psi[r] + 0.5 * (psii[r] * cx[r] * hi * (1.0 + vx) +
+                                      psij[r] * cy[r] * hj * (1.0 + vy) +
+                                      psik[r] * cz[r] * hk * (1.0 + vz));
+                  den = (p <= 0.0) ? 0.0
+                                   : (txs + cx[r] * hi * vx + cy[r] * hj * vy + cz[r] * hk * vz);
+                } else {
+                  const Real t = psii[r] * cx[r] * (Real(1) + vx) +
+                                 psij[r] * cy[r] * (Real(1) + vy) + psik[r] * cz[r] * (Real(1) + vz);
+                  p = psi[r] + Real(0.5) * t;
+                  den = (p <= Real(0)) ? Real(0) : (txs + cx[r] * vx + cy[r] * vy + cz[r] * vz);
+                }
+                pc[r] = (den < tolr) ? Real(0) : p / den;
+              }
+              const unsigned neg = count_negative();
+              if (neg == old_neg) break;
+              old_neg = neg;
+            }
           }
-          ox = fxx * hvx;
-          oy = fxy * hvy;
-          oz = fxz * hvz;
-        } else {
-          ox = Real(2) * pc - psii;
-          oy = Real(2) * pc - psij;
-          oz = Real(2) * pc - psik;
         }
-        *sx = ox;
-        *sy = oy;
-        *sz = oz;
-        pcw = w_d * (double)pc;
+#pragma unroll
+        for (int r = 0; r < APL; r++) {
+          const Real p2 = Real(2) * pc[r];
+          const Real ox = (p2 - psii[r]) * (hx[r] ? Real(1) : Real(0));
+          const Real oy = (p2 - psij[r]) * (hy[r] ? Real(1) : Real(0));
+          const Real oz = (p2 - psik[r]) * (hz[r] ? Real(1) : Real(0));
+          if (aok[r]) {
+            sx[an[r]] = ox;
+            sy[an[r]] = oy;
+            sz[an[r]] = oz;
+          }
+          pcw[r] = w_d[r] * (double)pc[r];
+        }
       }
       // ---- batch this step's cells for the moment sums
       if (lane_ok) {
-        pwb[(nbuf + cl) * nang + a] = pcw;
-        if (a == 0) cix[nbuf + cl] = (my < c1) ? cell : -1;
+#pragma unroll
+        for (int r = 0; r < APL; r++)
+          if (aok[r]) pwb[(nbuf + cl) * nang + an[r]] = pcw[r];
+        if (la == 0) cix[nbuf + cl] = cell;
       }
       nbuf += CPW;
       if (nbuf + CPW > SB) flush();
@@ -563,35 +828,32 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs A, SweepGeneral 
   // ---- tile outflow faces: to the face buffer, and at the chunk's far side
   // to the ghost arrays
   Real *out = faces_s + (size_t)tile_id * per_tile;
-  for (int idx = threadIdx.x; idx < fxn; idx += blockDim.x) {
-    out[idx] = fx[idx];
-    if (GEN && tx == A.nTx - 1 && G.gx.base) {
-      const int a2 = idx % nang, slot = idx / nang, j = slot % TY, k = slot / TY;
-      if (j < ey && k < ez) {
-        const int py = syp ? y0 + j : ny - 1 - (y0 + j), pz = szp ? z0 + k : nz - 1 - (z0 + k);
-        ((double *)G.gx.at(py, pz, 0))[a2] = (double)fx[idx];
+  for (int idx = threadIdx.x; idx < per_tile; idx += blockDim.x) out[idx] = fx[idx];
+  if constexpr (GEN) {
+    if (tx == A.nTx - 1 && G.gx.base)
+      for (int idx = threadIdx.x; idx < fxn; idx += blockDim.x) {
+        const int a2 = idx % nang, slot = idx / nang, j = slot % TY, k = slot / TY;
+        if (j < ey && k < ez) {
+          const int py = syp ? y0 + j : ny - 1 - (y0 + j), pz = szp ? z0 + k : nz - 1 - (z0 + k);
+          ((double *)G.gx.at(py, pz, 0))[a2] = (double)fx[idx];
+        }
       }
-    }
-  }
-  for (int idx = threadIdx.x; idx < fyn; idx += blockDim.x) {
-    out[fxn + idx] = fy[idx];
-    if (GEN && ty == A.nTy - 1 && G.gy.base) {
-      const int a2 = idx % nang, slot = idx / nang, i = slot % TX, k = slot / TX;
-      if (i < ex && k < ez) {
-        const int px = sxp ? x0 + i : nx - 1 - (x0 + i), pz = szp ? z0 + k : nz - 1 - (z0 + k);
-        ((double *)G.gy.at(px, pz, 0))[a2] = (double)fy[idx];
+    if (ty == A.nTy - 1 && G.gy.base)
+      for (int idx = threadIdx.x; idx < fyn; idx += blockDim.x) {
+        const int a2 = idx % nang, slot = idx / nang, i = slot % TX, k = slot / TX;
+        if (i < ex && k < ez) {
+          const int px = sxp ? x0 + i : nx - 1 - (x0 + i), pz = szp ? z0 + k : nz - 1 - (z0 + k);
+          ((double *)G.gy.at(px, pz, 0))[a2] = (double)fy[idx];
+        }
       }
-    }
-  }
-  for (int idx = threadIdx.x; idx < fzn; idx += blockDim.x) {
-    out[fxn + fyn + idx] = fz[idx];
-    if (GEN && tz == A.nTz - 1 && G.gz.base) {
-      const int a2 = idx % nang, slot = idx / nang, i = slot % TX, j = slot / TX;
-      if (i < ex && j < ey) {
-        const int px = sxp ? x0 + i : nx - 1 - (x0 + i), py = syp ? y0 + j : ny - 1 - (y0 + j);
-        ((double *)G.gz.at(px, py, 0))[a2] = (double)fz[idx];
+    if (tz == A.nTz - 1 && G.gz.base)
+      for (int idx = threadIdx.x; idx < fzn; idx += blockDim.x) {
+        const int a2 = idx % nang, slot = idx / nang, i = slot % TX, j = slot / TX;
+        if (i < ex && j < ey) {
+          const int px = sxp ? x0 + i : nx - 1 - (x0 + i), py = syp ? y0 + j : ny - 1 - (y0 + j);
+          ((double *)G.gz.at(px, py, 0))[a2] = (double)fz[idx];
+        }
       }
-    }
   }
 }
 
@@ -600,18 +862,37 @@ __global__ void __launch_bounds__(1024) k_sweep_tiles(SweepArgs A, SweepGeneral 
 // ---------------------------------------------------------------------------
 typedef void (*SweepKernel)(SweepArgs, SweepGeneral, int);
 
-template <typename Real, bool EXACT, bool GEN>
-static SweepKernel kernel_for(int cpw) {
+template <typename Real, int APL, bool EXACT, bool REC, bool GEN>
+static SweepKernel kernel_for_apl(int cpw) {
   switch (cpw) {
-    case 1: return k_sweep_tiles<Real, 1, EXACT, GEN>;
-    case 2: return k_sweep_tiles<Real, 2, EXACT, GEN>;
-    case 3: return k_sweep_tiles<Real, 3, EXACT, GEN>;
-    case 4: return k_sweep_tiles<Real, 4, EXACT, GEN>;
-    case 5: return k_sweep_tiles<Real, 5, EXACT, GEN>;
-    case 6: return k_sweep_tiles<Real, 6, EXACT, GEN>;
-    case 8: return k_sweep_tiles<Real, 8, EXACT, GEN>;
+    case 1: return k_sweep_tiles<Real, 1, APL, EXACT, REC, GEN>;
+    case 2: return k_sweep_tiles<Real, 2, APL, EXACT, REC, GEN>;
+    case 3: return k_sweep_tiles<Real, 3, APL, EXACT, REC, GEN>;
+    case 4: return k_sweep_tiles<Real, 4, APL, EXACT, REC, GEN>;
+    case 5: return k_sweep_tiles<Real, 5, APL, EXACT, REC, GEN>;
+    case 6: return k_sweep_tiles<Real, 6, APL, EXACT, REC, GEN>;
+    case 8: return k_sweep_tiles<Real, 8, APL, EXACT, REC, GEN>;
     default: return nullptr;
   }
+}
+
+// The drop-in (general) variant reads dinv as the reference does, and runs
+// two angles per lane only where one cell's angles exceed a wavefront.
+template <typename Real, bool EXACT>
+static SweepKernel kernel_for(int cpw, int apl, bool rec, bool gen) {
+  if (gen) {
+    if (rec) return nullptr;
+    if (apl == 1) return kernel_for_apl<Real, 1, EXACT, false, true>(cpw);
+    if (apl == 2 && cpw == 1) return k_sweep_tiles<Real, 1, 2, EXACT, false, true>;
+    return nullptr;
+  }
+  if (apl == 1)
+    return rec ? kernel_for_apl<Real, 1, EXACT, true, false>(cpw)
+               : kernel_for_apl<Real, 1, EXACT, false, false>(cpw);
+  if (apl == 2)
+    return rec ? kernel_for_apl<Real, 2, EXACT, true, false>(cpw)
+               : kernel_for_apl<Real, 2, EXACT, false, false>(cpw);
+  return nullptr;
 }
 
 static int ensure_lds_attr(SweepKernel k, size_t bytes) {
@@ -652,6 +933,7 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
   A.cmom = p.cmom;
   A.ncellplanes = p.ncellplanes;
   A.sb = p.sb;
+  A.lpc = p.lpc;
   A.ncells_tile = p.TX * p.TY * p.TZ;
   A.recompute_dinv = L.recompute_dinv ? 1 : 0;
   A.ng_job = L.ng_job;
@@ -690,14 +972,14 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
     if (!L.recompute_dinv && !A.dinv) return fail(SNAP_ERR_INVALID, "dinv is required");
   }
   const size_t lds = p.lds_bytes(L.fp32 ? 4 : 8, L.recompute_dinv);
-  SweepKernel k;
-  if (L.general)
-    k = L.fp32 ? kernel_for<float, false, true>(p.cpw)
-               : (L.exact ? kernel_for<double, true, true>(p.cpw) : kernel_for<double, false, true>(p.cpw));
-  else
-    k = L.fp32 ? kernel_for<float, false, false>(p.cpw)
-               : (L.exact ? kernel_for<double, true, false>(p.cpw) : kernel_for<double, false, false>(p.cpw));
-  if (!k) return fail(SNAP_ERR_INVALID, "no sweep kernel for %d cells per wave", p.cpw);
+  const SweepKernel k =
+      L.fp32 ? kernel_for<float, false>(p.cpw, p.apl, L.recompute_dinv, L.general)
+             : (L.exact ? kernel_for<double, true>(p.cpw, p.apl, L.recompute_dinv, L.general)
+                        : kernel_for<double, false>(p.cpw, p.apl, L.recompute_dinv, L.general));
+  if (!k)
+    return fail(SNAP_ERR_INVALID,
+                "no sweep kernel for %d cells per wave, %d angles per lane%s%s", p.cpw, p.apl,
+                L.recompute_dinv ? ", dinv recomputed" : "", L.general ? ", general" : "");
   SNAP_TRY(ensure_lds_attr(k, lds));
   const dim3 block(p.waves * 64);
   for (int D = 0; D < p.nplanes; D++) {
@@ -803,10 +1085,10 @@ extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream
   for (auto &pp : c.plans)
     if (pp.first.nx == e.nx && pp.first.ny == e.ny && pp.first.nz == e.nz) plan = pp.second;
   if (!plan) {
-    int TX, TY, TZ, waves;
-    plan_default_tile(e, c.quad.nang, c.quad.cmom, &TX, &TY, &TZ, &waves);
+    int TX = 0, TY = 0, TZ = 0, waves = 0, apl = 0;
+    plan_choose(e, c.quad.nang, c.quad.cmom, 8, &TX, &TY, &TZ, &waves, &apl);
     plan = new SweepPlan();
-    int rc = plan_create(plan, e, c.quad.nang, c.quad.cmom, TX, TY, TZ, waves);
+    int rc = plan_create(plan, e, c.quad.nang, c.quad.cmom, TX, TY, TZ, waves, apl);
     if (rc != SNAP_OK) {
       delete plan;
       return rc;

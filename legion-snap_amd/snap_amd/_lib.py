@@ -79,7 +79,7 @@ class SolveOptions(ctypes.Structure):
     _fields_ = [
         ("precision", ctypes.c_int), ("tile", ctypes.c_int * 3),
         ("waves_per_tile", ctypes.c_int), ("first_sweep_fp64", ctypes.c_int),
-        ("dinv_mode", ctypes.c_int),
+        ("dinv_mode", ctypes.c_int), ("angles_per_lane", ctypes.c_int),
     ]
 
 
@@ -137,6 +137,7 @@ SIGNATURES = {
     "snap_hip_solver_get_flux": ([_vp, _dp, _dp], ctypes.c_int),
     "snap_hip_solver_time_sweeps": (
         [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "snap_hip_solver_plan": ([_vp, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     "snap_hip_solver_destroy": ([_vp], None),
 }
 

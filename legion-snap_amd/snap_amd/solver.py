@@ -37,7 +37,7 @@ class Solver:
 
     def __init__(self, problem: Problem, precision: str = FP64, tile=(0, 0, 0),
                  waves_per_tile: int = 0, first_sweep_fp64: bool = False,
-                 dinv: str = "default"):
+                 dinv: str = "default", angles_per_lane: int = 0):
         problem.validate()
         self.problem = p = problem.contiguous()
         self.lib = _lib.load()
@@ -58,6 +58,7 @@ class Solver:
         o.first_sweep_fp64 = int(bool(first_sweep_fp64))
         o.dinv_mode = {"default": _lib.DINV_DEFAULT, "stream": _lib.DINV_STREAM,
                        "recompute": _lib.DINV_RECOMPUTE}[dinv]
+        o.angles_per_lane = angles_per_lane
         self.precision = precision
         h = ctypes.c_void_p()
         _lib.check(self.lib.snap_hip_solver_create(ctypes.byref(s), ctypes.byref(o), ctypes.byref(h)),
@@ -86,6 +87,14 @@ class Solver:
                                                          ctypes.byref(ml), ctypes.byref(nl)),
                    "snap_hip_solver_time_sweeps")
         return {"ms_per_sweep": ms.value, "ms_per_launch": ml.value, "launches_per_sweep": nl.value}
+
+    def plan(self) -> dict:
+        """The sweep plan the solver chose (tile edges, waves per tile,
+        angles per lane, cells per wavefront)."""
+        v = (ctypes.c_int32 * 6)()
+        _lib.check(self.lib.snap_hip_solver_plan(self._h, v), "snap_hip_solver_plan")
+        return {"tile": (v[0], v[1], v[2]), "waves": v[3], "angles_per_lane": v[4],
+                "cells_per_wave": v[5]}
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -27,6 +27,7 @@ CASES = {
     "config3_small": lambda: problems.config3(n=16, ng=4, nang=24),
     "p2_nang10": lambda: problems.config1(n=24, ng=4, nang=10, nmom=3),
     "odd_shape": lambda: problems.config1(n=9, ng=3, nang=7, nmom=1),
+    "nang80": lambda: problems.config1(n=10, ng=2, nang=80, nmom=2),
 }
 
 _oracle_cache = {}
@@ -85,6 +86,21 @@ def test_mixed_precision_error(gpu, oracle_mod, name):
     assert mx <= 5e-6 and l2 <= 1e-6
 
 
+@pytest.mark.parametrize("name", ["config1_small", "config1_ragged", "odd_shape"])
+@pytest.mark.parametrize("apl", [1, 2])
+def test_angles_per_lane_keep_exact_results(gpu, oracle_mod, name, apl):
+    # one or two angles per lane (and the cells per wavefront that follow)
+    # change only the lane layout, never the arithmetic
+    fo, fmo, _ = oracle_solution(oracle_mod, name)
+    with Solver(CASES[name](), precision="fp64_exact", angles_per_lane=apl) as s:
+        s.run()
+        f, fm = s.flux()
+        assert s.plan()["angles_per_lane"] == apl
+    assert np.array_equal(f, fo)
+    if fmo is not None:
+        assert np.array_equal(fm, fmo)
+
+
 def test_dinv_modes_agree(gpu):
     # forming dinv in the kernel is the same arithmetic as the geometry kernel
     p = CASES["config1_small"]()
@@ -118,7 +134,7 @@ def test_solver_rejects_bad_input(gpu):
     p.mat[0, 0, 0] = 5
     with pytest.raises(ValueError):
         Solver(p)
-    p = problems.config0(n=4, ng=2, nang=80)  # more angles than the tiled sweep handles
+    p = problems.config0(n=4, ng=2, nang=130)  # more angles than the tiled sweep handles
     with pytest.raises(_lib.SnapError) as e:
         Solver(p)
     assert e.value.code == _lib.SNAP_ERR_UNSUPPORTED
