@@ -130,6 +130,7 @@ def main() -> int:
         wall = time.perf_counter() - t0
         f64, fm64 = s.flux()
         ts = s.time_sweeps(3)
+        plan64 = s.plan()
     inner = stats[-1]["inner_total"]
     sweep_ms = float(np.mean([st["sweep_ms"] for st in stats]))
     solve_ms = float(np.mean([st["solve_ms"] for st in stats]))
@@ -140,7 +141,7 @@ def main() -> int:
     details["fp64"] = {"grind_ns": grind, "sweep_ms_per_inner": sweep_ms / inner,
                        "solve_s": solve_ms / 1e3, "inner": inner, "outer": stats[-1]["outer_total"],
                        "wall_s_per_step": wall / args.steps, "launch_ms": ts["ms_per_launch"],
-                       "launches_per_sweep": launches, "bytes_per_unit": b64}
+                       "launches_per_sweep": launches, "bytes_per_unit": b64, "plan": plan64}
 
     # ---- FP64 in the reference's operation order (bit-equal to the oracle)
     with Solver(p, precision="fp64_exact") as s:
@@ -150,19 +151,23 @@ def main() -> int:
                              "solve_s": st["solve_ms"] / 1e3, "inner": st["inner_total"],
                              "maxrel_vs_fp64_fused": float(np.max(np.abs(fex - f64) / np.abs(fex)))}
 
-    # ---- mixed precision (FP32 sweep, FP64 moments / sources / tests)
-    with Solver(p, precision="mixed") as s:
-        for _ in range(args.warmup):
-            s.run()
-        st = s.run()
-        fmx, _ = s.flux()
-    d = np.abs(fmx - fex)
-    details["mixed"] = {"grind_ns": st["sweep_ms"] * 1e6 / (units * st["inner_total"]),
-                        "solve_s": st["solve_ms"] / 1e3, "inner": st["inner_total"],
-                        "outer": st["outer_total"],
+    # ---- mixed precision (FP32 sweep, FP64 moments / sources / tests), plain
+    # and with per-outer defect correction
+    for key, kw in (("mixed", {}), ("mixed_refine", {"refine": True})):
+        with Solver(p, precision="mixed", **kw) as s:
+            for _ in range(args.warmup):
+                s.run()
+            st = s.run()
+            fmx, _ = s.flux()
+            plan = s.plan()
+        d = np.abs(fmx - fex)
+        g = st["sweep_ms"] * 1e6 / (units * st["inner_total"])
+        details[key] = {"grind_ns": g, "solve_s": st["solve_ms"] / 1e3, "inner": st["inner_total"],
+                        "outer": st["outer_total"], "sweeps_fp64": st["sweeps_fp64"],
+                        "sweeps_fp32": st["sweeps_fp32"],
                         "maxrel_vs_fp64": float(np.max(d / np.abs(fex))),
                         "l2rel_vs_fp64": float(np.linalg.norm(d) / np.linalg.norm(fex)),
-                        "speedup_vs_fp64_grind": grind / (st["sweep_ms"] * 1e6 / (units * st["inner_total"]))}
+                        "speedup_vs_fp64_grind": grind / g, "plan": plan}
 
     cpu = None
     if not args.no_cpu:

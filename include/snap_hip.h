@@ -220,7 +220,8 @@ typedef struct {
   int fixup;
 } snap_problem_t;
 
-#define SNAP_DINV_DEFAULT 0     /* FP64: stream dinv; MIXED: recompute */
+#define SNAP_DINV_DEFAULT 0     /* stream; MIXED reads the FP64-formed dinv rounded
+                                   once to FP32 (half the error of forming it in FP32) */
 #define SNAP_DINV_STREAM 1      /* sweep reads the dinv array (reference data flow) */
 #define SNAP_DINV_RECOMPUTE 2   /* sweep forms dinv from t_xs per cell and angle */
 
@@ -231,7 +232,16 @@ typedef struct {
   int first_sweep_fp64;     /* MIXED: run the first inner's sweep in FP64 */
   int dinv_mode;            /* SNAP_DINV_DEFAULT, _STREAM or _RECOMPUTE */
   int angles_per_lane;      /* sweep angles per GPU lane, 1 or 2; 0 = choose */
+  int refine;               /* MIXED: 1 = defect correction (below); 0 = FP32
+                               sweeps of the whole source */
 } snap_solve_options_t;
+
+/* MIXED with refine = 1 (iterative refinement of each outer iteration): the
+ * first inner sweep of an outer iteration runs in FP64 with the fixup and is
+ * kept as the base (source q0, fluxes phi0).  Each later inner computes the
+ * FP64 defect q - q0 of its source, sweeps it in FP32 without the fixup (the
+ * sweep is linear in its source when no flux is fixed) and adds the result
+ * to phi0 in FP64. */
 
 typedef struct {
   int inner_total;          /* inner iterations run */

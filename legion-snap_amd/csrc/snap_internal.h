@@ -127,7 +127,11 @@ int launch_inner_source(Acc3 sxs, Acc3 flux0, Acc3 q2grp0, Acc3 fluxm, Acc3 q2gr
                         Extent3 e, int ng, int cmom, const int *lma, int nmom, hipStream_t st);
 int launch_convergence(Acc3 flux0, Acc3 flux0p, Extent3 e, int ng, double epsi, int32_t *result,
                        hipStream_t st);
-int launch_octant_reduce(const double *part, double *out, int64_t n, hipStream_t st);
+// out = (base ? base : 0) + sum over the 8 octant partials, octants in order
+int launch_octant_reduce(const double *part, double *out, int64_t n, hipStream_t st,
+                         const double *base = nullptr);
+// y -= x
+int launch_subtract(double *y, const double *x, int64_t n, hipStream_t st);
 int launch_fill_int(int32_t *p, int32_t v, hipStream_t st);
 
 // ---------------------------------------------------------------------------

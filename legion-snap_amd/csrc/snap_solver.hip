@@ -15,6 +15,7 @@
 // All 8*ng sweeps of one inner iteration are one job: each tile-diagonal
 // launch covers every (corner, group) pair.  Sweeps write per-octant partial
 // fluxes, which are summed in octant order, so results are deterministic.
+#include <algorithm>
 #include <cstring>
 #include <memory>
 
@@ -54,17 +55,20 @@ struct snap_solver {
   int64_t cells = 0;
   double hi = 0, hj = 0, hk = 0;
   QuadDev quad;
-  SweepPlan plan;
+  SweepPlan plan;    // the sweep plan of the solver's precision
+  SweepPlan plan64;  // MIXED: the plan of its FP64 sweeps
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int32_t *h_flag = nullptr;  // pinned
   DeviceBuffer mat, sigt, slgg, qi, txs, txs32, sxs, dinv, dinv32;
   DeviceBuffer flux0, flux0pi, flux0po, fluxm, q2grp0, q2grpm, qtot;
   DeviceBuffer part_flux, part_fluxm, faces, flag;
+  DeviceBuffer qbase, flux0base, fluxmbase;  // defect correction: the outer's FP64 base
   bool have_fp32 = false;
 
   ~snap_solver() {
     plan_free(&plan);
+    plan_free(&plan64);
     quad_free(&quad);
     for (auto &x : ev)
       if (x) (void)hipEventDestroy(x);
@@ -80,16 +84,20 @@ struct snap_solver {
     return SNAP_OK;
   }
 
-  int sweep(int precision, hipEvent_t *events) {
+  const SweepPlan &plan_for(int precision) const {
+    return (precision != SNAP_PRECISION_MIXED && plan64.nplanes) ? plan64 : plan;
+  }
+
+  int sweep(int precision, hipEvent_t *events, bool fixup = true) {
     const bool fp32 = precision == SNAP_PRECISION_MIXED;
     SweepLaunch L;
     memset(&L, 0, sizeof(L));
-    L.plan = &plan;
+    L.plan = &plan_for(precision);
     L.quad = &quad;
     L.nsweeps = 8 * ng;
     L.ng_job = ng;
     L.corner0 = 0;
-    L.fixup = prob.fixup ? 1 : 0;
+    L.fixup = (prob.fixup && fixup) ? 1 : 0;
     L.hi = hi;
     L.hj = hj;
     L.hk = hk;
@@ -102,17 +110,19 @@ struct snap_solver {
     L.faces = faces.p;
     L.fp32 = fp32;
     L.exact = precision == SNAP_PRECISION_FP64_EXACT;
-    L.recompute_dinv = fp32 ? (opt.dinv_mode != SNAP_DINV_STREAM) : (opt.dinv_mode == SNAP_DINV_RECOMPUTE);
+    L.recompute_dinv = opt.dinv_mode == SNAP_DINV_RECOMPUTE;
     if (fp32 && !L.recompute_dinv && !dinv32.p) return fail(SNAP_ERR_INVALID, "no FP32 dinv");
     return sweep_enqueue(L, stream, events);
   }
 
-  int reduce_octants() {
+  // flux = sum of the octant partials (+ the base fluxes of a correction sweep)
+  int reduce_octants(bool add_base = false) {
     SNAP_TRY(launch_octant_reduce(part_flux.as<double>(), flux0.as<double>(), (int64_t)ng * cells,
-                                  stream));
+                                  stream, add_base ? flux0base.as<double>() : nullptr));
     if (cmom > 1)
       SNAP_TRY(launch_octant_reduce(part_fluxm.as<double>(), fluxm.as<double>(),
-                                    (int64_t)ng * cells * (cmom - 1), stream));
+                                    (int64_t)ng * cells * (cmom - 1), stream,
+                                    add_base ? fluxmbase.as<double>() : nullptr));
     return SNAP_OK;
   }
 
@@ -196,6 +206,14 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
   plan_choose(s->e, s->nang, s->cmom, s->opt.precision == SNAP_PRECISION_MIXED ? 4 : 8, &TX, &TY,
               &TZ, &waves, &apl);
   SNAP_TRY(plan_create(&s->plan, s->e, s->nang, s->cmom, TX, TY, TZ, waves, apl));
+  if (s->opt.precision == SNAP_PRECISION_MIXED) {
+    // the FP64 sweeps of a mixed solve (first sweep, refinement bases) get
+    // their own plan unless the caller fixed the tiling
+    int tx = s->opt.tile[0], ty = s->opt.tile[1], tz = s->opt.tile[2];
+    int w = s->opt.waves_per_tile, a = s->opt.angles_per_lane;
+    plan_choose(s->e, s->nang, s->cmom, 8, &tx, &ty, &tz, &w, &a);
+    SNAP_TRY(plan_create(&s->plan64, s->e, s->nang, s->cmom, tx, ty, tz, w, a));
+  }
 
   SNAP_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   for (auto &x : s->ev) SNAP_HIP_TRY(hipEventCreate(&x));
@@ -219,7 +237,9 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
   SNAP_TRY(s->q2grpm.alloc(sizeof(double) * gc * nm1));
   SNAP_TRY(s->part_flux.alloc(sizeof(double) * 8 * gc));
   SNAP_TRY(s->part_fluxm.alloc(sizeof(double) * 8 * gc * nm1));
-  SNAP_TRY(s->faces.alloc(sizeof(double) * s->plan.face_doubles_per_sweep() * 8 * s->ng));
+  SNAP_TRY(s->faces.alloc(sizeof(double) * 8 * s->ng *
+                          std::max(s->plan.face_doubles_per_sweep(),
+                                   s->plan64.nplanes ? s->plan64.face_doubles_per_sweep() : 0)));
   SNAP_TRY(s->flag.alloc(sizeof(int32_t)));
 
   // ExpandCrossSection(sigt -> t_xs), ExpandScatteringCrossSection,
@@ -259,6 +279,11 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
     for (size_t i = 0; i < gc; i++) hf[i] = (float)h[i];
     SNAP_HIP_TRY(hipMemcpy(s->txs32.p, hf.data(), sizeof(float) * gc, hipMemcpyHostToDevice));
     s->have_fp32 = true;
+    if (s->opt.refine) {
+      SNAP_TRY(s->qbase.alloc(sizeof(double) * gc * s->cmom));
+      SNAP_TRY(s->flux0base.alloc(sizeof(double) * gc));
+      SNAP_TRY(s->fluxmbase.alloc(sizeof(double) * gc * nm1));
+    }
   }
   SNAP_HIP_TRY(hipStreamSynchronize(st));
   *out = s.release();
@@ -286,13 +311,23 @@ extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) 
     for (int inno = 0; inno < s->prob.iitm; inno++) {
       SNAP_TRY(s->inner_source());
       SNAP_TRY(s->copy(s->flux0pi, s->flux0, sizeof(double) * gc));
-      const bool fp32 = mixed && !(s->opt.first_sweep_fp64 && st.inner_total == 0);
+      const bool refine = mixed && s->opt.refine;
+      const bool fp32 = mixed && !(s->opt.first_sweep_fp64 && st.inner_total == 0) &&
+                        !(refine && inno == 0);
+      if (refine && fp32)  // FP64 defect of the source against the outer's base
+        SNAP_TRY(launch_subtract(s->qtot.as<double>(), s->qbase.as<double>(),
+                                 (int64_t)gc * s->cmom, stream));
       SNAP_HIP_TRY(hipEventRecord(s->ev[0], stream));
       SNAP_TRY(s->sweep(fp32 ? SNAP_PRECISION_MIXED
                              : (mixed ? SNAP_PRECISION_FP64 : s->opt.precision),
-                        nullptr));
+                        nullptr, !(refine && fp32)));
       SNAP_HIP_TRY(hipEventRecord(s->ev[1], stream));
-      SNAP_TRY(s->reduce_octants());
+      SNAP_TRY(s->reduce_octants(refine && fp32));
+      if (refine && !fp32) {  // keep the FP64 base of this outer
+        SNAP_TRY(s->copy(s->qbase, s->qtot, sizeof(double) * gc * s->cmom));
+        SNAP_TRY(s->copy(s->flux0base, s->flux0, sizeof(double) * gc));
+        if (nm1) SNAP_TRY(s->copy(s->fluxmbase, s->fluxm, sizeof(double) * gc * nm1));
+      }
       if (fp32)
         st.sweeps_fp32++;
       else
@@ -342,7 +377,7 @@ extern "C" int snap_hip_solver_time_sweeps(snap_solver_t *s, int count, int prec
   const bool fp32 = precision == SNAP_PRECISION_MIXED;
   if (fp32 && !s->have_fp32) return fail(SNAP_ERR_INVALID, "solver was created without mixed precision");
   if (precision < 0 || precision > 2) return fail(SNAP_ERR_INVALID, "unknown precision %d", precision);
-  const int np = s->plan.nplanes;
+  const int np = s->plan_for(precision).nplanes;
   std::vector<hipEvent_t> evs((size_t)2 * np * count);
   for (auto &x : evs) SNAP_HIP_TRY(hipEventCreate(&x));
   SNAP_TRY(s->sweep(precision, nullptr));  // first launches load the code object

@@ -183,19 +183,33 @@ int launch_convergence(Acc3 flux0, Acc3 flux0p, Extent3 e, int ng, double epsi, 
 
 // Sum of the eight octant partials in octant order 0..7.
 __global__ void k_octant_reduce(const double *__restrict__ part, double *__restrict__ out,
-                                int64_t n) {
+                                const double *__restrict__ base, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     double acc = 0.0;
 #pragma unroll
     for (int o = 0; o < 8; o++) acc += part[(int64_t)o * n + i];
-    out[i] = acc;
+    out[i] = base ? base[i] + acc : acc;
   }
 }
 
-int launch_octant_reduce(const double *part, double *out, int64_t n, hipStream_t st) {
+int launch_octant_reduce(const double *part, double *out, int64_t n, hipStream_t st,
+                         const double *base) {
   if (n <= 0) return SNAP_OK;
-  hipLaunchKernelGGL(k_octant_reduce, dim3(grid_for(n, 256)), dim3(256), 0, st, part, out, n);
+  hipLaunchKernelGGL(k_octant_reduce, dim3(grid_for(n, 256)), dim3(256), 0, st, part, out, base, n);
+  SNAP_HIP_TRY(hipGetLastError());
+  return SNAP_OK;
+}
+
+__global__ void k_subtract(double *__restrict__ y, const double *__restrict__ x, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = y[i] - x[i];
+}
+
+int launch_subtract(double *y, const double *x, int64_t n, hipStream_t st) {
+  if (n <= 0) return SNAP_OK;
+  hipLaunchKernelGGL(k_subtract, dim3(grid_for(n, 256)), dim3(256), 0, st, y, x, n);
   SNAP_HIP_TRY(hipGetLastError());
   return SNAP_OK;
 }

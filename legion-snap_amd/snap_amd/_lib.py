@@ -79,7 +79,7 @@ class SolveOptions(ctypes.Structure):
     _fields_ = [
         ("precision", ctypes.c_int), ("tile", ctypes.c_int * 3),
         ("waves_per_tile", ctypes.c_int), ("first_sweep_fp64", ctypes.c_int),
-        ("dinv_mode", ctypes.c_int), ("angles_per_lane", ctypes.c_int),
+        ("dinv_mode", ctypes.c_int), ("angles_per_lane", ctypes.c_int), ("refine", ctypes.c_int),
     ]
 
 

@@ -33,11 +33,14 @@ class Solver:
     precision: "fp64" (FP64 sweeps with fused multiply-adds), "fp64_exact"
     (FP64 sweeps in the reference CPU operation order, bit-equal to the CPU
     arithmetic) or "mixed" (FP32 sweeps, FP64 moments, sources and tests).
+    refine (mixed only): defect correction per outer iteration -- the first
+    inner sweep runs in FP64 and is kept as the base; later inners sweep the
+    FP64 source defect in FP32 without the fixup and add it to the base.
     """
 
     def __init__(self, problem: Problem, precision: str = FP64, tile=(0, 0, 0),
                  waves_per_tile: int = 0, first_sweep_fp64: bool = False,
-                 dinv: str = "default", angles_per_lane: int = 0):
+                 dinv: str = "default", angles_per_lane: int = 0, refine: bool = False):
         problem.validate()
         self.problem = p = problem.contiguous()
         self.lib = _lib.load()
@@ -59,6 +62,7 @@ class Solver:
         o.dinv_mode = {"default": _lib.DINV_DEFAULT, "stream": _lib.DINV_STREAM,
                        "recompute": _lib.DINV_RECOMPUTE}[dinv]
         o.angles_per_lane = angles_per_lane
+        o.refine = int(bool(refine))
         self.precision = precision
         h = ctypes.c_void_p()
         _lib.check(self.lib.snap_hip_solver_create(ctypes.byref(s), ctypes.byref(o), ctypes.byref(h)),

@@ -101,6 +101,18 @@ def test_angles_per_lane_keep_exact_results(gpu, oracle_mod, name, apl):
         assert np.array_equal(fm, fmo)
 
 
+def test_mixed_defect_correction(gpu, oracle_mod):
+    # config 0 has few fixed cells: sweeping only the FP64 source defect in
+    # FP32 brings the FP32 error from ~1e-6 down by orders of magnitude
+    fo, _, sto = oracle_solution(oracle_mod, "config0")
+    f, _, st = gpu_solve(CASES["config0"](), "mixed", refine=True)
+    mx, l2 = rel_errors(f, fo)
+    print("config0 mixed refine", f"maxrel={mx:.3e} l2rel={l2:.3e}", st)
+    assert (st["inner_total"], st["outer_total"]) == (sto["inner_total"], sto["outer_total"])
+    assert st["sweeps_fp64"] == st["outer_total"]
+    assert mx <= 1e-8
+
+
 def test_dinv_modes_agree(gpu):
     # forming dinv in the kernel is the same arithmetic as the geometry kernel
     p = CASES["config1_small"]()
