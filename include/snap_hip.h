@@ -234,6 +234,8 @@ typedef struct {
   int angles_per_lane;      /* sweep angles per GPU lane, 1 or 2; 0 = choose */
   int refine;               /* MIXED: 1 = defect correction (below); 0 = FP32
                                sweeps of the whole source */
+  int outer_ordered;        /* 1 = outer source in the CPU's summation order even
+                               when not FP64_EXACT (default: FP64 matrix cores) */
 } snap_solve_options_t;
 
 /* MIXED with refine = 1 (iterative refinement of each outer iteration): the

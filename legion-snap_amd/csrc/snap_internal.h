@@ -127,6 +127,12 @@ int launch_inner_source(Acc3 sxs, Acc3 flux0, Acc3 q2grp0, Acc3 fluxm, Acc3 q2gr
                         Extent3 e, int ng, int cmom, const int *lma, int nmom, hipStream_t st);
 int launch_convergence(Acc3 flux0, Acc3 flux0p, Extent3 e, int ng, double epsi, int32_t *result,
                        hipStream_t st);
+// Outer source of the solver's dense layout: ordered = the CPU's summation
+// order (one thread per cell and moment), else FP64 matrix cores.
+int launch_outer_dense(const double *qi, const double *flux0, const double *fluxm,
+                       const double *slgg, const int32_t *mat, double *q0, double *qm,
+                       int64_t cells, int ng, int nmat, int nmom, bool ordered, hipStream_t st);
+size_t outer_mfma_lds(int ng, int cmom);
 // out = (base ? base : 0) + sum over the 8 octant partials, octants in order
 int launch_octant_reduce(const double *part, double *out, int64_t n, hipStream_t st,
                          const double *base = nullptr);

@@ -127,19 +127,14 @@ struct snap_solver {
   }
 
   int outer_source() {
-    const int lma[4] = {1, 3, 5, 7};
-    Acc2 sl;
-    sl.base = slgg.as<char>();
-    sl.s1 = (int64_t)nmom * 8;
-    sl.s0 = sl.s1 * ng;
-    sl.sg = sl.s0 * nmat;
-    const Acc3 m = g_acc(mat, 1, 4);
-    SNAP_TRY(launch_outer_source(g_acc(qi, 1, 8), g_acc(flux0, 1, 8), sl, g_acc(q2grp0, 1, 8), m,
-                                 e, ng, stream));
-    if (cmom > 1)
-      SNAP_TRY(launch_outer_source_m(g_acc(fluxm, cmom - 1, 8), sl, g_acc(q2grpm, cmom - 1, 8), m,
-                                     e, ng, cmom, lma, nmom, stream));
-    return SNAP_OK;
+    // the exact solve keeps the CPU's summation order; the others contract on
+    // the FP64 matrix cores
+    const bool ordered = opt.precision == SNAP_PRECISION_FP64_EXACT || opt.outer_ordered;
+    return launch_outer_dense(qi.as<double>(), flux0.as<double>(),
+                              cmom > 1 ? fluxm.as<double>() : nullptr, slgg.as<double>(),
+                              mat.as<int32_t>(), q2grp0.as<double>(),
+                              cmom > 1 ? q2grpm.as<double>() : nullptr, cells, ng, nmat, nmom,
+                              ordered, stream);
   }
 
   int inner_source() {

@@ -40,7 +40,8 @@ class Solver:
 
     def __init__(self, problem: Problem, precision: str = FP64, tile=(0, 0, 0),
                  waves_per_tile: int = 0, first_sweep_fp64: bool = False,
-                 dinv: str = "default", angles_per_lane: int = 0, refine: bool = False):
+                 dinv: str = "default", angles_per_lane: int = 0, refine: bool = False,
+                 outer_ordered: bool = False):
         problem.validate()
         self.problem = p = problem.contiguous()
         self.lib = _lib.load()
@@ -63,6 +64,7 @@ class Solver:
                        "recompute": _lib.DINV_RECOMPUTE}[dinv]
         o.angles_per_lane = angles_per_lane
         o.refine = int(bool(refine))
+        o.outer_ordered = int(bool(outer_ordered))
         self.precision = precision
         h = ctypes.c_void_p()
         _lib.check(self.lib.snap_hip_solver_create(ctypes.byref(s), ctypes.byref(o), ctypes.byref(h)),

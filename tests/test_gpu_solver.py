@@ -113,6 +113,20 @@ def test_mixed_defect_correction(gpu, oracle_mod):
     assert mx <= 1e-8
 
 
+@pytest.mark.parametrize("name", ["config1_small", "config1_ragged", "odd_shape"])
+def test_outer_source_matrix_cores(gpu, oracle_mod, name):
+    # the FP64 matrix-core outer source against the CPU-ordered one: the
+    # same products summed in another order
+    fo, fmo, sto = oracle_solution(oracle_mod, name)
+    f1, fm1, st1 = gpu_solve(CASES[name](), "fp64", outer_ordered=True)
+    f2, fm2, st2 = gpu_solve(CASES[name](), "fp64")
+    assert (st1["inner_total"], st1["outer_total"]) == (st2["inner_total"], st2["outer_total"])
+    mx, _ = rel_errors(f2, f1)
+    assert mx <= 1e-13
+    if fm1 is not None:
+        assert np.max(np.abs(fm2 - fm1)) <= 1e-13 * np.max(np.abs(fm1))
+
+
 def test_dinv_modes_agree(gpu):
     # forming dinv in the kernel is the same arithmetic as the geometry kernel
     p = CASES["config1_small"]()
