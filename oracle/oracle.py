@@ -74,6 +74,12 @@ def load():
         lib.snap_oracle_solve.argtypes = [ctypes.POINTER(OracleProblem), _dp, _dp,
                                           ctypes.POINTER(OracleStats), ctypes.c_int]
         lib.snap_oracle_solve.restype = ctypes.c_int
+        lib.snap_oracle_adjoint_sweep.argtypes = (
+            [ctypes.c_int] * 5 + [ctypes.c_double] * 3 + [_dp] * 6 + [ctypes.c_int] + [_dp] * 3)
+        lib.snap_oracle_adjoint_sweep.restype = None
+        lib.snap_oracle_adjoint.argtypes = [ctypes.POINTER(OracleProblem), _dp, _dp, ctypes.c_double,
+                                            ctypes.c_int, _dp, _dp, ctypes.c_int]
+        lib.snap_oracle_adjoint.restype = ctypes.c_int
         lib.snap_oracle_time_sweeps.argtypes = [ctypes.POINTER(OracleProblem), ctypes.c_int,
                                                 ctypes.c_int]
         lib.snap_oracle_time_sweeps.restype = ctypes.c_double
@@ -208,6 +214,40 @@ def solve(p, nthreads: int = 0):
     if rc:
         raise RuntimeError("oracle solve failed")
     return flux0, fluxm, st.as_dict()
+
+
+def adjoint_sweep(nx, ny, nz, nang, corner, h, quad, qtot_g, t_xs_g, fixup, zeta_g):
+    """One reverse sweep (nmom = 1): returns (u_g, grad_g) for the direct
+    response w_a * zeta_g of pc.  quad = (mu, eta, xi, w)."""
+    lib = load()
+    mu, eta, xi, w = (np.ascontiguousarray(a, dtype=np.float64) for a in quad)
+    q = np.ascontiguousarray(qtot_g, dtype=np.float64)
+    t = np.ascontiguousarray(t_xs_g, dtype=np.float64)
+    z = np.ascontiguousarray(zeta_g, dtype=np.float64)
+    u = np.zeros(nx * ny * nz)
+    g = np.zeros(nx * ny * nz)
+    lib.snap_oracle_adjoint_sweep(nx, ny, nz, nang, corner, h[0], h[1], h[2], _p(mu), _p(eta),
+                                  _p(xi), _p(w), _p(q), _p(t), int(fixup), _p(z), _p(u), _p(g))
+    return u, g
+
+
+def adjoint(p, flux0, resp, tol: float = 1e-13, maxit: int = 500, nthreads: int = 0):
+    """FP64 reference gradient dR/dt_xs[g, z, y, x] of R = sum(resp * flux0) at
+    the converged flux0 of p (nmom = 1), by adjoint source iteration.
+    Returns (grad, zeta, iterations)."""
+    lib = load()
+    p.validate()
+    keep = []
+    s = _problem_struct(p, keep)
+    f = np.ascontiguousarray(flux0, dtype=np.float64)
+    r = np.ascontiguousarray(resp, dtype=np.float64)
+    grad = np.zeros((p.ng, p.nz, p.ny, p.nx))
+    zeta = np.zeros((p.ng, p.nz, p.ny, p.nx))
+    it = lib.snap_oracle_adjoint(ctypes.byref(s), _p(f), _p(r), tol, maxit, _p(grad), _p(zeta),
+                                 nthreads)
+    if it < 0:
+        raise RuntimeError("oracle adjoint did not converge")
+    return grad, zeta, it
 
 
 def time_sweeps(p, ninner: int = 1, nthreads: int = 0) -> float:

@@ -568,3 +568,261 @@ double snap_oracle_time_sweeps(const oracle_problem_t *p, int ninner, int nthrea
   state_free(&s);
   return total;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint of the steady-state solve (no counterpart in the reference; this  */
+/* is the FP64 reference gradient the GPU adjoint is checked against)       */
+/* ------------------------------------------------------------------------ */
+
+/* Per cell-angle state of a forward sweep that the reverse pass needs. */
+enum { AF_OUTX = 1, AF_OUTY = 2, AF_OUTZ = 4, AF_PCX = 8, AF_PCY = 16, AF_PCZ = 32,
+       AF_FIXED = 64, AF_ZERO = 128 };
+
+/* Reverse-mode derivative of one (corner, group) sweep with isotropic
+ * source qtot_g (nmom = 1).  The forward cell equations are
+ *   pc = N / D,  N = q + sum_d h_d k_d psi_in,d,  D = t_xs + sum_d h_d v_d
+ *   psi_out,d = (2 pc - psi_in,d) o_d
+ * with h_d = (mu, eta, xi) * (hi, hj, hk); plain diamond difference has
+ * k = v = o = 1 (D = 1/dinv); a fixed cell has k_d = (1 + v_d)/2 with the
+ * pc-pass flags v_d, outflow masks o_d and pc = 0 when zeroed.  The fixup's
+ * decisions are those of the forward state (held fixed).  The direct
+ * response of pc is w_a zeta_g[c].  Adds sum_a dR/dq to u_g[c] and
+ * sum_a dR/dt_xs to grad_g[c]. */
+void snap_oracle_adjoint_sweep(int nx, int ny, int nz, int nang, int corner, double hi,
+                               double hj, double hk, const double *mu, const double *eta,
+                               const double *xi, const double *w, const double *qtot_g,
+                               const double *t_xs_g, int fixup, const double *zeta_g,
+                               double *u_g, double *grad_g) {
+  const int sxp = (corner & 1) != 0, syp = (corner & 2) != 0, szp = (corner & 4) != 0;
+  const size_t ab = (size_t)nang, nc = (size_t)nx * ny * nz;
+  double *pcs = (double *)malloc(sizeof(double) * nc * ab);
+  double *dens = (double *)malloc(sizeof(double) * nc * ab);
+  unsigned char *fl = (unsigned char *)malloc(nc * ab);
+  double *fx = (double *)calloc(ab, sizeof(double));
+  double *fyp = (double *)calloc(ab * nx, sizeof(double));
+  double *fzp = (double *)calloc(ab * nx * ny, sizeof(double));
+  double *pc = (double *)malloc(sizeof(double) * ab * 4);
+  double *in = pc + ab; /* 3 x nang inflows */
+  const double tolr = 1.0e-12;
+  /* ---- forward pass in sweep order, as snap_oracle_sweep */
+  for (int z = 0; z < nz; z++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        const int px = sxp ? x : nx - 1 - x, py = syp ? y : ny - 1 - y, pz = szp ? z : nz - 1 - z;
+        const size_t c = ((size_t)pz * ny + py) * nx + px;
+        const size_t t = ((size_t)z * ny + y) * nx + x; /* sweep-order index */
+        const double q = qtot_g[c], sig = t_xs_g[c];
+        double *pi = in, *pj = in + ab, *pk = in + 2 * ab;
+        for (int a = 0; a < nang; a++) {
+          pi[a] = (x == 0) ? 0.0 : fx[a];
+          pj[a] = (y == 0) ? 0.0 : fyp[(size_t)x * ab + a];
+          pk[a] = (z == 0) ? 0.0 : fzp[((size_t)y * nx + x) * ab + a];
+        }
+        double hvx[256], hvy[256], hvz[256], pvx[256], pvy[256], pvz[256];
+        int fixed = 0;
+        for (int a = 0; a < nang; a++) {
+          const double d = 1.0 / (((sig + hi * mu[a]) + hj * eta[a]) + hk * xi[a]);
+          pc[a] = (((q + pi[a] * mu[a] * hi) + pj[a] * eta[a] * hj) + pk[a] * xi[a] * hk) * d;
+          dens[t * ab + a] = 1.0 / d;
+          hvx[a] = hvy[a] = hvz[a] = 1.0;
+          pvx[a] = pvy[a] = pvz[a] = 1.0;
+          fl[t * ab + a] = AF_OUTX | AF_OUTY | AF_OUTZ;
+        }
+        if (fixup) {
+          unsigned old_neg = 0;
+          for (;;) {
+            unsigned neg = 0;
+            for (int a = 0; a < nang; a++) {
+              if (2.0 * pc[a] - pi[a] < 0.0) { hvx[a] = 0.0; neg++; }
+              if (2.0 * pc[a] - pj[a] < 0.0) { hvy[a] = 0.0; neg++; }
+              if (2.0 * pc[a] - pk[a] < 0.0) { hvz[a] = 0.0; neg++; }
+            }
+            if (neg == old_neg) break;
+            old_neg = neg;
+            fixed = 1;
+            for (int a = 0; a < nang; a++) {
+              pvx[a] = hvx[a];
+              pvy[a] = hvy[a];
+              pvz[a] = hvz[a];
+              const double p = q + 0.5 * (pi[a] * mu[a] * hi * (1.0 + hvx[a]) +
+                                          pj[a] * eta[a] * hj * (1.0 + hvy[a]) +
+                                          pk[a] * xi[a] * hk * (1.0 + hvz[a]));
+              const double den = (p <= 0.0) ? 0.0
+                                             : (sig + mu[a] * hi * hvx[a] + eta[a] * hj * hvy[a] +
+                                                xi[a] * hk * hvz[a]);
+              dens[t * ab + a] = sig + mu[a] * hi * hvx[a] + eta[a] * hj * hvy[a] + xi[a] * hk * hvz[a];
+              if (den < tolr) {
+                pc[a] = 0.0;
+                fl[t * ab + a] |= AF_ZERO;
+              } else {
+                pc[a] = p / den;
+                fl[t * ab + a] &= (unsigned char)~AF_ZERO;
+              }
+            }
+          }
+        }
+        for (int a = 0; a < nang; a++) {
+          unsigned char f = fl[t * ab + a] & AF_ZERO;
+          if (hvx[a] != 0.0) f |= AF_OUTX;
+          if (hvy[a] != 0.0) f |= AF_OUTY;
+          if (hvz[a] != 0.0) f |= AF_OUTZ;
+          if (fixed) {
+            f |= AF_FIXED;
+            if (pvx[a] != 0.0) f |= AF_PCX;
+            if (pvy[a] != 0.0) f |= AF_PCY;
+            if (pvz[a] != 0.0) f |= AF_PCZ;
+          }
+          fl[t * ab + a] = f;
+          pcs[t * ab + a] = pc[a];
+          fx[a] = (2.0 * pc[a] - pi[a]) * hvx[a];
+          fyp[(size_t)x * ab + a] = (2.0 * pc[a] - pj[a]) * hvy[a];
+          fzp[((size_t)y * nx + x) * ab + a] = (2.0 * pc[a] - pk[a]) * hvz[a];
+        }
+      }
+  /* ---- reverse pass: adjoint faces flow against the sweep direction */
+  memset(fx, 0, sizeof(double) * ab);
+  memset(fyp, 0, sizeof(double) * ab * nx);
+  memset(fzp, 0, sizeof(double) * ab * nx * ny);
+  for (int z = nz - 1; z >= 0; z--)
+    for (int y = ny - 1; y >= 0; y--)
+      for (int x = nx - 1; x >= 0; x--) {
+        const int px = sxp ? x : nx - 1 - x, py = syp ? y : ny - 1 - y, pz = szp ? z : nz - 1 - z;
+        const size_t c = ((size_t)pz * ny + py) * nx + px;
+        const size_t t = ((size_t)z * ny + y) * nx + x;
+        double su = 0.0, sg = 0.0;
+        for (int a = 0; a < nang; a++) {
+          const unsigned f = fl[t * ab + a];
+          /* adjoints of this cell's outflows: the downstream cells' inflow
+           * adjoints (zero past the far boundary) */
+          const double bx = (x == nx - 1) ? 0.0 : fx[a];
+          const double by = (y == ny - 1) ? 0.0 : fyp[(size_t)x * ab + a];
+          const double bz = (z == nz - 1) ? 0.0 : fzp[((size_t)y * nx + x) * ab + a];
+          const double ox = (f & AF_OUTX) ? 1.0 : 0.0, oy = (f & AF_OUTY) ? 1.0 : 0.0,
+                       oz = (f & AF_OUTZ) ? 1.0 : 0.0;
+          const double bpc = w[a] * zeta_g[c] + 2.0 * (ox * bx + oy * by + oz * bz);
+          double bn = 0.0, bd = 0.0;
+          if (!(f & AF_ZERO)) {
+            const double den = dens[t * ab + a];
+            bn = bpc / den;
+            bd = -bpc * pcs[t * ab + a] / den;
+          }
+          double kx = 1.0, ky = 1.0, kz = 1.0;
+          if (f & AF_FIXED) {
+            kx = (f & AF_PCX) ? 1.0 : 0.5;
+            ky = (f & AF_PCY) ? 1.0 : 0.5;
+            kz = (f & AF_PCZ) ? 1.0 : 0.5;
+          }
+          fx[a] = -ox * bx + bn * mu[a] * hi * kx;
+          fyp[(size_t)x * ab + a] = -oy * by + bn * eta[a] * hj * ky;
+          fzp[((size_t)y * nx + x) * ab + a] = -oz * bz + bn * xi[a] * hk * kz;
+          su += bn;
+          sg += bd;
+        }
+        u_g[c] += su;
+        grad_g[c] += sg;
+      }
+  free(pcs);
+  free(dens);
+  free(fl);
+  free(fx);
+  free(fyp);
+  free(fzp);
+  free(pc);
+}
+
+/* Gradient of R = sum_g sum_c resp[g][c] * flux0[g][c] at the converged
+ * solution flux0 of problem p (nmom = 1) with respect to t_xs[g][c].
+ * Adjoint source iteration zeta = resp + S^T u(zeta), u = sum over the 8*ng
+ * adjoint sweeps, until max|dzeta| <= tol * max|zeta|.  Returns the number
+ * of adjoint iterations, or -1. */
+int snap_oracle_adjoint(const oracle_problem_t *p, const double *flux0, const double *resp,
+                        double tol, int maxit, double *grad, double *zeta_out, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  if (p->nmom != 1 || p->nang > 256) return -1;
+  oracle_state_t s;
+  if (state_init(&s, p)) {
+    state_free(&s);
+    return -1;
+  }
+  const int ng = s.ng, nc = s.ncells;
+  const size_t gc = (size_t)ng * nc;
+  double *q2 = (double *)calloc(gc, sizeof(double));
+  double *qtot = (double *)calloc(gc, sizeof(double));
+  double *zeta = (double *)malloc(sizeof(double) * gc);
+  double *zn = (double *)malloc(sizeof(double) * gc);
+  double *u = (double *)malloc(sizeof(double) * 8 * gc);
+  double *gp = (double *)malloc(sizeof(double) * 8 * gc);
+  /* the forward source of the converged flux: q = qi + S flux0 */
+  snap_oracle_outer_source(nc, ng, p->nmat, 1, p->mat, p->qi, p->slgg, flux0, NULL, q2, NULL);
+  snap_oracle_inner_source(nc, ng, 1, s.s_xs, flux0, q2, NULL, NULL, qtot);
+  memcpy(zeta, resp, sizeof(double) * gc);
+  int it = 0, done = 0;
+  for (; it < maxit && !done; it++) {
+    memset(u, 0, sizeof(double) * 8 * gc);
+    memset(gp, 0, sizeof(double) * 8 * gc);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int task = 0; task < 8 * ng; task++) {
+      const int corner = task / ng, g = task % ng;
+      snap_oracle_adjoint_sweep(s.nx, s.ny, s.nz, s.nang, corner, s.hi, s.hj, s.hk, s.mu, s.eta,
+                                s.xi, s.w, qtot + (size_t)g * nc, s.t_xs + (size_t)g * nc,
+                                p->fixup, zeta + (size_t)g * nc,
+                                u + (size_t)corner * gc + (size_t)g * nc,
+                                gp + (size_t)corner * gc + (size_t)g * nc);
+    }
+    /* octant partials in order; zeta_new[g2] = resp[g2] + sum_g S[g][mat][g2] u[g] */
+    double dmax = 0.0, zmax = 0.0;
+    for (int g2 = 0; g2 < ng; g2++)
+      for (int c = 0; c < nc; c++) {
+        const int mt = p->mat[c];
+        double acc = resp[(size_t)g2 * nc + c];
+        for (int g = 0; g < ng; g++) {
+          double ug = 0.0;
+          for (int o = 0; o < 8; o++) ug += u[(size_t)o * gc + (size_t)g * nc + c];
+          acc += p->slgg[(((size_t)g * p->nmat + mt) * ng + g2)] * ug;
+        }
+        zn[(size_t)g2 * nc + c] = acc;
+        const double d = fabs(acc - zeta[(size_t)g2 * nc + c]);
+        if (d > dmax) dmax = d;
+        if (fabs(acc) > zmax) zmax = fabs(acc);
+      }
+    for (size_t i = 0; i < gc; i++) {
+      double acc = 0.0;
+      for (int o = 0; o < 8; o++) acc += gp[(size_t)o * gc + i];
+      grad[i] = acc;
+    }
+    done = dmax <= tol * zmax;
+    double *tmp = zeta;
+    zeta = zn;
+    zn = tmp;
+  }
+  /* grad holds the sweep with the last zeta before the update; one more
+   * sweep evaluates it at the converged zeta */
+  memset(gp, 0, sizeof(double) * 8 * gc);
+  memset(u, 0, sizeof(double) * 8 * gc);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int task = 0; task < 8 * ng; task++) {
+    const int corner = task / ng, g = task % ng;
+    snap_oracle_adjoint_sweep(s.nx, s.ny, s.nz, s.nang, corner, s.hi, s.hj, s.hk, s.mu, s.eta,
+                              s.xi, s.w, qtot + (size_t)g * nc, s.t_xs + (size_t)g * nc, p->fixup,
+                              zeta + (size_t)g * nc, u + (size_t)corner * gc + (size_t)g * nc,
+                              gp + (size_t)corner * gc + (size_t)g * nc);
+  }
+  for (size_t i = 0; i < gc; i++) {
+    double acc = 0.0;
+    for (int o = 0; o < 8; o++) acc += gp[(size_t)o * gc + i];
+    grad[i] = acc;
+  }
+  if (zeta_out) memcpy(zeta_out, zeta, sizeof(double) * gc);
+  free(q2);
+  free(qtot);
+  free(zeta);
+  free(zn);
+  free(u);
+  free(gp);
+  state_free(&s);
+  return done ? it : -1;
+}

@@ -107,6 +107,19 @@ int snap_oracle_solve(const oracle_problem_t *p, double *flux0, double *fluxm,
 
 /* Time `ninner` inner iterations' sweeps (all octants and groups) of problem
  * p starting from a zero flux: the CPU baseline sample. Returns seconds. */
+/* Reverse-mode derivative of one sweep (nmom = 1) with the forward fixup
+ * decisions held fixed: adds sum_a dR/dq to u_g and sum_a dR/dt_xs to grad_g
+ * for the direct response w_a * zeta_g[c] of pc. */
+void snap_oracle_adjoint_sweep(int nx, int ny, int nz, int nang, int corner, double hi,
+                               double hj, double hk, const double *mu, const double *eta,
+                               const double *xi, const double *w, const double *qtot_g,
+                               const double *t_xs_g, int fixup, const double *zeta_g,
+                               double *u_g, double *grad_g);
+/* dR/dt_xs[g][c] of R = sum resp[g][c] * flux0[g][c] at the converged flux0
+ * (nmom = 1), by adjoint source iteration to tol.  Returns iterations or -1. */
+int snap_oracle_adjoint(const oracle_problem_t *p, const double *flux0, const double *resp,
+                        double tol, int maxit, double *grad, double *zeta_out, int nthreads);
+
 double snap_oracle_time_sweeps(const oracle_problem_t *p, int ninner,
                                int nthreads);
 
