@@ -275,6 +275,18 @@ int snap_hip_solver_get_flux(snap_solver_t *solver, double *flux0, double *fluxm
 int snap_hip_solver_time_sweeps(snap_solver_t *solver, int count, int precision,
                                 double *ms_per_sweep, double *ms_per_launch,
                                 int *launches_per_sweep);
+/* Gradient of a detector response R = sum_g sum_c resp[g][c] * flux0[g][c]
+ * at the solver's current (converged) flux with respect to the total cross
+ * section t_xs[g][c], for nmom = 1.  No counterpart in the reference (the
+ * adjoint-sweep pattern; callers that differentiate a solve).
+ * precision: SNAP_PRECISION_FP64, or SNAP_PRECISION_MIXED for FP32 forward
+ * and reverse sweeps.  Runs one forward sweep of the current flux's source
+ * (its tile faces are the checkpoint), then reverse sweeps of the adjoint
+ * source iteration until max|d zeta| <= tol * max|zeta|.  resp and grad are
+ * host arrays [ng][nz][ny][nx]; response (optional) receives R. */
+int snap_hip_solver_gradient(snap_solver_t *solver, const double *resp, int precision,
+                             double tol, int maxit, double *grad, double *response,
+                             int *iterations);
 /* The sweep plan the solver chose: plan[0..2] tile edges (x,y,z), plan[3]
  * waves per tile workgroup, plan[4] angles per lane, plan[5] cells per
  * wavefront. */

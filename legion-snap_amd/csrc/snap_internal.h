@@ -133,6 +133,13 @@ int launch_outer_dense(const double *qi, const double *flux0, const double *flux
                        const double *slgg, const int32_t *mat, double *q0, double *qm,
                        int64_t cells, int ng, int nmat, int nmom, bool ordered, hipStream_t st);
 size_t outer_mfma_lds(int ng, int cmom);
+// zeta_new = r + S^T u and norms[0] = max|zeta_new - zeta|, norms[1] = max|zeta_new|
+// (as the bit patterns of the non-negative doubles)
+int launch_adjoint_update(const double *r, const double *u, const double *slgg,
+                          const int32_t *mat, const double *zeta, double *zn,
+                          unsigned long long *norms, int64_t cells, int ng, int nmat, int nmom,
+                          hipStream_t st);
+int launch_to_float(const double *a, float *b, int64_t n, hipStream_t st);
 // out = (base ? base : 0) + sum over the 8 octant partials, octants in order
 int launch_octant_reduce(const double *part, double *out, int64_t n, hipStream_t st,
                          const double *base = nullptr);
@@ -198,5 +205,13 @@ struct SweepLaunch {
 int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events = nullptr);
 // Quadrature of the drop-in context on device `dev` (NULL before init).
 const QuadDev *context_quad(int dev);
+
+// Reverse sweep of all 8*ng (corner, group) sweeps on plan p (see
+// snap_adjoint.hip); faces = the forward sweep's tile faces on the same plan.
+int adjoint_enqueue(const SweepPlan &p, const QuadDev &quad, int ng, int fixup, double hi,
+                    double hj, double hk, const double *q, const double *txs, const double *zeta,
+                    double *part_u, double *part_grad, const void *faces, void *adj_faces,
+                    bool fp32, hipStream_t st);
+int adjoint_plan_fits(const SweepPlan &p, int nang, int real_bytes, int *cpw, int *waves, int *sb);
 
 }  // namespace snap

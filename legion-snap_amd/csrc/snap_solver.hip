@@ -64,6 +64,7 @@ struct snap_solver {
   DeviceBuffer flux0, flux0pi, flux0po, fluxm, q2grp0, q2grpm, qtot;
   DeviceBuffer part_flux, part_fluxm, faces, flag;
   DeviceBuffer qbase, flux0base, fluxmbase;  // defect correction: the outer's FP64 base
+  DeviceBuffer adj_faces, part_grad;          // adjoint: reverse-sweep faces and partials
   bool have_fp32 = false;
 
   ~snap_solver() {
@@ -88,7 +89,7 @@ struct snap_solver {
     return (precision != SNAP_PRECISION_MIXED && plan64.nplanes) ? plan64 : plan;
   }
 
-  int sweep(int precision, hipEvent_t *events, bool fixup = true) {
+  int sweep(int precision, hipEvent_t *events, bool fixup = true, bool recompute = false) {
     const bool fp32 = precision == SNAP_PRECISION_MIXED;
     SweepLaunch L;
     memset(&L, 0, sizeof(L));
@@ -110,7 +111,7 @@ struct snap_solver {
     L.faces = faces.p;
     L.fp32 = fp32;
     L.exact = precision == SNAP_PRECISION_FP64_EXACT;
-    L.recompute_dinv = opt.dinv_mode == SNAP_DINV_RECOMPUTE;
+    L.recompute_dinv = recompute || opt.dinv_mode == SNAP_DINV_RECOMPUTE;
     if (fp32 && !L.recompute_dinv && !dinv32.p) return fail(SNAP_ERR_INVALID, "no FP32 dinv");
     return sweep_enqueue(L, stream, events);
   }
@@ -392,6 +393,85 @@ extern "C" int snap_hip_solver_time_sweeps(snap_solver_t *s, int count, int prec
   if (ms_per_sweep) *ms_per_sweep = total / count;
   if (ms_per_launch) *ms_per_launch = launch_sum / (double)(np * count);
   if (launches_per_sweep) *launches_per_sweep = np;
+  return SNAP_OK;
+}
+
+extern "C" int snap_hip_solver_gradient(snap_solver_t *s, const double *resp, int precision,
+                                        double tol, int maxit, double *grad, double *response,
+                                        int *iterations) {
+  if (!s || !resp || !grad) return fail(SNAP_ERR_INVALID, "NULL argument");
+  if (s->nmom != 1) return fail(SNAP_ERR_UNSUPPORTED, "the adjoint handles nmom = 1 (isotropic scattering)");
+  if (precision != SNAP_PRECISION_FP64 && precision != SNAP_PRECISION_MIXED)
+    return fail(SNAP_ERR_INVALID, "adjoint precision must be FP64 or MIXED (FP32 sweeps)");
+  if (!(tol > 0.0) || maxit < 1) return fail(SNAP_ERR_INVALID, "bad tolerance or iteration cap");
+  const bool fp32 = precision == SNAP_PRECISION_MIXED;
+  const SweepPlan &pl = s->plan_for(precision);
+  {
+    int cpw, waves, sb;
+    SNAP_TRY(adjoint_plan_fits(pl, s->nang, fp32 ? 4 : 8, &cpw, &waves, &sb));
+  }
+  const size_t gc = (size_t)s->ng * s->cells;
+  const hipStream_t st = s->stream;
+  DeviceBuffer r, zeta, zn, u, gsum, norms;
+  SNAP_TRY(r.alloc(sizeof(double) * gc));
+  SNAP_TRY(zeta.alloc(sizeof(double) * gc));
+  SNAP_TRY(zn.alloc(sizeof(double) * gc));
+  SNAP_TRY(u.alloc(sizeof(double) * gc));
+  SNAP_TRY(gsum.alloc(sizeof(double) * gc));
+  SNAP_TRY(norms.alloc(2 * sizeof(unsigned long long)));
+  if (!s->part_grad.p) SNAP_TRY(s->part_grad.alloc(sizeof(double) * 8 * gc));
+  if (!s->adj_faces.p) SNAP_TRY(s->adj_faces.alloc(s->faces.bytes));
+  if (fp32 && !s->txs32.p) {
+    SNAP_TRY(s->txs32.alloc(sizeof(float) * gc));
+    SNAP_TRY(launch_to_float(s->txs.as<double>(), s->txs32.as<float>(), (int64_t)gc, st));
+  }
+  SNAP_HIP_TRY(hipMemcpyAsync(r.p, resp, sizeof(double) * gc, hipMemcpyHostToDevice, st));
+  // forward source of the current flux, q = qi + S flux0, and one forward
+  // sweep of it: its tile faces are the checkpoint the reverse sweeps start from
+  SNAP_TRY(s->outer_source());
+  SNAP_TRY(s->inner_source());
+  SNAP_TRY(s->sweep(precision, nullptr, true, true));
+  SNAP_HIP_TRY(hipMemcpyAsync(zeta.p, r.p, sizeof(double) * gc, hipMemcpyDeviceToDevice, st));
+  auto reverse = [&](const DeviceBuffer &z) -> int {
+    SNAP_TRY(adjoint_enqueue(pl, s->quad, s->ng, s->prob.fixup ? 1 : 0, s->hi, s->hj, s->hk,
+                             s->qtot.as<double>(), s->txs.as<double>(), z.as<double>(),
+                             s->part_flux.as<double>(), s->part_grad.as<double>(), s->faces.p,
+                             s->adj_faces.p, fp32, st));
+    return launch_octant_reduce(s->part_flux.as<double>(), u.as<double>(), (int64_t)gc, st);
+  };
+  int it = 0;
+  bool done = false;
+  while (!done && it < maxit) {
+    SNAP_TRY(reverse(zeta));
+    SNAP_TRY(launch_adjoint_update(r.as<double>(), u.as<double>(), s->slgg.as<double>(),
+                                   s->mat.as<int32_t>(), zeta.as<double>(), zn.as<double>(),
+                                   norms.as<unsigned long long>(), s->cells, s->ng, s->nmat,
+                                   s->nmom, st));
+    unsigned long long h[2];
+    SNAP_HIP_TRY(hipMemcpyAsync(h, norms.p, sizeof(h), hipMemcpyDeviceToHost, st));
+    SNAP_HIP_TRY(hipStreamSynchronize(st));
+    double dmax, zmax;
+    memcpy(&dmax, &h[0], sizeof(double));
+    memcpy(&zmax, &h[1], sizeof(double));
+    std::swap(zeta.p, zn.p);
+    it++;
+    done = dmax <= tol * zmax;
+  }
+  // the gradient at the converged adjoint
+  SNAP_TRY(reverse(zeta));
+  SNAP_TRY(launch_octant_reduce(s->part_grad.as<double>(), gsum.as<double>(), (int64_t)gc, st));
+  SNAP_HIP_TRY(hipMemcpyAsync(grad, gsum.p, sizeof(double) * gc, hipMemcpyDeviceToHost, st));
+  if (response) {
+    std::vector<double> f(gc);
+    SNAP_HIP_TRY(hipMemcpyAsync(f.data(), s->flux0.p, sizeof(double) * gc, hipMemcpyDeviceToHost, st));
+    SNAP_HIP_TRY(hipStreamSynchronize(st));
+    double acc = 0.0;
+    for (size_t i = 0; i < gc; i++) acc += resp[i] * f[i];
+    *response = acc;
+  }
+  SNAP_HIP_TRY(hipStreamSynchronize(st));
+  if (iterations) *iterations = it;
+  if (!done) return fail(SNAP_ERR_INVALID, "adjoint iteration did not reach %g in %d steps", tol, maxit);
   return SNAP_OK;
 }
 

@@ -139,6 +139,9 @@ SIGNATURES = {
     "snap_hip_solver_time_sweeps": (
         [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "snap_hip_solver_plan": ([_vp, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+    "snap_hip_solver_gradient": (
+        [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp,
+         ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "snap_hip_solver_destroy": ([_vp], None),
 }
 

@@ -94,6 +94,25 @@ class Solver:
                    "snap_hip_solver_time_sweeps")
         return {"ms_per_sweep": ms.value, "ms_per_launch": ml.value, "launches_per_sweep": nl.value}
 
+    def gradient(self, resp: np.ndarray, precision: Optional[str] = None,
+                 tol: Optional[float] = None, maxit: int = 2000):
+        """dR/dt_xs[g, z, y, x] of R = sum(resp * flux0) at the current flux
+        (run() first; nmom = 1).  precision "fp64" or "mixed" (FP32 sweeps);
+        tol defaults to 1e-12 (FP64) or 1e-6 (FP32, whose rounding stalls the
+        adjoint iteration near 1e-7).  Returns (grad, R, adjoint iterations)."""
+        p = self.problem
+        r = np.ascontiguousarray(resp, dtype=np.float64).reshape(p.ng, p.nz, p.ny, p.nx)
+        grad = np.zeros_like(r)
+        R = ctypes.c_double()
+        it = ctypes.c_int()
+        prec = _PREC[precision or ("mixed" if self.precision == "mixed" else "fp64")]
+        if tol is None:
+            tol = 1e-6 if prec == _lib.PRECISION_MIXED else 1e-12
+        _lib.check(self.lib.snap_hip_solver_gradient(self._h, _ptr(r), prec, tol, maxit, _ptr(grad),
+                                                     ctypes.byref(R), ctypes.byref(it)),
+                   "snap_hip_solver_gradient")
+        return grad, R.value, it.value
+
     def plan(self) -> dict:
         """The sweep plan the solver chose (tile edges, waves per tile,
         angles per lane, cells per wavefront)."""
