@@ -293,6 +293,52 @@ int snap_hip_solver_gradient(snap_solver_t *solver, const double *resp, int prec
 int snap_hip_solver_plan(snap_solver_t *solver, int32_t plan[6]);
 void snap_hip_solver_destroy(snap_solver_t *solver);
 
+/* ------------------------------------------------------------------------
+ * Batches of independent problems (no counterpart in the reference; for
+ * callers with many small fixed-source problems of uneven size).  The
+ * problems must share ng, nang, nmom, epsi, iitm, oitm and fixup; meshes,
+ * materials and sources may differ.  Each problem follows its own iteration
+ * control exactly as snap_hip_solver_run does; the batch stops when every
+ * problem has converged or reached oitm.
+ * ------------------------------------------------------------------------ */
+typedef struct snap_batch snap_batch_t;
+
+typedef struct {
+  int precision;            /* SNAP_PRECISION_FP64 or SNAP_PRECISION_MIXED */
+  int tile[3];              /* sweep tile edges; 0 = autotune on the batch */
+  int waves_per_tile;       /* 0 = choose */
+  int angles_per_lane;      /* 0 = choose (autotuned with the tile) */
+  int use_graph;            /* 1 = capture one outer iteration in a hipGraph and replay it */
+} snap_batch_options_t;
+
+typedef struct {
+  int inner_total;          /* inner iterations of this problem */
+  int outer_total;          /* outer iterations of this problem */
+  int converged;
+  double time_ms;           /* from the start of the run to the end of the outer
+                               iteration in which it converged */
+} snap_problem_stats_t;
+
+typedef struct {
+  int outer_iterations;     /* outer iterations run by the batch */
+  int converged;            /* problems converged */
+  double solve_ms;          /* wall time of the run */
+  double units;             /* sum over problems of cells * 8*nang * ng * inner iterations */
+  double grind_ns;          /* solve_ms / units, in ns */
+  int tile[3];              /* tile shape used */
+  int waves;
+  int angles_per_lane;
+  double autotune_ms;       /* time the tile autotuner took at creation */
+} snap_batch_stats_t;
+
+int snap_hip_batch_create(const snap_problem_t *problems, int count,
+                          const snap_batch_options_t *options, snap_batch_t **batch);
+int snap_hip_batch_run(snap_batch_t *batch, snap_batch_stats_t *stats,
+                       snap_problem_stats_t *per_problem /* [count] or NULL */);
+/* flux0 [ng][cells of the problem], fluxm [ng][cells][cmom-1] (either NULL) */
+int snap_hip_batch_get_flux(snap_batch_t *batch, int problem, double *flux0, double *fluxm);
+void snap_hip_batch_destroy(snap_batch_t *batch);
+
 #ifdef __cplusplus
 }
 #endif

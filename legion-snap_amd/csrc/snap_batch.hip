@@ -1,0 +1,627 @@
+// snap_batch.hip -- many independent fixed-source problems of uneven grid
+// sizes solved together.
+//
+// No counterpart in the reference (which runs one problem per process).
+// The problems share ng, nang, nmom and the iteration controls; their
+// meshes, material maps, material tables and sources differ.  Every per-cell
+// array is the concatenation of the problems' arrays ([g][all cells][...]),
+// so the outer source, inner source, octant reduction and copies run once
+// over the whole batch.  The sweep runs one launch per tile diagonal D
+// covering diagonal D of every problem (a (problem, tile) table per launch
+// and a descriptor per problem), with one tile shape for the batch picked by
+// an autotuner that times candidate shapes on the batch itself.
+//
+// Each problem keeps its own iteration control (snap.cc:468-540) through
+// device predicates: active[p] (not yet converged) and done[p] (its inner
+// iteration converged in this outer).  Workgroups of a finished problem exit
+// at once, the elementwise steps leave its fluxes unchanged, and its
+// counters stop.  Because every decision is on the device, one whole outer
+// iteration -- outer source, iitm predicated inner iterations, outer test --
+// is captured once in a hipGraph and replayed; the host only reads the flags
+// between outer iterations.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+
+#include "snap_internal.h"
+
+namespace snap {
+
+namespace {
+
+struct Buf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  ~Buf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t n) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = n ? n : 16;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      p = nullptr;
+      return fail(SNAP_ERR_NO_MEMORY, "device allocation of %zu bytes failed", bytes);
+    }
+    return SNAP_OK;
+  }
+  template <typename T>
+  T *as() const {
+    return (T *)p;
+  }
+};
+
+__global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void k_fill(double *__restrict__ dst, double v, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = v;
+}
+
+__global__ void k_fill_ints(int *__restrict__ dst, int v, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = v;
+}
+
+// Per-problem convergence (inner.cc:283-324 / outer.cc:354-403 test body):
+// conv[p] stays 1 unless some group and cell of problem p fails
+// |flux0/flux0p - 1| <= eps (|flux0| when |flux0p| < 1e-12).  Block
+// (x, p) covers a slice of problem p's cells x groups.
+__global__ void k_problem_test(const double *__restrict__ f, const double *__restrict__ fp,
+                               const int64_t *__restrict__ cell_off, int64_t total, int ng,
+                               double eps, int *__restrict__ conv) {
+  const int p = blockIdx.y;
+  const int64_t c0 = cell_off[p], nc = cell_off[p + 1] - c0;
+  const int64_t n = nc * ng;
+  bool ok = true;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int g = (int)(t / nc);
+    const int64_t i = (int64_t)g * total + c0 + (t - (int64_t)g * nc);
+    double a = fp[i], df = 1.0;
+    if (fabs(a) < 1.0e-12) {
+      a = 1.0;
+      df = 0.0;
+    }
+    df = fabs(f[i] / a - df);
+    if (df > eps) ok = false;  // NaN compares false: counts as converged, as in the reference
+  }
+  const int all_ok = __syncthreads_and(ok ? 1 : 0);
+  if (threadIdx.x == 0 && !all_ok) atomicAnd(&conv[p], 0);
+}
+
+// Flag updates between the steps of the predicated outer iteration.
+__global__ void k_begin_outer(const int *__restrict__ active, int *__restrict__ done,
+                              int *__restrict__ outers, int np) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np && active[p]) {
+    done[p] = 0;
+    outers[p] += 1;
+  }
+}
+
+__global__ void k_after_inner(const int *__restrict__ active, int *__restrict__ done,
+                              const int *__restrict__ conv, int *__restrict__ inners, int np) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np && active[p] && !done[p]) {
+    inners[p] += 1;
+    done[p] = conv[p];
+  }
+}
+
+// snap.cc:519-538: no outer test on the first outer iteration; it fails
+// unless the last inner iteration converged
+__global__ void k_end_outer(int *__restrict__ active, const int *__restrict__ done,
+                            const int *__restrict__ conv, const int *__restrict__ otno,
+                            int *__restrict__ converged, int np) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np && active[p] && *otno > 0 && done[p] && conv[p]) {
+    active[p] = 0;
+    converged[p] = 1;
+  }
+}
+
+int64_t blocks_for(int64_t n) { return std::min<int64_t>((n + 255) / 256, 65536); }
+
+}  // namespace
+
+}  // namespace snap
+
+using namespace snap;
+
+struct snap_batch {
+  int count = 0, ng = 0, nang = 0, nmom = 0, cmom = 0, nmat = 0;  // nmat: all problems' materials
+  double epsi = 0;
+  int iitm = 0, oitm = 0, fixup = 0, precision = 0;
+  snap_batch_options_t opt{};
+  int64_t total = 0;
+  std::vector<int64_t> cell_off;  // host [count + 1]
+  std::vector<Extent3> ext;
+  std::vector<double> hi, hj, hk;
+  QuadDev quad;
+  SweepPlan plan;        // tile shape and lane layout (its extent: the largest problem)
+  BatchSweep bs;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  Buf mat, slgg, qi, txs, txs32, sxs, dinv, dinv32, flux0, flux0pi, flux0po, fluxm, q2grp0,
+      q2grpm, qtot, part_flux, part_fluxm, faces, descs, tiles, d_cell_off;
+  Buf active, done, conv, inners, outers, converged, otno;
+  double autotune_ms = 0;
+
+  ~snap_batch() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    plan_free(&plan);
+    quad_free(&quad);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  Acc3 acc(const Buf &b, int rec, int elem) const {
+    return dense_acc3(b.p, Extent3{(int)total, 1, 1}, rec, elem);
+  }
+  // problem p's cells of a [g][all cells][rec] array as a 3-D accessor
+  Acc3 sub(const Buf &b, int p, int rec, int elem) const {
+    Acc3 a;
+    a.base = (char *)b.p + (size_t)cell_off[p] * rec * elem;
+    a.s0 = (int64_t)rec * elem;
+    a.s1 = a.s0 * ext[p].nx;
+    a.s2 = a.s1 * ext[p].ny;
+    a.sg = (int64_t)total * rec * elem;
+    return a;
+  }
+
+  // tile shape (TX, TY, TZ, waves, apl) -> plan, per-problem descriptors,
+  // (problem, tile) table by diagonal, face buffer
+  int build_plan(int TX, int TY, int TZ, int waves, int apl) {
+    Extent3 big{1, 1, 1};
+    for (auto &e : ext) {
+      big.nx = std::max(big.nx, e.nx);
+      big.ny = std::max(big.ny, e.ny);
+      big.nz = std::max(big.nz, e.nz);
+    }
+    SNAP_TRY(plan_create(&plan, big, nang, cmom, TX, TY, TZ, waves, apl));
+    const int per_tile = (TY * TZ + TX * TZ + TX * TY) * nang;
+    std::vector<BatchDesc> d(count);
+    int64_t face_off = 0;
+    int maxd = 0;
+    for (int p = 0; p < count; p++) {
+      BatchDesc &b = d[p];
+      b.ext = ext[p];
+      b.nTx = (ext[p].nx + TX - 1) / TX;
+      b.nTy = (ext[p].ny + TY - 1) / TY;
+      b.nTz = (ext[p].nz + TZ - 1) / TZ;
+      b.ntiles = b.nTx * b.nTy * b.nTz;
+      b.cell_off = cell_off[p];
+      b.face_off = face_off;
+      b.hi = hi[p];
+      b.hj = hj[p];
+      b.hk = hk[p];
+      face_off += (int64_t)8 * ng * b.ntiles * per_tile;
+      maxd = std::max(maxd, b.nTx + b.nTy + b.nTz - 2);
+    }
+    std::vector<int2> tab;
+    bs.plane_off.assign(maxd + 1, 0);
+    for (int D = 0; D < maxd; D++) {
+      bs.plane_off[D] = (int)tab.size();
+      for (int p = 0; p < count; p++)
+        for (int tz = 0; tz < d[p].nTz; tz++)
+          for (int ty = 0; ty < d[p].nTy; ty++) {
+            const int tx = D - ty - tz;
+            if (tx >= 0 && tx < d[p].nTx) tab.push_back(make_int2(p, tx | (ty << 10) | (tz << 20)));
+          }
+    }
+    bs.plane_off[maxd] = (int)tab.size();
+    bs.nplanes = maxd;
+    SNAP_TRY(descs.alloc(sizeof(BatchDesc) * count));
+    SNAP_TRY(tiles.alloc(sizeof(int2) * tab.size()));
+    SNAP_HIP_TRY(hipMemcpy(descs.p, d.data(), sizeof(BatchDesc) * count, hipMemcpyHostToDevice));
+    SNAP_HIP_TRY(hipMemcpy(tiles.p, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice));
+    const size_t face_bytes = (size_t)face_off * sizeof(double);
+    if (faces.bytes < face_bytes) SNAP_TRY(faces.alloc(face_bytes));
+    bs.batch_tiles = tiles.as<int2>();
+    bs.descs = descs.as<BatchDesc>();
+    bs.active = active.as<int>();
+    bs.done = done.as<int>();
+    bs.total_cells = total;
+    if (graph_exec) {  // a captured outer iteration holds the old launches
+      (void)hipGraphExecDestroy(graph_exec);
+      (void)hipGraphDestroy(graph);
+      graph_exec = nullptr;
+      graph = nullptr;
+    }
+    return SNAP_OK;
+  }
+
+  int sweep(hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr) {
+    const bool fp32 = precision == SNAP_PRECISION_MIXED;
+    SweepLaunch L;
+    memset(&L, 0, sizeof(L));
+    L.plan = &plan;
+    L.quad = &quad;
+    L.nsweeps = 8 * ng;
+    L.ng_job = ng;
+    L.fixup = fixup;
+    L.q = qtot.as<double>();
+    L.dinv_dense = fp32 ? dinv32.p : dinv.p;
+    L.txs_dense = fp32 ? txs32.p : txs.p;
+    L.part_flux = part_flux.as<double>();
+    L.part_fluxm = cmom > 1 ? part_fluxm.as<double>() : nullptr;
+    L.faces = faces.p;
+    L.fp32 = fp32;
+    L.batch = &bs;
+    if (t0) SNAP_HIP_TRY(hipEventRecord(t0, stream));
+    SNAP_TRY(sweep_enqueue(L, stream));
+    if (t1) SNAP_HIP_TRY(hipEventRecord(t1, stream));
+    return SNAP_OK;
+  }
+
+  int copy(Buf &dst, const Buf &src, int64_t n) {
+    hipLaunchKernelGGL(k_copy, dim3(blocks_for(n)), dim3(256), 0, stream, dst.as<double>(),
+                       src.as<double>(), n);
+    SNAP_HIP_TRY(hipGetLastError());
+    return SNAP_OK;
+  }
+
+  int test(const Buf &prev, double eps) {
+    hipLaunchKernelGGL(k_fill_ints, dim3(1), dim3(256), 0, stream, conv.as<int>(), 1, count);
+    int64_t maxn = 0;
+    for (int p = 0; p < count; p++) maxn = std::max(maxn, (cell_off[p + 1] - cell_off[p]) * ng);
+    const int bx = (int)std::min<int64_t>((maxn + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_problem_test, dim3(bx, count), dim3(256), 0, stream, flux0.as<double>(),
+                       prev.as<double>(), d_cell_off.as<int64_t>(), total, ng, eps, conv.as<int>());
+    SNAP_HIP_TRY(hipGetLastError());
+    return SNAP_OK;
+  }
+
+  // one outer iteration, every step predicated on the device flags
+  int outer_iteration() {
+    const int lma[4] = {1, 3, 5, 7};
+    const int gb = (count + 255) / 256;
+    const int64_t gc = (int64_t)ng * total;
+    const int nm1 = cmom - 1;
+    hipLaunchKernelGGL(k_begin_outer, dim3(gb), dim3(256), 0, stream, active.as<int>(),
+                       done.as<int>(), outers.as<int>(), count);
+    SNAP_TRY(launch_outer_dense(qi.as<double>(), flux0.as<double>(),
+                                cmom > 1 ? fluxm.as<double>() : nullptr, slgg.as<double>(),
+                                mat.as<int32_t>(), q2grp0.as<double>(),
+                                cmom > 1 ? q2grpm.as<double>() : nullptr, total, ng, nmat, nmom,
+                                false, stream));
+    SNAP_TRY(copy(flux0po, flux0, gc));
+    Acc3 none{};
+    for (int inno = 0; inno < iitm; inno++) {
+      SNAP_TRY(launch_inner_source(acc(sxs, nmom, 8), acc(flux0, 1, 8), acc(q2grp0, 1, 8),
+                                   cmom > 1 ? acc(fluxm, nm1, 8) : none,
+                                   cmom > 1 ? acc(q2grpm, nm1, 8) : none, acc(qtot, cmom, 8),
+                                   Extent3{(int)total, 1, 1}, ng, cmom, lma, nmom, stream));
+      SNAP_TRY(copy(flux0pi, flux0, gc));
+      SNAP_TRY(sweep());
+      SNAP_TRY(launch_octant_reduce(part_flux.as<double>(), flux0.as<double>(), gc, stream));
+      if (cmom > 1)
+        SNAP_TRY(launch_octant_reduce(part_fluxm.as<double>(), fluxm.as<double>(), gc * nm1, stream));
+      SNAP_TRY(test(flux0pi, epsi));
+      hipLaunchKernelGGL(k_after_inner, dim3(gb), dim3(256), 0, stream, active.as<int>(),
+                         done.as<int>(), conv.as<int>(), inners.as<int>(), count);
+    }
+    SNAP_TRY(test(flux0po, 100.0 * epsi));
+    hipLaunchKernelGGL(k_end_outer, dim3(gb), dim3(256), 0, stream, active.as<int>(),
+                       done.as<int>(), conv.as<int>(), otno.as<int>(), converged.as<int>(), count);
+    SNAP_HIP_TRY(hipGetLastError());
+    return SNAP_OK;
+  }
+
+  // the initial state of a run: zero fluxes, every problem active
+  int reset() {
+    const int64_t gc = (int64_t)ng * total;
+    hipLaunchKernelGGL(k_fill, dim3(blocks_for(gc)), dim3(256), 0, stream, flux0.as<double>(), 0.0, gc);
+    if (cmom > 1)
+      hipLaunchKernelGGL(k_fill, dim3(blocks_for(gc * (cmom - 1))), dim3(256), 0, stream,
+                         fluxm.as<double>(), 0.0, gc * (cmom - 1));
+    hipLaunchKernelGGL(k_fill_ints, dim3(1), dim3(256), 0, stream, active.as<int>(), 1, count);
+    for (Buf *b : {&done, &inners, &outers, &converged})
+      hipLaunchKernelGGL(k_fill_ints, dim3(1), dim3(256), 0, stream, b->as<int>(), 0, count);
+    SNAP_HIP_TRY(hipGetLastError());
+    return SNAP_OK;
+  }
+
+  // time `reps` sweeps of the current plan with every problem active
+  int time_plan(int reps, float *ms) {
+    SNAP_TRY(reset());
+    hipEvent_t e0, e1;
+    SNAP_HIP_TRY(hipEventCreate(&e0));
+    SNAP_HIP_TRY(hipEventCreate(&e1));
+    SNAP_TRY(sweep());  // untimed: first launches load the code object
+    SNAP_HIP_TRY(hipEventRecord(e0, stream));
+    for (int r = 0; r < reps; r++) SNAP_TRY(sweep());
+    SNAP_HIP_TRY(hipEventRecord(e1, stream));
+    SNAP_HIP_TRY(hipEventSynchronize(e1));
+    SNAP_HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+    *ms /= reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return SNAP_OK;
+  }
+};
+
+extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
+                                     const snap_batch_options_t *o, snap_batch_t **out) {
+  if (!probs || !out || count < 1) return fail(SNAP_ERR_INVALID, "empty batch");
+  *out = nullptr;
+  std::unique_ptr<snap_batch> b(new snap_batch());
+  if (o) b->opt = *o;
+  const snap_problem_t &p0 = probs[0];
+  b->count = count;
+  b->ng = p0.ng;
+  b->nang = p0.nang;
+  b->nmom = p0.nmom;
+  b->cmom = p0.nmom * p0.nmom;
+  b->epsi = p0.epsi;
+  b->iitm = p0.iitm;
+  b->oitm = p0.oitm;
+  b->fixup = p0.fixup ? 1 : 0;
+  b->precision = b->opt.precision;
+  if (b->precision != SNAP_PRECISION_FP64 && b->precision != SNAP_PRECISION_MIXED)
+    return fail(SNAP_ERR_INVALID, "batch precision must be FP64 or MIXED");
+  if (b->nmom < 1 || b->nmom > 4 || b->ng < 1 || b->nang < 1)
+    return fail(SNAP_ERR_INVALID, "bad ng / nang / nmom");
+  b->cell_off.assign(count + 1, 0);
+  int nmat_all = 0;
+  for (int p = 0; p < count; p++) {
+    const snap_problem_t &q = probs[p];
+    if (q.ng != b->ng || q.nang != b->nang || q.nmom != b->nmom || q.epsi != b->epsi ||
+        q.iitm != b->iitm || q.oitm != b->oitm || (q.fixup != 0) != (b->fixup != 0))
+      return fail(SNAP_ERR_INVALID, "problem %d differs in ng, nang, nmom or iteration controls", p);
+    if (q.nx < 1 || q.ny < 1 || q.nz < 1 || q.nmat < 1 || !q.mat || !q.sigt || !q.slgg || !q.qi)
+      return fail(SNAP_ERR_INVALID, "problem %d is incomplete", p);
+    if (!(q.lx > 0 && q.ly > 0 && q.lz > 0)) return fail(SNAP_ERR_INVALID, "problem %d: bad lengths", p);
+    b->ext.push_back(Extent3{q.nx, q.ny, q.nz});
+    b->hi.push_back(2.0 / (q.lx / double(q.nx)));
+    b->hj.push_back(2.0 / (q.ly / double(q.ny)));
+    b->hk.push_back(2.0 / (q.lz / double(q.nz)));
+    b->cell_off[p + 1] = b->cell_off[p] + (int64_t)q.nx * q.ny * q.nz;
+    nmat_all += q.nmat;
+  }
+  b->total = b->cell_off[count];
+  if (b->total >= (int64_t)1 << 31) return fail(SNAP_ERR_UNSUPPORTED, "batches of 2^31 cells or more");
+  b->nmat = nmat_all;
+  const int ng = b->ng, nmom = b->nmom, cmom = b->cmom, nang = b->nang;
+  const int64_t T = b->total;
+  const size_t gc = (size_t)ng * T, nm1 = cmom > 1 ? (size_t)(cmom - 1) : 0;
+
+  // ---- host assembly of the concatenated inputs: material ids offset per
+  // problem, tables stacked along the material axis, t_xs per cell
+  std::vector<int32_t> mat(T);
+  std::vector<double> slgg((size_t)ng * nmat_all * ng * nmom), qi(gc), txs(gc);
+  int moff = 0;
+  for (int p = 0; p < count; p++) {
+    const snap_problem_t &q = probs[p];
+    const int64_t nc = b->cell_off[p + 1] - b->cell_off[p], c0 = b->cell_off[p];
+    for (int64_t c = 0; c < nc; c++) {
+      if (q.mat[c] < 0 || q.mat[c] >= q.nmat)
+        return fail(SNAP_ERR_INVALID, "problem %d: material index out of range", p);
+      mat[c0 + c] = q.mat[c] + moff;
+    }
+    for (int g1 = 0; g1 < ng; g1++)
+      for (int m = 0; m < q.nmat; m++)
+        memcpy(&slgg[(((size_t)g1 * nmat_all + moff + m) * ng) * nmom],
+               &q.slgg[(((size_t)g1 * q.nmat + m) * ng) * nmom], sizeof(double) * ng * nmom);
+    for (int g = 0; g < ng; g++)
+      for (int64_t c = 0; c < nc; c++) {
+        qi[(size_t)g * T + c0 + c] = q.qi[(size_t)g * nc + c];
+        txs[(size_t)g * T + c0 + c] =
+            q.sigt_cell ? q.sigt_cell[(size_t)g * nc + c] : q.sigt[(size_t)q.mat[c] * ng + g];
+      }
+    moff += q.nmat;
+  }
+
+  std::vector<double> mu, eta, xi, w, ec;
+  SNAP_TRY(quad_compute(nang, nmom, mu, eta, xi, w, ec));
+  SNAP_TRY(quad_upload(&b->quad, ec.data(), mu.data(), eta.data(), xi.data(), w.data(), nang, nmom));
+  SNAP_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  const hipStream_t st = b->stream;
+  auto up = [&](Buf &d, const void *h, size_t n) -> int {
+    SNAP_TRY(d.alloc(n));
+    SNAP_HIP_TRY(hipMemcpy(d.p, h, n, hipMemcpyHostToDevice));
+    return SNAP_OK;
+  };
+  SNAP_TRY(up(b->mat, mat.data(), sizeof(int32_t) * T));
+  SNAP_TRY(up(b->slgg, slgg.data(), sizeof(double) * slgg.size()));
+  SNAP_TRY(up(b->qi, qi.data(), sizeof(double) * gc));
+  SNAP_TRY(up(b->txs, txs.data(), sizeof(double) * gc));
+  SNAP_TRY(up(b->d_cell_off, b->cell_off.data(), sizeof(int64_t) * (count + 1)));
+  SNAP_TRY(b->sxs.alloc(sizeof(double) * gc * nmom));
+  SNAP_TRY(b->dinv.alloc(sizeof(double) * gc * nang));
+  for (Buf *x : {&b->flux0, &b->flux0pi, &b->flux0po, &b->q2grp0}) SNAP_TRY(x->alloc(sizeof(double) * gc));
+  SNAP_TRY(b->qtot.alloc(sizeof(double) * gc * cmom));
+  SNAP_TRY(b->fluxm.alloc(sizeof(double) * gc * nm1));
+  SNAP_TRY(b->q2grpm.alloc(sizeof(double) * gc * nm1));
+  SNAP_TRY(b->part_flux.alloc(sizeof(double) * 8 * gc));
+  SNAP_TRY(b->part_fluxm.alloc(sizeof(double) * 8 * gc * nm1));
+  for (Buf *x : {&b->active, &b->done, &b->conv, &b->inners, &b->outers, &b->converged})
+    SNAP_TRY(x->alloc(sizeof(int) * count));
+  SNAP_TRY(b->otno.alloc(sizeof(int)));
+
+  // ExpandScatteringCrossSection over the batch, geometry per problem
+  {
+    Acc2 sl;
+    sl.base = b->slgg.as<char>();
+    sl.s1 = (int64_t)nmom * 8;
+    sl.s0 = sl.s1 * ng;
+    sl.sg = sl.s0 * nmat_all;
+    SNAP_TRY(launch_expand_scat(sl, 0, b->acc(b->mat, 1, 4), b->acc(b->sxs, nmom, 8),
+                                Extent3{(int)T, 1, 1}, ng, nmom, st));
+  }
+  const bool fp32 = b->precision == SNAP_PRECISION_MIXED;
+  if (fp32) {
+    SNAP_TRY(b->dinv32.alloc(sizeof(float) * gc * nang));
+    SNAP_TRY(b->txs32.alloc(sizeof(float) * gc));
+    SNAP_TRY(launch_to_float(b->txs.as<double>(), b->txs32.as<float>(), (int64_t)gc, st));
+  }
+  for (int p = 0; p < count; p++) {
+    SNAP_TRY(launch_geometry(b->sub(b->txs, p, 1, 8), b->sub(b->dinv, p, nang, 8), nullptr,
+                             b->ext[p], ng, nang, b->hi[p], b->hj[p], b->hk[p], b->quad, false, st));
+    if (fp32)
+      SNAP_TRY(launch_geometry(b->sub(b->txs, p, 1, 8), b->sub(b->dinv32, p, nang, 4), nullptr,
+                               b->ext[p], ng, nang, b->hi[p], b->hj[p], b->hk[p], b->quad, true, st));
+  }
+  // the first inner's source of every problem (flux zero): qtot[0] = qi
+  {
+    const int lma[4] = {1, 3, 5, 7};
+    Acc3 none{};
+    SNAP_TRY(b->reset());
+    SNAP_TRY(launch_outer_dense(b->qi.as<double>(), b->flux0.as<double>(),
+                                cmom > 1 ? b->fluxm.as<double>() : nullptr, b->slgg.as<double>(),
+                                b->mat.as<int32_t>(), b->q2grp0.as<double>(),
+                                cmom > 1 ? b->q2grpm.as<double>() : nullptr, T, ng, nmat_all, nmom,
+                                false, st));
+    SNAP_TRY(launch_inner_source(b->acc(b->sxs, nmom, 8), b->acc(b->flux0, 1, 8),
+                                 b->acc(b->q2grp0, 1, 8), cmom > 1 ? b->acc(b->fluxm, cmom - 1, 8) : none,
+                                 cmom > 1 ? b->acc(b->q2grpm, cmom - 1, 8) : none,
+                                 b->acc(b->qtot, cmom, 8), Extent3{(int)T, 1, 1}, ng, cmom, lma,
+                                 nmom, st));
+  }
+
+  // ---- tile shape: the caller's, or the fastest candidate on this batch
+  int TX = b->opt.tile[0], TY = b->opt.tile[1], TZ = b->opt.tile[2];
+  int waves = b->opt.waves_per_tile, apl = b->opt.angles_per_lane;
+  if (TX > 0 && TY > 0 && TZ > 0) {
+    SNAP_TRY(b->build_plan(TX, TY, TZ, waves, apl));
+  } else {
+    const auto t0 = std::chrono::steady_clock::now();
+    static const int cand[][3] = {{8, 8, 4}, {8, 4, 4}, {4, 4, 4}, {8, 8, 8}, {16, 8, 4}};
+    float best = 1e30f;
+    int bi = -1, bapl = 0;
+    for (int c = 0; c < (int)(sizeof(cand) / sizeof(cand[0])); c++) {
+      for (int a = 1; a <= 2; a++) {
+        if (apl > 0 && a != apl) continue;
+        if (a == 2 && !fp32 && nang <= 64) continue;  // FP64 two-angle lanes halve occupancy
+        int w = waves;
+        const int rc = b->build_plan(cand[c][0], cand[c][1], cand[c][2], w, a);
+        if (rc != SNAP_OK) continue;  // shape does not fit (LDS, waves)
+        float ms = 0.f;
+        SNAP_TRY(b->time_plan(1, &ms));
+        if (ms < best) {
+          best = ms;
+          bi = c;
+          bapl = a;
+        }
+      }
+    }
+    if (bi < 0) return fail(SNAP_ERR_INVALID, "no candidate tile fits this batch");
+    SNAP_TRY(b->build_plan(cand[bi][0], cand[bi][1], cand[bi][2], waves, bapl));
+    b->autotune_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  SNAP_HIP_TRY(hipStreamSynchronize(st));
+  *out = b.release();
+  return SNAP_OK;
+}
+
+extern "C" int snap_hip_batch_run(snap_batch_t *b, snap_batch_stats_t *stats,
+                                  snap_problem_stats_t *per_problem) {
+  if (!b) return fail(SNAP_ERR_INVALID, "NULL batch");
+  const hipStream_t st = b->stream;
+  const int n = b->count;
+  std::vector<int> active(n, 1), prev(n, 1), inners(n), outers(n), conv(n);
+  std::vector<double> t_done(n, 0.0);
+  SNAP_HIP_TRY(hipStreamSynchronize(st));
+  const auto t0 = std::chrono::steady_clock::now();
+  SNAP_TRY(b->reset());
+  int otno = 0;
+  for (; otno < b->oitm; otno++) {
+    SNAP_HIP_TRY(hipMemcpyAsync(b->otno.p, &otno, sizeof(int), hipMemcpyHostToDevice, st));
+    if (b->opt.use_graph) {
+      if (!b->graph_exec) {
+        SNAP_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = b->outer_iteration();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc != SNAP_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        SNAP_HIP_TRY(e);
+        b->graph = g;
+        SNAP_HIP_TRY(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+      }
+      SNAP_HIP_TRY(hipGraphLaunch(b->graph_exec, st));
+    } else {
+      SNAP_TRY(b->outer_iteration());
+    }
+    SNAP_HIP_TRY(hipMemcpyAsync(active.data(), b->active.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    SNAP_HIP_TRY(hipStreamSynchronize(st));
+    const double now =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    int left = 0;
+    for (int p = 0; p < n; p++) {
+      if (prev[p] && !active[p]) t_done[p] = now;
+      prev[p] = active[p];
+      left += active[p];
+    }
+    if (!left) {
+      otno++;
+      break;
+    }
+  }
+  const double solve_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  SNAP_HIP_TRY(hipMemcpy(inners.data(), b->inners.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  SNAP_HIP_TRY(hipMemcpy(outers.data(), b->outers.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  SNAP_HIP_TRY(hipMemcpy(conv.data(), b->converged.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  double units = 0.0;
+  int nconv = 0;
+  for (int p = 0; p < n; p++) {
+    const double cells = (double)(b->cell_off[p + 1] - b->cell_off[p]);
+    units += cells * 8.0 * b->nang * b->ng * inners[p];
+    nconv += conv[p];
+    if (per_problem) {
+      per_problem[p].inner_total = inners[p];
+      per_problem[p].outer_total = outers[p];
+      per_problem[p].converged = conv[p];
+      per_problem[p].time_ms = conv[p] ? t_done[p] : solve_ms;
+    }
+  }
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    stats->outer_iterations = otno;
+    stats->converged = nconv;
+    stats->solve_ms = solve_ms;
+    stats->units = units;
+    stats->grind_ns = units > 0 ? solve_ms * 1e6 / units : 0.0;
+    stats->tile[0] = b->plan.TX;
+    stats->tile[1] = b->plan.TY;
+    stats->tile[2] = b->plan.TZ;
+    stats->waves = b->plan.waves;
+    stats->angles_per_lane = b->plan.apl;
+    stats->autotune_ms = b->autotune_ms;
+  }
+  return SNAP_OK;
+}
+
+extern "C" int snap_hip_batch_get_flux(snap_batch_t *b, int problem, double *flux0, double *fluxm) {
+  if (!b || problem < 0 || problem >= b->count) return fail(SNAP_ERR_INVALID, "bad problem index");
+  SNAP_HIP_TRY(hipStreamSynchronize(b->stream));
+  const int64_t c0 = b->cell_off[problem], nc = b->cell_off[problem + 1] - c0;
+  const int nm1 = b->cmom - 1;
+  for (int g = 0; g < b->ng; g++) {
+    if (flux0)
+      SNAP_HIP_TRY(hipMemcpy(flux0 + (size_t)g * nc, b->flux0.as<double>() + (size_t)g * b->total + c0,
+                             sizeof(double) * nc, hipMemcpyDeviceToHost));
+    if (fluxm && nm1 > 0)
+      SNAP_HIP_TRY(hipMemcpy(fluxm + (size_t)g * nc * nm1,
+                             b->fluxm.as<double>() + ((size_t)g * b->total + c0) * nm1,
+                             sizeof(double) * nc * nm1, hipMemcpyDeviceToHost));
+  }
+  return SNAP_OK;
+}
+
+extern "C" void snap_hip_batch_destroy(snap_batch_t *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  delete b;
+}

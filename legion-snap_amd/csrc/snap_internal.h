@@ -178,6 +178,28 @@ void plan_free(SweepPlan *p);
 void plan_choose(Extent3 e, int nang, int cmom, int real_bytes, int *TX, int *TY, int *TZ,
                  int *waves, int *apl);
 
+// One problem of a batch sweep: its chunk, tile grid, offsets into the
+// batch's concatenated arrays and face buffer, and cell spacing.
+struct BatchDesc {
+  Extent3 ext;
+  int nTx, nTy, nTz, ntiles;
+  int64_t cell_off;  // first cell in the [g][all cells] arrays
+  int64_t face_off;  // first face element (of the sweep's real type)
+  double hi, hj, hk;
+};
+
+// Tile-diagonal launches of a batch: launch D covers diagonal D of every
+// problem; batch_tiles holds (problem, packed tile) entries by diagonal.
+struct BatchSweep {
+  int nplanes = 0;
+  std::vector<int> plane_off;       // host: offsets of each diagonal in batch_tiles
+  const int2 *batch_tiles = nullptr;
+  const BatchDesc *descs = nullptr;
+  const int *active = nullptr;      // device flags: problem still iterating
+  const int *done = nullptr;        // device flags: inner iteration converged this outer
+  int64_t total_cells = 0;
+};
+
 struct SweepLaunch {
   const SweepPlan *plan;
   const QuadDev *quad;
@@ -199,6 +221,7 @@ struct SweepLaunch {
   bool fp32;                   // FP32 sweep arithmetic and faces
   bool exact;                  // FP64 in the reference's operation order, no fused ops
   bool recompute_dinv;         // form dinv from t_xs in the kernel instead of reading it
+  const BatchSweep *batch;     // non-NULL: a batch of problems (plan gives the tile shape)
 };
 // Enqueue every tile diagonal of the sweep (plan->nplanes launches).  When
 // `events` is given (2*nplanes events) each launch is bracketed by a pair.

@@ -303,6 +303,12 @@ struct SweepArgs {
   const int *plane_tiles;
   const int *cell_off;
   const int16_t *cells;
+  // batch of problems (BATCH kernels): per-launch (problem, tile) entries,
+  // per-problem chunks, and the predicates that skip finished problems
+  const int2 *batch_tiles;
+  const BatchDesc *descs;
+  const int *active, *done;
+  int64_t total_cells;
 };
 
 // Arguments of the general variant (the drop-in entry point): Realm affine
@@ -358,8 +364,11 @@ __device__ inline unsigned seg_count(const unsigned long long (&b)[NB], unsigned
 // APL: angles per lane.  EXACT: the reference's operation order with separate
 // roundings (bit-equal to the CPU oracle); otherwise fused multiply-adds.
 // REC: form dinv from t_xs instead of reading it.  GEN: general accessors,
-// ghost faces and atomic accumulation (the drop-in entry point).
-template <typename Real, int CPW, int APL, bool EXACT, bool REC, bool GEN>
+// ghost faces and atomic accumulation (the drop-in entry point).  BATCH:
+// each workgroup's (problem, tile) comes from the launch's table, arrays are
+// the batch's concatenation ([g][all cells]), and a workgroup whose problem
+// is finished or has converged its inner iteration exits at once.
+template <typename Real, int CPW, int APL, bool EXACT, bool REC, bool GEN, bool BATCH = false>
 __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, SweepGeneral G, int plane_off) {
   constexpr bool kFP64 = std::is_same<Real, double>::value;
   constexpr int MS = max_steps(APL);  // cell groups per wave and diagonal
@@ -387,22 +396,47 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
   const int s = blockIdx.y;
   const int corner = A.corner0 + s / A.ng_job;
   const int g = s % A.ng_job;
-  const int tp = A.plane_tiles[plane_off + blockIdx.x];
+  // the chunk: the launch's own, or one problem of a batch
+  Extent3 ext = A.ext;
+  int nTx = A.nTx, nTy = A.nTy, nTz = A.nTz, ntiles = A.ntiles;
+  double dhi = A.hi, dhj = A.hj, dhk = A.hk;
+  size_t gstride, coff = 0, face_off = 0;
+  int tp;
+  if constexpr (BATCH) {
+    const int2 bt = A.batch_tiles[plane_off + blockIdx.x];
+    if (!A.active[bt.x] || A.done[bt.x]) return;  // the whole workgroup
+    const BatchDesc &P = A.descs[bt.x];
+    ext = P.ext;
+    nTx = P.nTx;
+    nTy = P.nTy;
+    nTz = P.nTz;
+    ntiles = P.ntiles;
+    dhi = P.hi;
+    dhj = P.hj;
+    dhk = P.hk;
+    gstride = (size_t)A.total_cells;
+    coff = (size_t)P.cell_off;
+    face_off = (size_t)P.face_off;
+    tp = bt.y;
+  } else {
+    gstride = (size_t)ext.nx * ext.ny * ext.nz;
+    tp = A.plane_tiles[plane_off + blockIdx.x];
+  }
   const int tx = tp & 1023, ty = (tp >> 10) & 1023, tz = (tp >> 20) & 1023;
   const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
-  const int nx = A.ext.nx, ny = A.ext.ny, nz = A.ext.nz;
+  const int nx = ext.nx, ny = ext.ny, nz = ext.nz;
   const int ex = min(TX, nx - x0), ey = min(TY, ny - y0), ez = min(TZ, nz - z0);
   const bool sxp = (corner & 1) != 0, syp = (corner & 2) != 0, szp = (corner & 4) != 0;
-  const int tile_id = (tz * A.nTy + ty) * A.nTx + tx;
+  const int tile_id = (tz * nTy + ty) * nTx + tx;
   const int per_tile = fxn + fyn + fzn;
-  Real *faces_s = (Real *)A.faces + (size_t)s * A.ntiles * per_tile;
-  const int ncells = nx * ny * nz;
+  Real *faces_s = (Real *)A.faces + face_off + (size_t)s * ntiles * per_tile;
   // dense per-sweep bases
-  const double *qg = A.q + (size_t)g * ncells * cmom;
-  const Real *dg = (const Real *)A.dinv + (size_t)g * ncells * nang;
-  const Real *tg = (const Real *)A.txs + (size_t)g * ncells;
-  double *pfg = A.part_flux + ((size_t)corner * A.ng_job + g) * ncells;
-  double *pmg = A.part_fluxm + ((size_t)corner * A.ng_job + g) * ncells * (cmom - 1);
+  const double *qg = A.q + ((size_t)g * gstride + coff) * cmom;
+  const Real *dg = (const Real *)A.dinv + ((size_t)g * gstride + coff) * nang;
+  const Real *tg = (const Real *)A.txs + (size_t)g * gstride + coff;
+  double *pfg = A.part_flux + ((size_t)corner * A.ng_job + g) * gstride + coff;
+  double *pmg =
+      A.part_fluxm + (((size_t)corner * A.ng_job + g) * gstride + coff) * (size_t)(cmom - 1);
 
   // ---- per-lane constants: lane = cell slot * LPC + la, angles la + r*LPC
   const int cl = (CPW == 1) ? 0 : lane / LPC;
@@ -460,7 +494,7 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
     }
   }
   {
-    const Real *up = faces_s + (size_t)(tile_id - A.nTx) * per_tile + fxn;
+    const Real *up = faces_s + (size_t)(tile_id - nTx) * per_tile + fxn;
     if (ty > 0) {
       for (int idx = threadIdx.x; idx < fyn; idx += blockDim.x) fy[idx] = up[idx];
     } else if (GEN && G.gy.base) {
@@ -478,7 +512,7 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
     }
   }
   {
-    const Real *up = faces_s + (size_t)(tile_id - A.nTx * A.nTy) * per_tile + fxn + fyn;
+    const Real *up = faces_s + (size_t)(tile_id - nTx * nTy) * per_tile + fxn + fyn;
     if (tz > 0) {
       for (int idx = threadIdx.x; idx < fzn; idx += blockDim.x) fz[idx] = up[idx];
     } else if (GEN && G.gz.base) {
@@ -512,13 +546,13 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
       cy[r] = (Real)eta_d;
       cz[r] = (Real)xi_d;
     } else {
-      cx[r] = (Real)(mu_d * A.hi);
-      cy[r] = (Real)(eta_d * A.hj);
-      cz[r] = (Real)(xi_d * A.hk);
+      cx[r] = (Real)(mu_d * dhi);
+      cy[r] = (Real)(eta_d * dhj);
+      cz[r] = (Real)(xi_d * dhk);
     }
-    g_sum32[r] = (float)((A.hi * mu_d + A.hj * eta_d) + A.hk * xi_d);
+    g_sum32[r] = (float)((dhi * mu_d + dhj * eta_d) + dhk * xi_d);
   }
-  const Real hi = (Real)A.hi, hj = (Real)A.hj, hk = (Real)A.hk;
+  const Real hi = (Real)dhi, hj = (Real)dhj, hk = (Real)dhk;
   const Real tolr = (Real)1.0e-12;
   const int nm1 = cmom - 1;
   // source moments travel in lanes (lane la of a cell's segment holds moments
@@ -830,7 +864,7 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
   Real *out = faces_s + (size_t)tile_id * per_tile;
   for (int idx = threadIdx.x; idx < per_tile; idx += blockDim.x) out[idx] = fx[idx];
   if constexpr (GEN) {
-    if (tx == A.nTx - 1 && G.gx.base)
+    if (tx == nTx - 1 && G.gx.base)
       for (int idx = threadIdx.x; idx < fxn; idx += blockDim.x) {
         const int a2 = idx % nang, slot = idx / nang, j = slot % TY, k = slot / TY;
         if (j < ey && k < ez) {
@@ -838,7 +872,7 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
           ((double *)G.gx.at(py, pz, 0))[a2] = (double)fx[idx];
         }
       }
-    if (ty == A.nTy - 1 && G.gy.base)
+    if (ty == nTy - 1 && G.gy.base)
       for (int idx = threadIdx.x; idx < fyn; idx += blockDim.x) {
         const int a2 = idx % nang, slot = idx / nang, i = slot % TX, k = slot / TX;
         if (i < ex && k < ez) {
@@ -846,7 +880,7 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
           ((double *)G.gy.at(px, pz, 0))[a2] = (double)fy[idx];
         }
       }
-    if (tz == A.nTz - 1 && G.gz.base)
+    if (tz == nTz - 1 && G.gz.base)
       for (int idx = threadIdx.x; idx < fzn; idx += blockDim.x) {
         const int a2 = idx % nang, slot = idx / nang, i = slot % TX, j = slot / TX;
         if (i < ex && j < ey) {
@@ -895,6 +929,19 @@ static SweepKernel kernel_for(int cpw, int apl, bool rec, bool gen) {
   return nullptr;
 }
 
+// Batch kernels: dense, dinv streamed, FP64 (fused or exact) or FP32.
+template <typename Real, bool EXACT>
+static SweepKernel kernel_for_batch(int cpw, int apl) {
+#define SNAP_BATCH_CASE(C, P) \
+  if (cpw == C && apl == P) return k_sweep_tiles<Real, C, P, EXACT, false, false, true>;
+  SNAP_BATCH_CASE(1, 1) SNAP_BATCH_CASE(2, 1) SNAP_BATCH_CASE(3, 1) SNAP_BATCH_CASE(4, 1)
+  SNAP_BATCH_CASE(5, 1) SNAP_BATCH_CASE(6, 1) SNAP_BATCH_CASE(8, 1)
+  SNAP_BATCH_CASE(1, 2) SNAP_BATCH_CASE(2, 2) SNAP_BATCH_CASE(3, 2) SNAP_BATCH_CASE(4, 2)
+  SNAP_BATCH_CASE(5, 2) SNAP_BATCH_CASE(6, 2) SNAP_BATCH_CASE(8, 2)
+#undef SNAP_BATCH_CASE
+  return nullptr;
+}
+
 static int ensure_lds_attr(SweepKernel k, size_t bytes) {
   // dynamic LDS above 64 KiB must be requested per kernel
   static std::mutex mu;
@@ -917,6 +964,8 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
     return fail(SNAP_ERR_INVALID, "the general sweep handles one (corner, group) per call");
   if ((int64_t)p.ext.nx * p.ext.ny * p.ext.nz >= (int64_t)1 << 31)
     return fail(SNAP_ERR_UNSUPPORTED, "chunks of 2^31 cells or more");
+  if (L.batch && (L.general || L.recompute_dinv))
+    return fail(SNAP_ERR_INVALID, "batch sweeps are dense with dinv streamed");
   SweepArgs A;
   memset(&A, 0, sizeof(A));
   SweepGeneral G;
@@ -956,6 +1005,13 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
   A.plane_tiles = p.plane_tiles;
   A.cell_off = p.cell_off;
   A.cells = p.cells;
+  if (L.batch) {
+    A.batch_tiles = L.batch->batch_tiles;
+    A.descs = L.batch->descs;
+    A.active = L.batch->active;
+    A.done = L.batch->done;
+    A.total_cells = L.batch->total_cells;
+  }
   if (L.general) {
     G.qtot = L.qtot;
     G.dinv = L.dinv;
@@ -973,19 +1029,25 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
   }
   const size_t lds = p.lds_bytes(L.fp32 ? 4 : 8, L.recompute_dinv);
   const SweepKernel k =
-      L.fp32 ? kernel_for<float, false>(p.cpw, p.apl, L.recompute_dinv, L.general)
-             : (L.exact ? kernel_for<double, true>(p.cpw, p.apl, L.recompute_dinv, L.general)
-                        : kernel_for<double, false>(p.cpw, p.apl, L.recompute_dinv, L.general));
+      L.batch ? (L.fp32 ? kernel_for_batch<float, false>(p.cpw, p.apl)
+                        : (L.exact ? kernel_for_batch<double, true>(p.cpw, p.apl)
+                                   : kernel_for_batch<double, false>(p.cpw, p.apl)))
+      : L.fp32 ? kernel_for<float, false>(p.cpw, p.apl, L.recompute_dinv, L.general)
+               : (L.exact ? kernel_for<double, true>(p.cpw, p.apl, L.recompute_dinv, L.general)
+                          : kernel_for<double, false>(p.cpw, p.apl, L.recompute_dinv, L.general));
   if (!k)
     return fail(SNAP_ERR_INVALID,
                 "no sweep kernel for %d cells per wave, %d angles per lane%s%s", p.cpw, p.apl,
                 L.recompute_dinv ? ", dinv recomputed" : "", L.general ? ", general" : "");
   SNAP_TRY(ensure_lds_attr(k, lds));
   const dim3 block(p.waves * 64);
-  for (int D = 0; D < p.nplanes; D++) {
-    const int count = p.plane_off[D + 1] - p.plane_off[D];
+  const int nplanes = L.batch ? L.batch->nplanes : p.nplanes;
+  const std::vector<int> &poff = L.batch ? L.batch->plane_off : p.plane_off;
+  for (int D = 0; D < nplanes; D++) {
+    const int count = poff[D + 1] - poff[D];
+    if (count == 0) continue;
     if (events) SNAP_HIP_TRY(hipEventRecord(events[2 * D], st));
-    hipLaunchKernelGGL(k, dim3(count, L.nsweeps), block, lds, st, A, G, p.plane_off[D]);
+    hipLaunchKernelGGL(k, dim3(count, L.nsweeps), block, lds, st, A, G, poff[D]);
     if (events) SNAP_HIP_TRY(hipEventRecord(events[2 * D + 1], st));
   }
   SNAP_HIP_TRY(hipGetLastError());

@@ -84,6 +84,24 @@ class SolveOptions(ctypes.Structure):
     ]
 
 
+class BatchOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("tile", ctypes.c_int * 3),
+                ("waves_per_tile", ctypes.c_int), ("angles_per_lane", ctypes.c_int),
+                ("use_graph", ctypes.c_int)]
+
+
+class ProblemStats(ctypes.Structure):
+    _fields_ = [("inner_total", ctypes.c_int), ("outer_total", ctypes.c_int),
+                ("converged", ctypes.c_int), ("time_ms", ctypes.c_double)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("outer_iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("solve_ms", ctypes.c_double), ("units", ctypes.c_double),
+                ("grind_ns", ctypes.c_double), ("tile", ctypes.c_int * 3), ("waves", ctypes.c_int),
+                ("angles_per_lane", ctypes.c_int), ("autotune_ms", ctypes.c_double)]
+
+
 class SolveStats(ctypes.Structure):
     _fields_ = [
         ("inner_total", ctypes.c_int), ("outer_total", ctypes.c_int),
@@ -139,6 +157,13 @@ SIGNATURES = {
     "snap_hip_solver_time_sweeps": (
         [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "snap_hip_solver_plan": ([_vp, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+    "snap_hip_batch_create": (
+        [ctypes.POINTER(Problem), ctypes.c_int, ctypes.POINTER(BatchOptions), ctypes.POINTER(_vp)],
+        ctypes.c_int),
+    "snap_hip_batch_run": ([_vp, ctypes.POINTER(BatchStats), ctypes.POINTER(ProblemStats)],
+                           ctypes.c_int),
+    "snap_hip_batch_get_flux": ([_vp, ctypes.c_int, _dp, _dp], ctypes.c_int),
+    "snap_hip_batch_destroy": ([_vp], None),
     "snap_hip_solver_gradient": (
         [_vp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp,
          ctypes.POINTER(ctypes.c_int)], ctypes.c_int),

@@ -1,0 +1,66 @@
+"""Batches of independent problems solved together on the HIP device (the
+snap_hip_batch_* entry points): one launch per sweep stage for all
+problems, per-problem iteration control on the device, a hipGraph-captured
+outer iteration, and a tile shape autotuned on the batch."""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from . import _lib
+from .problems import Problem
+from .solver import _PREC, _ptr, problem_struct
+
+
+class Batch:
+    def __init__(self, problems: List[Problem], precision: str = "fp64", tile=(0, 0, 0),
+                 waves_per_tile: int = 0, angles_per_lane: int = 0, use_graph: bool = True):
+        self.problems = [q.contiguous() for q in problems]
+        for q in self.problems:
+            q.validate()
+        self.lib = _lib.load()
+        arr = (_lib.Problem * len(self.problems))()
+        for i, q in enumerate(self.problems):
+            arr[i] = problem_struct(q)
+        o = _lib.BatchOptions()
+        o.precision = _PREC[precision]
+        o.tile[0], o.tile[1], o.tile[2] = tile
+        o.waves_per_tile = waves_per_tile
+        o.angles_per_lane = angles_per_lane
+        o.use_graph = int(bool(use_graph))
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.snap_hip_batch_create(arr, len(self.problems), ctypes.byref(o),
+                                                  ctypes.byref(h)), "snap_hip_batch_create")
+        self._h = h
+
+    def run(self) -> Tuple[dict, List[dict]]:
+        st = _lib.BatchStats()
+        per = (_lib.ProblemStats * len(self.problems))()
+        _lib.check(self.lib.snap_hip_batch_run(self._h, ctypes.byref(st), per), "snap_hip_batch_run")
+        stats = {k: getattr(st, k) for k, _ in st._fields_}
+        stats["tile"] = tuple(st.tile)
+        return stats, [{k: getattr(x, k) for k, _ in x._fields_} for x in per]
+
+    def flux(self, i: int) -> Tuple[np.ndarray, Optional[np.ndarray]]:
+        p = self.problems[i]
+        f0 = np.zeros((p.ng, p.nz, p.ny, p.nx))
+        fm = np.zeros((p.ng, p.nz, p.ny, p.nx, p.cmom - 1)) if p.cmom > 1 else None
+        _lib.check(self.lib.snap_hip_batch_get_flux(self._h, i, _ptr(f0), _ptr(fm)),
+                   "snap_hip_batch_get_flux")
+        return f0, fm
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self.lib.snap_hip_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()

@@ -27,6 +27,22 @@ def _ptr(a: Optional[np.ndarray], typ=ctypes.c_double):
     return a.ctypes.data_as(ctypes.POINTER(typ))
 
 
+def problem_struct(p: Problem) -> "_lib.Problem":
+    """The C-ABI view of a contiguous Problem (its arrays must outlive the
+    call that reads them)."""
+    s = _lib.Problem()
+    s.nx, s.ny, s.nz = p.nx, p.ny, p.nz
+    s.lx, s.ly, s.lz = p.lx, p.ly, p.lz
+    s.ng, s.nang, s.nmom, s.nmat = p.ng, p.nang, p.nmom, p.nmat
+    s.mat = _ptr(p.mat, ctypes.c_int32)
+    s.sigt = _ptr(p.sigt)
+    s.slgg = _ptr(p.slgg)
+    s.qi = _ptr(p.qi)
+    s.sigt_cell = _ptr(p.sigt_cell)
+    s.epsi, s.iitm, s.oitm, s.fixup = p.epsi, p.iitm, p.oitm, int(bool(p.fixup))
+    return s
+
+
 class Solver:
     """One problem on the current HIP device.
 
@@ -45,16 +61,7 @@ class Solver:
         problem.validate()
         self.problem = p = problem.contiguous()
         self.lib = _lib.load()
-        s = _lib.Problem()
-        s.nx, s.ny, s.nz = p.nx, p.ny, p.nz
-        s.lx, s.ly, s.lz = p.lx, p.ly, p.lz
-        s.ng, s.nang, s.nmom, s.nmat = p.ng, p.nang, p.nmom, p.nmat
-        s.mat = _ptr(p.mat, ctypes.c_int32)
-        s.sigt = _ptr(p.sigt)
-        s.slgg = _ptr(p.slgg)
-        s.qi = _ptr(p.qi)
-        s.sigt_cell = _ptr(p.sigt_cell)
-        s.epsi, s.iitm, s.oitm, s.fixup = p.epsi, p.iitm, p.oitm, int(bool(p.fixup))
+        s = problem_struct(p)
         o = _lib.SolveOptions()
         o.precision = _PREC[precision]
         o.tile[0], o.tile[1], o.tile[2] = tile
