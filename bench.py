@@ -89,6 +89,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the adjoint (config 3) and batch (config 4) measurements")
     ap.add_argument("--size", type=int, default=64, help="grid edge (default: the 64^3 workload)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy and exit (calibrates the "
@@ -168,6 +170,44 @@ def main() -> int:
                         "maxrel_vs_fp64": float(np.max(d / np.abs(fex))),
                         "l2rel_vs_fp64": float(np.linalg.norm(d) / np.linalg.norm(fex)),
                         "speedup_vs_fp64_grind": grind / g, "plan": plan}
+
+    # ---- adjoint gradient (config 3: 48^3, 16 groups, 24 angles, P0): the
+    # forward solve, then dR/dsiga for all cells by reverse sweeps
+    if not args.no_extra:
+        c3 = problems.config3()
+        resp = np.zeros((c3.ng, c3.nz, c3.ny, c3.nx))
+        resp[:, -8:, -8:, -8:] = 1.0
+        for prec in ("fp64", "mixed"):
+            with Solver(c3, precision=prec) as s:
+                st = s.run()
+                s.gradient(resp, precision=prec)  # warm-up
+                t0 = time.perf_counter()
+                g, R, it = s.gradient(resp, precision=prec)
+                tg = time.perf_counter() - t0
+            details[f"adjoint_config3_{prec}"] = {
+                "forward_solve_s": st["solve_ms"] / 1e3, "gradient_s": tg, "adjoint_iterations": it,
+                "response": R, "cells": c3.ncells,
+                "grad_siga_max_abs": float(np.max(np.abs(g.sum(axis=0))))}
+
+    # ---- batch of uneven problems (config 4: 48 problems, edges 24..64,
+    # 16 groups, 24 angles, P3), one launch per sweep stage, graph-captured
+    # outer iterations, autotuned tile shape
+    if not args.no_extra:
+        from snap_amd.batch import Batch
+        c4 = problems.config4()
+        for prec in ("fp64", "mixed"):
+            with Batch(c4, precision=prec, use_graph=True) as b:
+                b.run()  # warm-up (captures the graph)
+                st, per = b.run()
+            times = np.array([x["time_ms"] for x in per])
+            details[f"batch_config4_{prec}"] = {
+                "problems": len(c4), "cells": int(sum(q.ncells for q in c4)),
+                "grind_ns": st["grind_ns"], "solve_s": st["solve_ms"] / 1e3,
+                "converged": st["converged"], "outer_iterations": st["outer_iterations"],
+                "time_to_convergence_p50_ms": float(np.percentile(times, 50)),
+                "time_to_convergence_p99_ms": float(np.percentile(times, 99)),
+                "tile": list(st["tile"]), "angles_per_lane": st["angles_per_lane"],
+                "autotune_s": st["autotune_ms"] / 1e3}
 
     cpu = None
     if not args.no_cpu:

@@ -504,7 +504,7 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
         const int rc = b->build_plan(cand[c][0], cand[c][1], cand[c][2], w, a);
         if (rc != SNAP_OK) continue;  // shape does not fit (LDS, waves)
         float ms = 0.f;
-        SNAP_TRY(b->time_plan(1, &ms));
+        SNAP_TRY(b->time_plan(2, &ms));
         if (ms < best) {
           best = ms;
           bi = c;
