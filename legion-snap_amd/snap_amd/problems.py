@@ -57,6 +57,14 @@ class Problem:
     fixup: bool = True
     sigt_cell: Optional[np.ndarray] = None
     name: str = "problem"
+    # time dependence (snap.cc:716-738, 400-541): nsteps steps of dt = tf/nsteps
+    # with vdelt_g = 2/(dt (ng-g)); a steady problem runs one step with vdelt 0
+    timedep: bool = False
+    tf: float = 1.0
+    nsteps: int = 1
+    # manufactured-solution source (src_opt 3, mms.cc): the solver builds qi
+    # and the angular source itself and qi is ignored
+    mms: bool = False
 
     @property
     def nmat(self) -> int:
@@ -103,6 +111,12 @@ class Problem:
             raise ValueError("qi must be [ng, nz, ny, nx]")
         if self.sigt_cell is not None and self.sigt_cell.shape != (self.ng,) + shp:
             raise ValueError("sigt_cell must be [ng, nz, ny, nx]")
+        if self.nsteps < 1:
+            raise ValueError("nsteps must be >= 1")
+        if self.timedep and not self.tf > 0.0:
+            raise ValueError("a time-dependent problem needs tf > 0")
+        if self.mms and (self.nmom != 1 or self.sigt_cell is not None):
+            raise ValueError("the manufactured solution is defined for nmom = 1 and material cross sections")
 
     def contiguous(self) -> "Problem":
         """Copy with every array C-contiguous in the dtype the ABIs expect."""

@@ -28,7 +28,8 @@ class OracleProblem(ctypes.Structure):
         ("nmat", ctypes.c_int),
         ("mat", _ip), ("sigt", _dp), ("slgg", _dp), ("qi", _dp), ("sigt_cell", _dp),
         ("epsi", ctypes.c_double), ("iitm", ctypes.c_int), ("oitm", ctypes.c_int),
-        ("fixup", ctypes.c_int),
+        ("fixup", ctypes.c_int), ("timedep", ctypes.c_int), ("tf", ctypes.c_double),
+        ("nsteps", ctypes.c_int), ("mms", ctypes.c_int),
     ]
 
 
@@ -37,6 +38,7 @@ class OracleStats(ctypes.Structure):
         ("inner_total", ctypes.c_int), ("outer_total", ctypes.c_int),
         ("converged", ctypes.c_int), ("fixup_cells", ctypes.c_int),
         ("sweep_seconds", ctypes.c_double), ("solve_seconds", ctypes.c_double),
+        ("time_steps", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -71,6 +73,14 @@ def load():
         lib.snap_oracle_sweep.argtypes = (
             [ctypes.c_int] * 6 + [ctypes.c_double] * 3 + [_dp] * 8 + [ctypes.c_int] + [_dp] * 8)
         lib.snap_oracle_sweep.restype = ctypes.c_int
+        lib.snap_oracle_sweep_td.argtypes = (
+            [ctypes.c_int] * 6 + [ctypes.c_double] * 3 + [_dp] * 8 + [ctypes.c_int]
+            + [ctypes.c_double] + [_dp] * 8)
+        lib.snap_oracle_sweep_td.restype = ctypes.c_int
+        lib.snap_oracle_mms_setup.argtypes = [ctypes.POINTER(OracleProblem), _dp, _dp, _dp]
+        lib.snap_oracle_mms_setup.restype = ctypes.c_int
+        lib.snap_oracle_mms_compare.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        lib.snap_oracle_mms_compare.restype = None
         lib.snap_oracle_solve.argtypes = [ctypes.POINTER(OracleProblem), _dp, _dp,
                                           ctypes.POINTER(OracleStats), ctypes.c_int]
         lib.snap_oracle_solve.restype = ctypes.c_int
@@ -197,11 +207,57 @@ def _problem_struct(p, keep):
     s.qi = _p(q.qi)
     s.sigt_cell = _p(q.sigt_cell)
     s.epsi, s.iitm, s.oitm, s.fixup = q.epsi, q.iitm, q.oitm, int(bool(q.fixup))
+    s.timedep, s.tf, s.nsteps, s.mms = int(bool(q.timedep)), q.tf, q.nsteps, int(bool(q.mms))
     return s
 
 
+def sweep_td(nx, ny, nz, nang, cmom, corner, hijk, quad, qtot_g, dinv_g, t_xs_g, fixup, vdelt,
+             tflux_in, qim_g=None):
+    """One (corner, group) sweep from vacuum inflow with the time-dependent
+    terms (sweep.cc:414-534) and the optional MMS angular source.
+    tflux_in / qim_g: [nz,ny,nx,nang].  Returns (part_flux, part_fluxm or
+    None, fixup_cells, tflux_out, (ghost_x_out, ghost_y_out, ghost_z_out))."""
+    lib = load()
+    mu, eta, xi, w, ec = quad
+    pf = np.zeros((nz, ny, nx))
+    pm = np.zeros((nz, ny, nx, cmom - 1)) if cmom > 1 else None
+    tout = np.zeros((nz, ny, nx, nang))
+    tin = np.ascontiguousarray(tflux_in, dtype=np.float64)
+    qm = None if qim_g is None else np.ascontiguousarray(qim_g, dtype=np.float64)
+    outs = (np.zeros((nz, ny, nang)), np.zeros((nz, nx, nang)), np.zeros((ny, nx, nang)))
+    n = lib.snap_oracle_sweep_td(nx, ny, nz, nang, cmom, corner, hijk[0], hijk[1], hijk[2],
+                                 _p(mu), _p(eta), _p(xi), _p(w), _p(ec), _p(qtot_g), _p(dinv_g),
+                                 _p(t_xs_g), int(bool(fixup)), float(vdelt), _p(tin), _p(tout),
+                                 _p(qm), _p(outs[0]), _p(outs[1]), _p(outs[2]), _p(pf), _p(pm))
+    return pf, pm, n, tout, outs
+
+
+def mms_setup(p):
+    """MMSInitFlux / MMSInitSource / MMSInitTimeDependent (mms.cc:81-409):
+    (ref_flux [ng,nz,ny,nx], qim [8,ng,nz,ny,nx,nang] unscaled, qi [ng,nz,ny,nx])."""
+    lib = load()
+    keep = []
+    s = _problem_struct(p, keep)
+    ref = np.zeros((p.ng, p.nz, p.ny, p.nx))
+    qim = np.zeros((8, p.ng, p.nz, p.ny, p.nx, p.nang))
+    qi = np.zeros((p.ng, p.nz, p.ny, p.nx))
+    if lib.snap_oracle_mms_setup(ctypes.byref(s), _p(ref), _p(qim), _p(qi)):
+        raise ValueError("MMS needs nmom = 1")
+    return ref, qim, qi
+
+
+def mms_compare(flux0, ref):
+    """MMSCompare (mms.cc:497-541): (max, min, mean) of |flux/ref - 1|."""
+    lib = load()
+    f = np.ascontiguousarray(flux0, dtype=np.float64)
+    r = np.ascontiguousarray(ref, dtype=np.float64)
+    out = np.zeros(3)
+    lib.snap_oracle_mms_compare(f.size, _p(f), _p(r), _p(out))
+    return float(out[0]), float(out[1]), float(out[2])
+
+
 def solve(p, nthreads: int = 0):
-    """transport_solve (snap.cc:468-540).  Returns (flux0 [ng,nz,ny,nx],
+    """transport_solve (snap.cc:386-541), every time step.  Returns (flux0 [ng,nz,ny,nx],
     fluxm [ng,nz,ny,nx,cmom-1] or None, stats dict)."""
     lib = load()
     p.validate()

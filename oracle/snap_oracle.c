@@ -202,24 +202,28 @@ int snap_oracle_converged(int n, const double *flux0, const double *flux0p,
   return 1;
 }
 
-/* MiniKBATask::cpu_implementation, sweep.cc:242-616 (generic variant),
- * time-independent (vdelt == 0), non-MMS. */
-int snap_oracle_sweep(int nx, int ny, int nz, int nang, int cmom, int corner,
-                      double hi, double hj, double hk, const double *mu,
-                      const double *eta, const double *xi, const double *w,
-                      const double *ec, const double *qtot_g,
-                      const double *dinv_g, const double *t_xs_g, int fixup,
-                      const double *ghost_x_in, const double *ghost_y_in,
-                      const double *ghost_z_in, double *ghost_x_out,
-                      double *ghost_y_out, double *ghost_z_out,
-                      double *part_flux, double *part_fluxm) {
+/* MiniKBATask::cpu_implementation, sweep.cc:242-616 (generic variant).
+ * vdelt != 0 adds the time terms (sweep.cc:414-419, 461-479, 512-519,
+ * 529-533): time_flux_in/out are [cells][nang] of this (corner, group);
+ * qim (NULL unless MMS) is the angular source [cells][nang] added to psi
+ * (sweep.cc:347-351). */
+static int sweep_impl(int nx, int ny, int nz, int nang, int cmom, int corner, double hi,
+                      double hj, double hk, const double *mu, const double *eta,
+                      const double *xi, const double *w, const double *ec,
+                      const double *qtot_g, const double *dinv_g, const double *t_xs_g,
+                      int fixup, const double *ghost_x_in, const double *ghost_y_in,
+                      const double *ghost_z_in, double *ghost_x_out, double *ghost_y_out,
+                      double *ghost_z_out, double *part_flux, double *part_fluxm,
+                      double vdelt, const double *time_flux_in, double *time_flux_out,
+                      const double *qim_g) {
   const int sxp = (corner & 1) != 0, syp = (corner & 2) != 0, szp = (corner & 4) != 0;
   const size_t ab = (size_t)nang;
-  double *buf = (double *)malloc(sizeof(double) * ab * 13);
+  double *buf = (double *)malloc(sizeof(double) * ab * 16);
   double *psi = buf, *pc = buf + ab, *psii = buf + 2 * ab, *psij = buf + 3 * ab,
          *psik = buf + 4 * ab, *hv_x = buf + 5 * ab, *hv_y = buf + 6 * ab,
          *hv_z = buf + 7 * ab, *fx_hv_x = buf + 8 * ab, *fx_hv_y = buf + 9 * ab,
-         *fx_hv_z = buf + 10 * ab, *tmp = buf + 11 * ab;
+         *fx_hv_z = buf + 10 * ab, *tmp = buf + 11 * ab, *hv_t = buf + 12 * ab,
+         *fx_hv_t = buf + 13 * ab, *tin = buf + 14 * ab;
   double *yflux_pencil = (double *)malloc(sizeof(double) * ab * nx);
   double *zflux_plane = (double *)malloc(sizeof(double) * ab * nx * ny);
   const double *ecb = ec + (size_t)corner * cmom * nang;
@@ -239,6 +243,9 @@ int snap_oracle_sweep(int nx, int ny, int nz, int nang, int cmom, int corner,
         for (int a = 0; a < nang; a++) psi[a] = quad[0];
         for (int l = 1; l < cmom; l++)
           for (int a = 0; a < nang; a++) psi[a] += ecb[l * nang + a] * quad[l];
+        /* MMS angular source (sweep.cc:347-351) */
+        if (qim_g)
+          for (int a = 0; a < nang; a++) psi[a] += qim_g[c * ab + a];
         for (int a = 0; a < nang; a++) pc[a] = psi[a];
         /* x inflow (sweep.cc:356-364) */
         if (x == 0) {
@@ -268,17 +275,23 @@ int snap_oracle_sweep(int nx, int ny, int nz, int nang, int cmom, int corner,
           memcpy(psik, zflux_plane + ((size_t)y * nx + x) * ab, sizeof(double) * ab);
         }
         for (int a = 0; a < nang; a++) pc[a] += psik[a] * xi[a] * hk;
-        /* multiply by dinv (sweep.cc:397-400) */
+        /* time-edge inflow (sweep.cc:414-419) */
+        if (vdelt != 0.0) {
+          memcpy(tin, time_flux_in + c * ab, sizeof(double) * ab);
+          for (int a = 0; a < nang; a++) pc[a] += vdelt * tin[a];
+        }
+        /* multiply by dinv (sweep.cc:420-423) */
         const double *dinv = dinv_g + c * ab;
         for (int a = 0; a < nang; a++) pc[a] *= dinv[a];
 
         if (fixup) {
-          /* negative flux fixup, sweep.cc:402-505 (vdelt == 0 branch) */
+          /* negative flux fixup, sweep.cc:424-520 */
           unsigned old_negative_fluxes = 0;
           int ran = 0;
           for (int a = 0; a < nang; a++) hv_x[a] = 1.0;
           for (int a = 0; a < nang; a++) hv_y[a] = 1.0;
           for (int a = 0; a < nang; a++) hv_z[a] = 1.0;
+          for (int a = 0; a < nang; a++) hv_t[a] = 1.0;
           const double t_xs = t_xs_g[c];
           while (1) {
             unsigned negative_fluxes = 0;
@@ -303,32 +316,62 @@ int snap_oracle_sweep(int nx, int ny, int nz, int nang, int cmom, int corner,
                 negative_fluxes++;
               }
             }
+            if (vdelt != 0.0) {
+              for (int a = 0; a < nang; a++) {
+                fx_hv_t[a] = 2.0 * pc[a] - tin[a];
+                if (fx_hv_t[a] < 0.0) {
+                  hv_t[a] = 0.0;
+                  negative_fluxes++;
+                }
+              }
+            }
             if (negative_fluxes == old_negative_fluxes) break;
             old_negative_fluxes = negative_fluxes;
             ran = 1;
-            for (int a = 0; a < nang; a++) {
-              pc[a] = psi[a] + 0.5 * (psii[a] * mu[a] * hi * (1.0 + hv_x[a]) +
-                                      psij[a] * eta[a] * hj * (1.0 + hv_y[a]) +
-                                      psik[a] * xi[a] * hk * (1.0 + hv_z[a]));
-              double den = (pc[a] <= 0.0) ? 0.0
-                                          : (t_xs + mu[a] * hi * hv_x[a] +
-                                             eta[a] * hj * hv_y[a] +
-                                             xi[a] * hk * hv_z[a]);
-              if (den < tolr)
-                pc[a] = 0.0;
-              else
-                pc[a] /= den;
+            if (vdelt != 0.0) {
+              for (int a = 0; a < nang; a++) {
+                pc[a] = psi[a] + 0.5 * (psii[a] * mu[a] * hi * (1.0 + hv_x[a]) +
+                                        psij[a] * eta[a] * hj * (1.0 + hv_y[a]) +
+                                        psik[a] * xi[a] * hk * (1.0 + hv_z[a]) +
+                                        tin[a] * vdelt * (1.0 + hv_t[a]));
+                double den = (pc[a] <= 0.0) ? 0.0
+                                            : (t_xs + mu[a] * hi * hv_x[a] +
+                                               eta[a] * hj * hv_y[a] +
+                                               xi[a] * hk * hv_z[a] + vdelt * hv_t[a]);
+                if (den < tolr)
+                  pc[a] = 0.0;
+                else
+                  pc[a] /= den;
+              }
+            } else {
+              for (int a = 0; a < nang; a++) {
+                pc[a] = psi[a] + 0.5 * (psii[a] * mu[a] * hi * (1.0 + hv_x[a]) +
+                                        psij[a] * eta[a] * hj * (1.0 + hv_y[a]) +
+                                        psik[a] * xi[a] * hk * (1.0 + hv_z[a]));
+                double den = (pc[a] <= 0.0) ? 0.0
+                                            : (t_xs + mu[a] * hi * hv_x[a] +
+                                               eta[a] * hj * hv_y[a] +
+                                               xi[a] * hk * hv_z[a]);
+                if (den < tolr)
+                  pc[a] = 0.0;
+                else
+                  pc[a] /= den;
+              }
             }
           }
           fixed_cells += ran;
           for (int a = 0; a < nang; a++) psii[a] = fx_hv_x[a] * hv_x[a];
           for (int a = 0; a < nang; a++) psij[a] = fx_hv_y[a] * hv_y[a];
           for (int a = 0; a < nang; a++) psik[a] = fx_hv_z[a] * hv_z[a];
+          if (vdelt != 0.0)
+            for (int a = 0; a < nang; a++) time_flux_out[c * ab + a] = fx_hv_t[a] * hv_t[a];
         } else {
           /* sweep.cc:521-534 */
           for (int a = 0; a < nang; a++) psii[a] = 2.0 * pc[a] - psii[a];
           for (int a = 0; a < nang; a++) psij[a] = 2.0 * pc[a] - psij[a];
           for (int a = 0; a < nang; a++) psik[a] = 2.0 * pc[a] - psik[a];
+          if (vdelt != 0.0)
+            for (int a = 0; a < nang; a++) time_flux_out[c * ab + a] = 2.0 * pc[a] - tin[a];
         }
         /* outflow faces (sweep.cc:536-566) */
         if (x == nx - 1 && ghost_x_out)
@@ -369,6 +412,34 @@ int snap_oracle_sweep(int nx, int ny, int nz, int nang, int cmom, int corner,
   return fixed_cells;
 }
 
+int snap_oracle_sweep(int nx, int ny, int nz, int nang, int cmom, int corner,
+                      double hi, double hj, double hk, const double *mu,
+                      const double *eta, const double *xi, const double *w,
+                      const double *ec, const double *qtot_g,
+                      const double *dinv_g, const double *t_xs_g, int fixup,
+                      const double *ghost_x_in, const double *ghost_y_in,
+                      const double *ghost_z_in, double *ghost_x_out,
+                      double *ghost_y_out, double *ghost_z_out,
+                      double *part_flux, double *part_fluxm) {
+  return sweep_impl(nx, ny, nz, nang, cmom, corner, hi, hj, hk, mu, eta, xi, w, ec, qtot_g,
+                    dinv_g, t_xs_g, fixup, ghost_x_in, ghost_y_in, ghost_z_in, ghost_x_out,
+                    ghost_y_out, ghost_z_out, part_flux, part_fluxm, 0.0, NULL, NULL, NULL);
+}
+
+int snap_oracle_sweep_td(int nx, int ny, int nz, int nang, int cmom, int corner, double hi,
+                         double hj, double hk, const double *mu, const double *eta,
+                         const double *xi, const double *w, const double *ec,
+                         const double *qtot_g, const double *dinv_g, const double *t_xs_g,
+                         int fixup, double vdelt, const double *time_flux_in,
+                         double *time_flux_out, const double *qim_g, double *ghost_x_out,
+                         double *ghost_y_out, double *ghost_z_out, double *part_flux,
+                         double *part_fluxm) {
+  return sweep_impl(nx, ny, nz, nang, cmom, corner, hi, hj, hk, mu, eta, xi, w, ec, qtot_g,
+                    dinv_g, t_xs_g, fixup, NULL, NULL, NULL, ghost_x_out, ghost_y_out,
+                    ghost_z_out, part_flux, part_fluxm, vdelt, time_flux_in, time_flux_out,
+                    qim_g);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Solve loop (snap.cc:225-568, steady state, one time step)                 */
 /* ------------------------------------------------------------------------ */
@@ -379,7 +450,16 @@ typedef struct {
   double *mu, *eta, *xi, *w, *ec;
   double *t_xs, *s_xs, *dinv;
   double *part_flux, *part_fluxm; /* [8][ng][cells] and [8][ng][cells][cmom-1] */
+  double dt;                      /* tf / nsteps (snap.cc:1174) */
+  double *vdelt;                  /* [ng] */
+  /* time-dependent sweeps: the step's inflow and outflow time-edge fluxes,
+   * [8][ng][cells][nang] each (NULL when vdelt is zero) */
+  const double *tflux_in;
+  double *tflux_out;
+  const double *qim;              /* MMS angular source [8][ng][cells][nang] or NULL */
 } oracle_state_t;
+
+static int problem_steps(const oracle_problem_t *p) { return p->nsteps > 0 ? p->nsteps : 1; }
 
 static int state_init(oracle_state_t *s, const oracle_problem_t *p) {
   memset(s, 0, sizeof(*s));
@@ -408,7 +488,13 @@ static int state_init(oracle_state_t *s, const oracle_problem_t *p) {
   s->dinv = (double *)malloc(sizeof(double) * gc * na);
   s->part_flux = (double *)malloc(sizeof(double) * 8 * gc);
   s->part_fluxm = s->cmom > 1 ? (double *)malloc(sizeof(double) * 8 * gc * (s->cmom - 1)) : NULL;
-  if (!s->t_xs || !s->s_xs || !s->dinv || !s->part_flux) return -1;
+  s->vdelt = (double *)calloc((size_t)s->ng, sizeof(double));
+  if (!s->t_xs || !s->s_xs || !s->dinv || !s->part_flux || !s->vdelt) return -1;
+  /* Snap::initialize_velocity (snap.cc:716-738): v_g = ng - g and
+   * vdelt_g = 2 / (dt v_g) when time dependent, else 0 */
+  s->dt = p->tf / (double)problem_steps(p);
+  if (p->timedep)
+    for (int g = 0; g < s->ng; g++) s->vdelt[g] = 2.0 / (s->dt * (double)(s->ng - g));
   /* ExpandCrossSection / ExpandScatteringCrossSection / CalculateGeometryParam
    * (snap.cc:419-450); a per-cell total cross section replaces the gather. */
   if (p->sigt_cell)
@@ -416,7 +502,7 @@ static int state_init(oracle_state_t *s, const oracle_problem_t *p) {
   else
     snap_oracle_expand_xs(s->ncells, s->ng, p->mat, p->sigt, s->t_xs);
   snap_oracle_expand_scat_xs(s->ncells, s->ng, p->nmat, s->nmom, p->mat, p->slgg, s->s_xs);
-  snap_oracle_geometry_param(s->ncells, s->ng, p->nang, s->t_xs, NULL, s->hi, s->hj, s->hk,
+  snap_oracle_geometry_param(s->ncells, s->ng, p->nang, s->t_xs, s->vdelt, s->hi, s->hj, s->hk,
                              s->mu, s->eta, s->xi, s->dinv);
   return 0;
 }
@@ -432,6 +518,7 @@ static void state_free(oracle_state_t *s) {
   free(s->dinv);
   free(s->part_flux);
   free(s->part_fluxm);
+  free(s->vdelt);
 }
 
 /* perform_sweeps, snap.cc:849-887: 8 corners x all groups, independent tasks.
@@ -444,12 +531,16 @@ static int perform_sweeps(oracle_state_t *s, int fixup, const double *qtot,
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : fixed)
   for (int task = 0; task < 8 * ng; task++) {
     const int corner = task / ng, g = task % ng;
-    fixed += snap_oracle_sweep(
+    const size_t ta = ((size_t)corner * gc + (size_t)g * nc) * s->nang;
+    const double vd = s->vdelt[g];
+    fixed += sweep_impl(
         s->nx, s->ny, s->nz, s->nang, cm, corner, s->hi, s->hj, s->hk, s->mu, s->eta, s->xi,
         s->w, s->ec, qtot + (size_t)g * nc * cm, s->dinv + (size_t)g * nc * s->nang,
         s->t_xs + (size_t)g * nc, fixup, NULL, NULL, NULL, NULL, NULL, NULL,
         s->part_flux + (size_t)corner * gc + (size_t)g * nc,
-        cm > 1 ? s->part_fluxm + ((size_t)corner * gc + (size_t)g * nc) * (cm - 1) : NULL);
+        cm > 1 ? s->part_fluxm + ((size_t)corner * gc + (size_t)g * nc) * (cm - 1) : NULL, vd,
+        vd != 0.0 ? s->tflux_in + ta : NULL, vd != 0.0 ? s->tflux_out + ta : NULL,
+        s->qim ? s->qim + ta : NULL);
   }
 #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < gc; i++) {
@@ -469,6 +560,141 @@ static int perform_sweeps(oracle_state_t *s, int fixup, const double *qtot,
   return fixed;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Method of manufactured solutions (mms.cc)                                */
+/* ------------------------------------------------------------------------ */
+
+#define MMS_PI 3.14159265358979 /* the reference's PI (snap.h:37) */
+
+/* mms_trigint (mms.cc:34-57): cell averages over the edges cb[0..lc] of
+ * sin(d x) (COS form: (cos(d cb_i) - cos(d cb_i+1)) / (d del)) and of
+ * d cos(d x) (SIN form: (sin(d cb_i+1) - sin(d cb_i)) / del). */
+static void mms_trigint(int cos_form, int lc, double d, double del, const double *cb, double *fn) {
+  const double denom = d * del;
+  for (int i = 0; i < lc; i++) {
+    if (cos_form) {
+      fn[i] = cos(d * cb[i]) - cos(d * cb[i + 1]);
+      fn[i] /= denom;
+    } else {
+      fn[i] = sin(d * cb[i + 1]) - sin(d * cb[i]);
+      fn[i] /= del;
+    }
+  }
+}
+
+/* Cell edges of one axis from 0 in steps of del, summed as mms.cc:108-117
+ * does (the reference steps the y and z edges by dx; for the cubic cells of
+ * every reference deck that is the same as dy and dz). */
+static void mms_edges(int n, double del, double *b) {
+  b[0] = 0.0;
+  for (int i = 1; i <= n; i++) b[i] = b[i - 1] + del;
+}
+
+int snap_oracle_mms_setup(const oracle_problem_t *p, double *ref_flux, double *qim, double *qi) {
+  if (p->nmom != 1) return -1; /* the reference's MMS moments are defined for nmom = 1 only */
+  const int nx = p->nx, ny = p->ny, nz = p->nz, ng = p->ng, nang = p->nang;
+  const size_t nc = (size_t)nx * ny * nz, gc = (size_t)ng * nc;
+  double *mu = (double *)malloc(sizeof(double) * nang), *eta = (double *)malloc(sizeof(double) * nang);
+  double *xi = (double *)malloc(sizeof(double) * nang), *w = (double *)malloc(sizeof(double) * nang);
+  double *ec = (double *)malloc(sizeof(double) * 8 * nang);
+  double *ib = (double *)malloc(sizeof(double) * (nx + 1)), *jb = (double *)malloc(sizeof(double) * (ny + 1));
+  double *kb = (double *)malloc(sizeof(double) * (nz + 1));
+  double *cx = (double *)malloc(sizeof(double) * nx), *sx = (double *)malloc(sizeof(double) * nx);
+  double *cy = (double *)malloc(sizeof(double) * ny), *sy = (double *)malloc(sizeof(double) * ny);
+  double *cz = (double *)malloc(sizeof(double) * nz), *sz = (double *)malloc(sizeof(double) * nz);
+  snap_oracle_quadrature(nang, 1, mu, eta, xi, w, ec);
+  const double a = MMS_PI / p->lx, b = MMS_PI / p->ly, cc = MMS_PI / p->lz;
+  const double dx = p->lx / (double)nx, dy = p->ly / (double)ny, dz = p->lz / (double)nz;
+  mms_edges(nx, dx, ib);
+  mms_edges(ny, dy, jb);
+  mms_edges(nz, dz, kb);
+  mms_trigint(1, nx, a, dx, ib, cx);
+  mms_trigint(0, nx, a, dx, ib, sx);
+  mms_trigint(1, ny, b, dy, jb, cy);
+  mms_trigint(0, ny, b, dy, jb, sy);
+  mms_trigint(1, nz, cc, dz, kb, cz);
+  mms_trigint(0, nz, cc, dz, kb, sz);
+  /* MMSInitFlux (mms.cc:81-148): phi_g = g * tx * ty * tz, g 1-based */
+  for (int g = 0; g < ng; g++)
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++)
+          ref_flux[(size_t)g * nc + ((size_t)k * ny + j) * nx + i] =
+              (double)(g + 1) * cx[i] * cy[j] * cz[k];
+  /* MMSInitSource (mms.cc:206-344), per corner: the streaming, collision
+   * and scattering terms of the manufactured flux */
+  for (int corner = 0; corner < 8; corner++) {
+    const double is = (corner & 1) ? 1.0 : -1.0, js = (corner & 2) ? 1.0 : -1.0,
+                 ks = (corner & 4) ? 1.0 : -1.0;
+    for (int g = 0; g < ng; g++)
+      for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+          for (int i = 0; i < nx; i++) {
+            const size_t c = ((size_t)k * ny + j) * nx + i;
+            const int m = p->mat[c];
+            const double sigt = p->sigt[(size_t)m * ng + g];
+            const double flux_update = sigt * ref_flux[(size_t)g * nc + c];
+            double *q = qim + (((size_t)corner * ng + g) * nc + c) * nang;
+            for (int an = 0; an < nang; an++) {
+              double v = 0.0;
+              v += (double)(g + 1) * is * mu[an] * sx[i] * cy[j] * cz[k];
+              v += flux_update;
+              v += (double)(g + 1) * js * eta[an] * cx[i] * sy[j] * cz[k];
+              v += (double)(g + 1) * ks * xi[an] * cx[i] * cy[j] * sz[k];
+              for (int gp = 0; gp < ng; gp++)
+                v -= p->slgg[(((size_t)g * p->nmat + m) * ng + gp) * p->nmom] *
+                     ref_flux[(size_t)gp * nc + c];
+              q[an] = v;
+            }
+          }
+  }
+  /* MMSInitTimeDependent (mms.cc:382-409): qi = phi / v_g, phi scaled to
+   * the last step's mid-time tf - dt/2 */
+  if (p->timedep) {
+    const double dt = p->tf / (double)problem_steps(p);
+    const double t_scale = p->tf - 0.5 * dt;
+    for (int g = 0; g < ng; g++) {
+      const double vg = (double)(ng - g);
+      for (size_t c = 0; c < nc; c++) {
+        const double f = ref_flux[(size_t)g * nc + c];
+        qi[(size_t)g * nc + c] = f / vg;
+        ref_flux[(size_t)g * nc + c] = f * t_scale;
+      }
+    }
+  } else {
+    memset(qi, 0, sizeof(double) * gc);
+  }
+  free(mu); free(eta); free(xi); free(w); free(ec);
+  free(ib); free(jb); free(kb);
+  free(cx); free(sx); free(cy); free(sy); free(cz); free(sz);
+  return 0;
+}
+
+/* MMSCompare (mms.cc:497-541): df = |phi/ref - 1| (|phi| where ref < 1e-12);
+ * result = {max, min, sum / n}. */
+void snap_oracle_mms_compare(int n, const double *flux, const double *ref, double result[3]) {
+  const double tolr = 1.0e-12;
+  double mx = -INFINITY, mn = INFINITY, sum = 0.0;
+  for (int i = 0; i < n; i++) {
+    double r = ref[i], df = 1.0;
+    if (r < tolr) {
+      r = 1.0;
+      df = 0.0;
+    }
+    df = fabs(flux[i] / r - df);
+    if (df > mx) mx = df;
+    if (df < mn) mn = df;
+    sum += df;
+  }
+  result[0] = mx;
+  result[1] = mn;
+  result[2] = n > 0 ? sum / (double)n : 0.0;
+}
+
+/* Snap::transport_solve (snap.cc:386-541): time steps of the outer/inner
+ * loop.  The flux is not reset between steps; the time-edge fluxes
+ * alternate between two arrays (time_flux_even / _odd, snap.cc:288-298,
+ * 497-498) that start at zero. */
 int snap_oracle_solve(const oracle_problem_t *p, double *flux0, double *fluxm,
                       oracle_stats_t *stats, int nthreads) {
 #ifdef _OPENMP
@@ -483,48 +709,80 @@ int snap_oracle_solve(const oracle_problem_t *p, double *flux0, double *fluxm,
     state_free(&s);
     return -1;
   }
-  const int ng = s.ng, nc = s.ncells, cm = s.cmom;
-  const size_t gc = (size_t)ng * nc;
+  const int ng = s.ng, nc = s.ncells, cm = s.cmom, nsteps = problem_steps(p);
+  const size_t gc = (size_t)ng * nc, ta = 8 * gc * (size_t)s.nang;
   double *flux0po = (double *)calloc(gc, sizeof(double));
   double *flux0pi = (double *)calloc(gc, sizeof(double));
   double *q2grp0 = (double *)calloc(gc, sizeof(double));
   double *qtot = (double *)calloc(gc * cm, sizeof(double));
   double *q2grpm = cm > 1 ? (double *)calloc(gc * (cm - 1), sizeof(double)) : NULL;
   double *fluxm_local = NULL;
+  double *tflux[2] = {NULL, NULL}, *qim = NULL, *qi_mms = NULL, *ref = NULL;
+  const double *qi = p->qi;
   if (cm > 1 && !fluxm) fluxm = fluxm_local = (double *)calloc(gc * (cm - 1), sizeof(double));
+  if (p->timedep) {
+    tflux[0] = (double *)calloc(ta, sizeof(double));
+    tflux[1] = (double *)calloc(ta, sizeof(double));
+  }
+  int rc = 0;
+  if (p->mms) {
+    qim = (double *)malloc(sizeof(double) * ta);
+    qi_mms = (double *)malloc(sizeof(double) * gc);
+    ref = (double *)malloc(sizeof(double) * gc);
+    rc = snap_oracle_mms_setup(p, ref, qim, qi_mms);
+    qi = qi_mms;
+    s.qim = qim;
+  }
   memset(flux0, 0, sizeof(double) * gc);
   if (cm > 1) memset(fluxm, 0, sizeof(double) * gc * (cm - 1));
 
   const double t0 = now_seconds();
-  int inner_converged = 0;
-  for (int otno = 0; otno < p->oitm; otno++) {
-    /* CalcOuterSource, then save_fluxes flux0 -> flux0po (snap.cc:477-481) */
-    snap_oracle_outer_source(nc, ng, p->nmat, s.nmom, p->mat, p->qi, p->slgg, flux0, fluxm,
-                             q2grp0, q2grpm);
-    memcpy(flux0po, flux0, sizeof(double) * gc);
-    st.outer_total++;
-    inner_converged = 0;
-    for (int inno = 0; inno < p->iitm; inno++) {
-      /* snap.cc:490-503 */
-      snap_oracle_inner_source(nc, ng, s.nmom, s.s_xs, flux0, q2grp0, fluxm, q2grpm, qtot);
-      memcpy(flux0pi, flux0, sizeof(double) * gc);
-      const double ts = now_seconds();
-      st.fixup_cells += perform_sweeps(&s, p->fixup, qtot, flux0, fluxm);
-      st.sweep_seconds += now_seconds() - ts;
-      st.inner_total++;
-      inner_converged = snap_oracle_converged((int)gc, flux0, flux0pi, p->epsi);
-      if (inner_converged) break;
+  int all_converged = 1;
+  for (int cy = 0; cy < nsteps && rc == 0; cy++) {
+    /* even steps read time_flux_even and write _odd; odd steps the reverse */
+    if (p->timedep) {
+      s.tflux_in = tflux[cy & 1];
+      s.tflux_out = tflux[(cy & 1) ^ 1];
     }
-    /* snap.cc:519-538: the outer test is skipped on the first outer and
-     * fails whenever the last inner did not converge (outer.cc:362-365). */
-    if (otno == 0) continue;
-    const int outer_converged =
-        inner_converged && snap_oracle_converged((int)gc, flux0, flux0po, 100.0 * p->epsi);
-    if (outer_converged) {
-      st.converged = 1;
-      break;
+    /* MMSScale (snap.cc:452-467): the manufactured source at the step's
+     * mid-time, scaled in place step after step */
+    if (p->mms) {
+      const double f = (cy == 0) ? s.dt * 0.5 : (double)(2 * cy + 1) / (double)(2 * cy - 1);
+      for (size_t i = 0; i < ta; i++) qim[i] *= f;
     }
+    int inner_converged = 0, converged = 0;
+    for (int otno = 0; otno < p->oitm; otno++) {
+      /* CalcOuterSource, then save_fluxes flux0 -> flux0po (snap.cc:477-481) */
+      snap_oracle_outer_source(nc, ng, p->nmat, s.nmom, p->mat, qi, p->slgg, flux0, fluxm,
+                               q2grp0, q2grpm);
+      memcpy(flux0po, flux0, sizeof(double) * gc);
+      st.outer_total++;
+      inner_converged = 0;
+      for (int inno = 0; inno < p->iitm; inno++) {
+        /* snap.cc:490-503 */
+        snap_oracle_inner_source(nc, ng, s.nmom, s.s_xs, flux0, q2grp0, fluxm, q2grpm, qtot);
+        memcpy(flux0pi, flux0, sizeof(double) * gc);
+        const double ts = now_seconds();
+        st.fixup_cells += perform_sweeps(&s, p->fixup, qtot, flux0, fluxm);
+        st.sweep_seconds += now_seconds() - ts;
+        st.inner_total++;
+        inner_converged = snap_oracle_converged((int)gc, flux0, flux0pi, p->epsi);
+        if (inner_converged) break;
+      }
+      /* snap.cc:519-538: the outer test is skipped on the first outer and
+       * fails whenever the last inner did not converge (outer.cc:362-365). */
+      if (otno == 0) continue;
+      const int outer_converged =
+          inner_converged && snap_oracle_converged((int)gc, flux0, flux0po, 100.0 * p->epsi);
+      if (outer_converged) {
+        converged = 1;
+        break;
+      }
+    }
+    all_converged &= converged;
+    st.time_steps++;
   }
+  st.converged = all_converged && rc == 0;
   st.solve_seconds = now_seconds() - t0;
   if (stats) *stats = st;
   free(flux0po);
@@ -533,8 +791,13 @@ int snap_oracle_solve(const oracle_problem_t *p, double *flux0, double *fluxm,
   free(qtot);
   free(q2grpm);
   free(fluxm_local);
+  free(tflux[0]);
+  free(tflux[1]);
+  free(qim);
+  free(qi_mms);
+  free(ref);
   state_free(&s);
-  return 0;
+  return rc;
 }
 
 double snap_oracle_time_sweeps(const oracle_problem_t *p, int ninner, int nthreads) {
@@ -543,6 +806,7 @@ double snap_oracle_time_sweeps(const oracle_problem_t *p, int ninner, int nthrea
 #else
   (void)nthreads;
 #endif
+  if (p->timedep || p->mms) return -1.0; /* steady-state sweeps only */
   oracle_state_t s;
   if (state_init(&s, p)) {
     state_free(&s);
@@ -742,7 +1006,7 @@ int snap_oracle_adjoint(const oracle_problem_t *p, const double *flux0, const do
 #else
   (void)nthreads;
 #endif
-  if (p->nmom != 1 || p->nang > 256) return -1;
+  if (p->nmom != 1 || p->nang > 256 || p->timedep || p->mms) return -1;
   oracle_state_t s;
   if (state_init(&s, p)) {
     state_free(&s);

@@ -39,6 +39,10 @@ typedef struct {
   double epsi;
   int iitm, oitm;
   int fixup;
+  int timedep;              /* time dependent: vdelt_g = 2/(dt (ng-g)) (snap.cc:716-738) */
+  double tf;                /* total simulated time; dt = tf / nsteps (snap.cc:1174) */
+  int nsteps;               /* time steps (0 counts as 1) */
+  int mms;                  /* manufactured-solution source (src_opt 3): qi is ignored */
 } oracle_problem_t;
 
 typedef struct {
@@ -48,6 +52,7 @@ typedef struct {
   int fixup_cells;          /* (cell,octant,group) sweeps whose fixup loop changed pc, summed over the solve */
   double sweep_seconds;     /* wall time spent in sweeps */
   double solve_seconds;     /* wall time of the whole outer/inner loop */
+  int time_steps;           /* time steps run; converged = every step converged */
 } oracle_stats_t;
 
 /* compute_derived_globals, snap.cc:1171-1342 (3-D branch). */
@@ -99,7 +104,30 @@ int snap_oracle_sweep(int nx, int ny, int nz, int nang, int cmom, int corner,
                       double *ghost_y_out, double *ghost_z_out,
                       double *part_flux, double *part_fluxm);
 
-/* transport_solve steady-state loop, snap.cc:468-540.  flux0 [ng][cells],
+/* The sweep above from vacuum inflow, with the time-dependent terms (vdelt
+ * != 0: inflow time_flux_in and outflow time_flux_out, [cells][nang] of this
+ * corner and group) and the MMS angular source qim_g ([cells][nang] or
+ * NULL); ghost_*_out as above (optional). */
+int snap_oracle_sweep_td(int nx, int ny, int nz, int nang, int cmom, int corner, double hi,
+                         double hj, double hk, const double *mu, const double *eta,
+                         const double *xi, const double *w, const double *ec,
+                         const double *qtot_g, const double *dinv_g, const double *t_xs_g,
+                         int fixup, double vdelt, const double *time_flux_in,
+                         double *time_flux_out, const double *qim_g, double *ghost_x_out,
+                         double *ghost_y_out, double *ghost_z_out, double *part_flux,
+                         double *part_fluxm);
+
+/* MMSInitFlux, MMSInitSource and MMSInitTimeDependent (mms.cc:81-409) for
+ * nmom = 1: ref_flux [ng][cells] (scaled to tf - dt/2 when time dependent),
+ * the unscaled angular source qim [8][ng][cells][nang] and the fixed source
+ * qi [ng][cells] (phi/v_g when time dependent, else 0).  Returns -1 for
+ * nmom != 1. */
+int snap_oracle_mms_setup(const oracle_problem_t *p, double *ref_flux, double *qim, double *qi);
+/* MMSCompare (mms.cc:497-541): result = {max, min, mean} of |phi/ref - 1|. */
+void snap_oracle_mms_compare(int n, const double *flux, const double *ref, double result[3]);
+
+/* transport_solve loop, snap.cc:386-541: nsteps time steps of the outer /
+ * inner iteration (one step when steady).  flux0 [ng][cells],
  * fluxm [ng][cells][cmom-1] (may be NULL when nmom == 1).  nthreads <= 0
  * means all OpenMP threads. */
 int snap_oracle_solve(const oracle_problem_t *p, double *flux0, double *fluxm,

@@ -1,0 +1,88 @@
+"""The input-deck reader (Snap::parse_arguments, snap.cc:1032-1168) and the
+reference's material, scattering and source set-up it restates
+(initialize_scattering snap.cc:571-714, InitMaterial / InitSource
+init.cc:49-191)."""
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from snap_amd import deck
+
+DECKS = os.path.join(os.path.dirname(__file__), "golden", "decks")
+
+
+def test_every_reference_deck_reads():
+    names = sorted(os.path.basename(f) for f in glob.glob(os.path.join(DECKS, "*.in")))
+    assert len(names) == 24
+    for n in names:
+        p, extra = deck.read_deck(os.path.join(DECKS, n))
+        p.validate()
+        assert p.qi.shape == (p.ng, p.nz, p.ny, p.nx)
+        assert p.timedep == bool(extra["timedep"]) and p.nsteps == extra["nsteps"]
+        assert p.mms == (extra["src_opt"] == 3)
+
+
+def test_steady_decks():
+    p, _ = deck.read_deck(os.path.join(DECKS, "base.in"))
+    assert (p.nx, p.ny, p.nz, p.ng, p.nang, p.nmom) == (6, 6, 6, 4, 10, 1)
+    assert not p.timedep and not p.mms and p.fixup and p.oitm == 30 and p.nmat == 1
+    assert np.all(p.qi == 1.0)
+    p, _ = deck.read_deck(os.path.join(DECKS, "tiny_512_1x1x1.in"))
+    assert p.timedep and p.nsteps == 10 and (p.ng, p.nang) == (16, 32)
+
+
+@pytest.mark.parametrize("ng,nmom", [(1, 1), (2, 2), (5, 4)])
+def test_scattering_conserves_sigs(ng, nmom):
+    """Each source group's scattering sums to sigs over destination groups
+    (in-group + up + down fractions are 0.2+0.1+0.7, 0.3+0.7, 0.9+0.1 for
+    material 1 and 0.5+0.1+0.4, 0.6+0.4, 0.9+0.1 for material 2), and
+    Legendre order l is the order-0 table times 0.1, 0.05, 0.025 (material 1)
+    or 0.8, 0.48, 0.288 (material 2)."""
+    sigt, siga, sigs, slgg = deck.reference_cross_sections(ng, nmom, True)
+    np.testing.assert_allclose(slgg[:, :, :, 0].sum(axis=0), sigs, rtol=1e-14)
+    g = np.arange(ng)
+    np.testing.assert_allclose(sigt[0], 0.01 ** g, rtol=1e-12)
+    np.testing.assert_allclose(siga[1], 0.8 * 0.005 ** g, rtol=1e-12)
+    r1, r2 = [1.0, 0.1, 0.05, 0.025], [1.0, 0.8, 0.48, 0.288]
+    for l in range(nmom):
+        np.testing.assert_allclose(slgg[:, 0, :, l], r1[l] * slgg[:, 0, :, 0], rtol=1e-14)
+        np.testing.assert_allclose(slgg[:, 1, :, l], r2[l] * slgg[:, 1, :, 0], rtol=1e-14)
+    if ng > 2:
+        # downscatter from group 0 is spread evenly over the lower groups
+        np.testing.assert_allclose(slgg[1:, 0, 0, 0], 0.7 * sigs[0, 0] / (ng - 1), rtol=1e-14)
+        # upscatter into group 0 from the last group is 0.1 sigs / (ng - 1)
+        assert np.isclose(slgg[0, 0, ng - 1, 0], 0.1 * sigs[0, ng - 1] / (ng - 1), rtol=1e-14)
+
+
+def test_layouts():
+    mat, qi = deck.reference_layouts(8, 8, 4, 2, 1, 1)
+    # CENTER: 1-based [n/4+1, 3n/4] per axis
+    assert mat.sum() == 4 * 4 * 2 and mat[1:3, 2:6, 2:6].all() and qi[:, 1:3, 2:6, 2:6].all()
+    assert qi.sum() == 2 * 32
+    mat, qi = deck.reference_layouts(8, 8, 4, 2, 2, 2)
+    # CORNER: 1-based [1, n/2] per axis
+    assert mat.sum() == 4 * 4 * 2 and mat[:2, :4, :4].all() and qi[:, :2, :4, :4].all()
+    mat, qi = deck.reference_layouts(8, 8, 4, 2, 0, 3)
+    assert mat.sum() == 0 and qi.sum() == 0.0
+
+
+def test_bad_decks_are_refused(tmp_path):
+    good = open(os.path.join(DECKS, "base.in")).read()
+    # angcpy=1 and swp_typ=0 exit in the reference (snap.cc:1125-1137); the
+    # chunk count must divide the edge (snap.cc:1150-1152)
+    cases = {"angcpy": "angcpy = 1", "swp_typ": "swp_typ = 0", "ichunk": "ichunk = 4",
+             "epsi": "bogus = 1", "nx": "nx = 3"}
+    for key, new in cases.items():
+        f = tmp_path / "d.in"
+        f.write_text(re.sub(rf"^  {key} = .*$", "  " + new, good, flags=re.M))
+        with pytest.raises(ValueError):
+            deck.read_deck(str(f))
+
+
+def test_base_deck_solves(oracle_mod):
+    p, _ = deck.read_deck(os.path.join(DECKS, "base.in"))
+    flux, _, st = oracle_mod.solve(p)
+    assert st["converged"] and np.all(flux > 0)
