@@ -218,6 +218,16 @@ typedef struct {
   double epsi;
   int iitm, oitm;
   int fixup;
+  /* time dependence (Snap::initialize_velocity snap.cc:716-738, the time-step
+   * loop snap.cc:400-541): nsteps steps of dt = tf / nsteps with
+   * vdelt_g = 2 / (dt v_g), v_g = ng - g; timedep = 0 is one steady step */
+  int timedep;
+  double tf;
+  int nsteps;               /* >= 1 (0 counts as 1) */
+  /* manufactured-solution source (src_opt 3, mms.cc): the solver builds the
+   * fixed and angular sources itself and qi may be NULL (nmom = 1 and
+   * material cross sections only) */
+  int mms;
 } snap_problem_t;
 
 #define SNAP_DINV_DEFAULT 0     /* stream; MIXED reads the FP64-formed dinv rounded
@@ -255,6 +265,7 @@ typedef struct {
   double solve_ms;          /* GPU time of the whole loop */
   double units_per_sweep;   /* cells * 8*nang * ng */
   double grind_ns;          /* sweep_ms*1e6 / (units_per_sweep * inner_total) */
+  int time_steps;           /* time steps run; converged = every step converged */
 } snap_solve_stats_t;
 
 typedef struct snap_solver snap_solver_t;
@@ -263,8 +274,18 @@ typedef struct snap_solver snap_solver_t;
 int snap_hip_solver_create(const snap_problem_t *problem,
                            const snap_solve_options_t *options,
                            snap_solver_t **solver);
-/* Run the outer/inner loop from a zero flux to convergence or oitm. */
+/* Run the outer/inner loop from a zero flux to convergence or oitm, once per
+ * time step (the flux carries over between steps; the time-edge fluxes start
+ * at zero and alternate between two arrays, snap.cc:288-298, 497-498). */
 int snap_hip_solver_run(snap_solver_t *solver, snap_solve_stats_t *stats);
+/* MMSCompare (mms.cc:497-541, snap.cc:542-559) of the current flux against
+ * the manufactured solution: result = {max, min, mean} of |phi/ref - 1|
+ * (|phi| where ref < 1e-12) over all cells and groups.  The reference
+ * divides its sum by nx*ichunk * ny*npey * nz*npez cells (snap.cc:546);
+ * here the mean is over the ng*nx*ny*nz values.  Needs an mms problem. */
+int snap_hip_solver_mms_compare(snap_solver_t *solver, double result[3]);
+/* The manufactured scalar flux [ng][cells] the compare uses. */
+int snap_hip_solver_get_mms_reference(snap_solver_t *solver, double *ref_flux);
 /* Copy the scalar flux [ng][cells] and flux moments [ng][cells][cmom-1]
  * (may be NULL) to host memory. */
 int snap_hip_solver_get_flux(snap_solver_t *solver, double *flux0, double *fluxm);

@@ -379,6 +379,8 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
     if (q.nx < 1 || q.ny < 1 || q.nz < 1 || q.nmat < 1 || !q.mat || !q.sigt || !q.slgg || !q.qi)
       return fail(SNAP_ERR_INVALID, "problem %d is incomplete", p);
     if (!(q.lx > 0 && q.ly > 0 && q.lz > 0)) return fail(SNAP_ERR_INVALID, "problem %d: bad lengths", p);
+    if (q.timedep || q.mms)
+      return fail(SNAP_ERR_UNSUPPORTED, "problem %d: batches are of steady fixed-source problems", p);
     b->ext.push_back(Extent3{q.nx, q.ny, q.nz});
     b->hi.push_back(2.0 / (q.lx / double(q.nx)));
     b->hj.push_back(2.0 / (q.ly / double(q.ny)));

@@ -160,6 +160,7 @@ struct SweepPlan {
   int cpw = 0;                 // cells per 64-lane wavefront
   int sb = 0;                  // cells per per-wave moment batch
   int maxdiag = 0;             // most cells on one cell diagonal of a tile
+  bool td = false;             // for the time-dependent kernels (512 threads at most)
   int nplanes = 0;             // tile diagonals
   int ncellplanes = 0;         // cell diagonals inside one tile
   std::vector<int> plane_off;  // host: offset of each tile diagonal in plane_tiles
@@ -170,13 +171,15 @@ struct SweepPlan {
   size_t face_doubles_per_sweep() const;  // x + y + z faces, all tiles
 };
 // apl = 0 picks the angles-per-lane count that fills the wavefront best.
+// td: a plan for the time-dependent kernels, whose workgroups stop at 512
+// threads (their extra per-lane state needs more than 128 registers).
 int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int TZ, int waves,
-                int apl = 0);
+                int apl = 0, bool td = false);
 void plan_free(SweepPlan *p);
 // Fill in the tile edges, waves per workgroup and angles per lane left at 0
 // for a sweep whose faces are real_bytes wide (8 FP64, 4 FP32).
 void plan_choose(Extent3 e, int nang, int cmom, int real_bytes, int *TX, int *TY, int *TZ,
-                 int *waves, int *apl);
+                 int *waves, int *apl, bool td = false);
 
 // One problem of a batch sweep: its chunk, tile grid, offsets into the
 // batch's concatenated arrays and face buffer, and cell spacing.
@@ -222,6 +225,16 @@ struct SweepLaunch {
   bool exact;                  // FP64 in the reference's operation order, no fused ops
   bool recompute_dinv;         // form dinv from t_xs in the kernel instead of reading it
   const BatchSweep *batch;     // non-NULL: a batch of problems (plan gives the tile shape)
+  // time-dependent terms (td): vdelt per job group (dense) or vdelt0 (general);
+  // time-edge fluxes in / out and the MMS angular source (NULL = none), dense
+  // [8][ng_job][cells][nang] doubles or per-cell accessors of nang doubles
+  bool td;
+  const double *vdelt;
+  double vdelt0;
+  const double *tin;
+  double *tout;
+  const double *qim;
+  Acc3 tin_acc, tout_acc, qim_acc;
 };
 // Enqueue every tile diagonal of the sweep (plan->nplanes launches).  When
 // `events` is given (2*nplanes events) each launch is bracketed by a pair.
@@ -236,5 +249,23 @@ int adjoint_enqueue(const SweepPlan &p, const QuadDev &quad, int ng, int fixup, 
                     double *part_u, double *part_grad, const void *faces, void *adj_faces,
                     bool fp32, hipStream_t st);
 int adjoint_plan_fits(const SweepPlan &p, int nang, int real_bytes, int *cpw, int *waves, int *sb);
+
+// Method of manufactured solutions (snap_mms.hip): host tables, reference
+// flux and fixed source; the angular source on the device.
+struct MmsState {
+  std::vector<double> ref;   // host [ng][cells], at tf - dt/2 when time dependent
+  std::vector<double> qi;    // host [ng][cells]: phi / v_g when time dependent, else 0
+  double *tab = nullptr;     // device: per-axis cell averages cx|sx|cy|sy|cz|sz
+  double *ref_dev = nullptr; // device: the unscaled manufactured flux [ng][cells]
+};
+int mms_prepare(const snap_problem_t &p, MmsState *m);
+void mms_free(MmsState *m);
+// qim [8][ng][cells][nang] = the unscaled angular source (MMSInitSource)
+int mms_source(const MmsState &m, const QuadDev &q, Extent3 e, int ng, int nmat, int nmom,
+               const int32_t *mat, const double *sigt, const double *slgg, double *qim,
+               hipStream_t st);
+int launch_scale(double *x, int64_t n, double f, hipStream_t st);
+// MMSCompare (mms.cc:497-541): {max, min, mean} of |phi/ref - 1|
+void mms_compare(const double *flux, const double *ref, size_t n, double result[3]);
 
 }  // namespace snap

@@ -15,6 +15,12 @@
 // All 8*ng sweeps of one inner iteration are one job: each tile-diagonal
 // launch covers every (corner, group) pair.  Sweeps write per-octant partial
 // fluxes, which are summed in octant order, so results are deterministic.
+//
+// Time dependence (snap.cc:386-541) repeats the loop once per time step:
+// dinv includes vdelt, the sweeps read the previous step's time-edge fluxes
+// and write this step's (two arrays that swap roles each step), the flux
+// carries over, and a manufactured-solution source is rescaled to the step's
+// mid-time before its outer iterations.
 #include <algorithm>
 #include <cstring>
 #include <memory>
@@ -66,8 +72,16 @@ struct snap_solver {
   DeviceBuffer qbase, flux0base, fluxmbase;  // defect correction: the outer's FP64 base
   DeviceBuffer adj_faces, part_grad;          // adjoint: reverse-sweep faces and partials
   bool have_fp32 = false;
+  // time dependence and the manufactured solution
+  bool td = false;                  // sweeps carry time terms or an angular source
+  std::vector<double> vdelt_h;      // [ng]: 2 / (dt v_g), or 0 when steady
+  DeviceBuffer vdelt_d, qim;        // [ng]; MMS angular source [8][ng][cells][nang]
+  DeviceBuffer tflux[2];            // time-edge fluxes [8][ng][cells][nang], even / odd
+  int step = 0;                     // time step in progress: tflux[step & 1] is the inflow
+  MmsState mms;
 
   ~snap_solver() {
+    mms_free(&mms);
     plan_free(&plan);
     plan_free(&plan64);
     quad_free(&quad);
@@ -113,6 +127,15 @@ struct snap_solver {
     L.exact = precision == SNAP_PRECISION_FP64_EXACT;
     L.recompute_dinv = recompute || opt.dinv_mode == SNAP_DINV_RECOMPUTE;
     if (fp32 && !L.recompute_dinv && !dinv32.p) return fail(SNAP_ERR_INVALID, "no FP32 dinv");
+    L.td = td;
+    if (td) {
+      L.vdelt = vdelt_d.as<double>();
+      if (prob.timedep) {
+        L.tin = tflux[step & 1].as<double>();
+        L.tout = tflux[(step & 1) ^ 1].as<double>();
+      }
+      L.qim = prob.mms ? qim.as<double>() : nullptr;
+    }
     return sweep_enqueue(L, stream, events);
   }
 
@@ -172,13 +195,22 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
   if (!(p->epsi > 0.0 && p->epsi < 1e-2)) return fail(SNAP_ERR_INVALID, "epsi must be in (0, 1e-2)");
   if (p->iitm < 1 || p->oitm < 1) return fail(SNAP_ERR_INVALID, "iitm and oitm must be >= 1");
   if (!(p->lx > 0.0 && p->ly > 0.0 && p->lz > 0.0)) return fail(SNAP_ERR_INVALID, "domain lengths must be positive");
-  if (!p->mat || !p->sigt || !p->slgg || !p->qi) return fail(SNAP_ERR_INVALID, "NULL problem table");
+  if (!p->mat || !p->sigt || !p->slgg || (!p->qi && !p->mms))
+    return fail(SNAP_ERR_INVALID, "NULL problem table");
+  if (p->timedep && !(p->tf > 0.0)) return fail(SNAP_ERR_INVALID, "a time-dependent problem needs tf > 0");
+  if (p->nsteps < 0) return fail(SNAP_ERR_INVALID, "nsteps must be >= 1");
   std::unique_ptr<snap_solver> s(new snap_solver());
   s->prob = *p;
   if (o) s->opt = *o;
   if (s->opt.precision != SNAP_PRECISION_FP64 && s->opt.precision != SNAP_PRECISION_MIXED &&
       s->opt.precision != SNAP_PRECISION_FP64_EXACT)
     return fail(SNAP_ERR_INVALID, "unknown precision %d", s->opt.precision);
+  if (s->prob.nsteps == 0) s->prob.nsteps = 1;
+  s->td = p->timedep || p->mms;
+  if (s->td && s->opt.refine)
+    return fail(SNAP_ERR_UNSUPPORTED, "defect correction of time-dependent or MMS problems");
+  if (s->td && s->opt.dinv_mode == SNAP_DINV_RECOMPUTE)
+    return fail(SNAP_ERR_UNSUPPORTED, "time-dependent sweeps read dinv (it includes vdelt)");
   s->e = Extent3{p->nx, p->ny, p->nz};
   s->ng = p->ng;
   s->nang = p->nang;
@@ -200,15 +232,15 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
   int TX = s->opt.tile[0], TY = s->opt.tile[1], TZ = s->opt.tile[2], waves = s->opt.waves_per_tile;
   int apl = s->opt.angles_per_lane;
   plan_choose(s->e, s->nang, s->cmom, s->opt.precision == SNAP_PRECISION_MIXED ? 4 : 8, &TX, &TY,
-              &TZ, &waves, &apl);
-  SNAP_TRY(plan_create(&s->plan, s->e, s->nang, s->cmom, TX, TY, TZ, waves, apl));
+              &TZ, &waves, &apl, s->td);
+  SNAP_TRY(plan_create(&s->plan, s->e, s->nang, s->cmom, TX, TY, TZ, waves, apl, s->td));
   if (s->opt.precision == SNAP_PRECISION_MIXED) {
     // the FP64 sweeps of a mixed solve (first sweep, refinement bases) get
     // their own plan unless the caller fixed the tiling
     int tx = s->opt.tile[0], ty = s->opt.tile[1], tz = s->opt.tile[2];
     int w = s->opt.waves_per_tile, a = s->opt.angles_per_lane;
-    plan_choose(s->e, s->nang, s->cmom, 8, &tx, &ty, &tz, &w, &a);
-    SNAP_TRY(plan_create(&s->plan64, s->e, s->nang, s->cmom, tx, ty, tz, w, a));
+    plan_choose(s->e, s->nang, s->cmom, 8, &tx, &ty, &tz, &w, &a, s->td);
+    SNAP_TRY(plan_create(&s->plan64, s->e, s->nang, s->cmom, tx, ty, tz, w, a, s->td));
   }
 
   SNAP_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -220,7 +252,24 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
   SNAP_TRY(s->upload(s->mat, p->mat, sizeof(int32_t) * s->cells));
   SNAP_TRY(s->upload(s->sigt, p->sigt, sizeof(double) * p->nmat * p->ng));
   SNAP_TRY(s->upload(s->slgg, p->slgg, sizeof(double) * (size_t)p->ng * p->nmat * p->ng * p->nmom));
-  SNAP_TRY(s->upload(s->qi, p->qi, sizeof(double) * gc));
+  // Snap::initialize_velocity (snap.cc:716-738)
+  s->vdelt_h.assign(s->ng, 0.0);
+  if (p->timedep) {
+    const double dt = p->tf / (double)s->prob.nsteps;
+    for (int g = 0; g < s->ng; g++) s->vdelt_h[g] = 2.0 / (dt * (double)(s->ng - g));
+  }
+  SNAP_TRY(s->upload(s->vdelt_d, s->vdelt_h.data(), sizeof(double) * s->ng));
+  if (p->mms) {
+    SNAP_TRY(mms_prepare(s->prob, &s->mms));
+    SNAP_TRY(s->upload(s->qi, s->mms.qi.data(), sizeof(double) * gc));
+    SNAP_TRY(s->qim.alloc(sizeof(double) * 8 * gc * s->nang));
+  } else {
+    SNAP_TRY(s->upload(s->qi, p->qi, sizeof(double) * gc));
+  }
+  if (p->timedep) {
+    SNAP_TRY(s->tflux[0].alloc(sizeof(double) * 8 * gc * s->nang));
+    SNAP_TRY(s->tflux[1].alloc(sizeof(double) * 8 * gc * s->nang));
+  }
   SNAP_TRY(s->txs.alloc(sizeof(double) * gc));
   SNAP_TRY(s->sxs.alloc(sizeof(double) * gc * s->nmom));
   SNAP_TRY(s->dinv.alloc(sizeof(double) * gc * s->nang));
@@ -260,12 +309,13 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
     SNAP_TRY(launch_expand_scat(sl, 0, s->g_acc(s->mat, 1, 4), s->g_acc(s->sxs, s->nmom, 8), s->e,
                                 s->ng, s->nmom, st));
   }
-  SNAP_TRY(launch_geometry(s->g_acc(s->txs, 1, 8), s->g_acc(s->dinv, s->nang, 8), nullptr, s->e,
+  const double *vd = p->timedep ? s->vdelt_h.data() : nullptr;
+  SNAP_TRY(launch_geometry(s->g_acc(s->txs, 1, 8), s->g_acc(s->dinv, s->nang, 8), vd, s->e,
                            s->ng, s->nang, s->hi, s->hj, s->hk, s->quad, false, st));
   if (s->opt.precision == SNAP_PRECISION_MIXED) {
     SNAP_TRY(s->dinv32.alloc(sizeof(float) * gc * s->nang));
     SNAP_TRY(s->txs32.alloc(sizeof(float) * gc));
-    SNAP_TRY(launch_geometry(s->g_acc(s->txs, 1, 8), s->g_acc(s->dinv32, s->nang, 4), nullptr, s->e,
+    SNAP_TRY(launch_geometry(s->g_acc(s->txs, 1, 8), s->g_acc(s->dinv32, s->nang, 4), vd, s->e,
                              s->ng, s->nang, s->hi, s->hj, s->hk, s->quad, true, st));
     // t_xs in float for the FP32 fixup
     std::vector<double> h(gc);
@@ -296,54 +346,74 @@ extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) 
   const bool mixed = s->opt.precision == SNAP_PRECISION_MIXED;
   SNAP_HIP_TRY(hipMemsetAsync(s->flux0.p, 0, sizeof(double) * gc, stream));
   if (nm1) SNAP_HIP_TRY(hipMemsetAsync(s->fluxm.p, 0, sizeof(double) * gc * nm1, stream));
+  // the time-edge fluxes start at zero (snap.cc:332-335) and the angular
+  // source unscaled (MMSInitSource)
+  const size_t ta = 8 * gc * (size_t)s->nang;
+  if (s->prob.timedep)
+    for (auto &b : s->tflux) SNAP_HIP_TRY(hipMemsetAsync(b.p, 0, sizeof(double) * ta, stream));
+  if (s->prob.mms)
+    SNAP_TRY(mms_source(s->mms, s->quad, s->e, s->ng, s->nmat, s->nmom, s->mat.as<int32_t>(),
+                        s->sigt.as<double>(), s->slgg.as<double>(), s->qim.as<double>(), stream));
   SNAP_HIP_TRY(hipEventRecord(s->ev[2], stream));
   float sweep_ms = 0.f;
-  bool inner_converged = false;
-  for (int otno = 0; otno < s->prob.oitm; otno++) {
-    SNAP_TRY(s->outer_source());
-    SNAP_TRY(s->copy(s->flux0po, s->flux0, sizeof(double) * gc));
-    st.outer_total++;
-    inner_converged = false;
-    for (int inno = 0; inno < s->prob.iitm; inno++) {
-      SNAP_TRY(s->inner_source());
-      SNAP_TRY(s->copy(s->flux0pi, s->flux0, sizeof(double) * gc));
-      const bool refine = mixed && s->opt.refine;
-      const bool fp32 = mixed && !(s->opt.first_sweep_fp64 && st.inner_total == 0) &&
-                        !(refine && inno == 0);
-      if (refine && fp32)  // FP64 defect of the source against the outer's base
-        SNAP_TRY(launch_subtract(s->qtot.as<double>(), s->qbase.as<double>(),
-                                 (int64_t)gc * s->cmom, stream));
-      SNAP_HIP_TRY(hipEventRecord(s->ev[0], stream));
-      SNAP_TRY(s->sweep(fp32 ? SNAP_PRECISION_MIXED
-                             : (mixed ? SNAP_PRECISION_FP64 : s->opt.precision),
-                        nullptr, !(refine && fp32)));
-      SNAP_HIP_TRY(hipEventRecord(s->ev[1], stream));
-      SNAP_TRY(s->reduce_octants(refine && fp32));
-      if (refine && !fp32) {  // keep the FP64 base of this outer
-        SNAP_TRY(s->copy(s->qbase, s->qtot, sizeof(double) * gc * s->cmom));
-        SNAP_TRY(s->copy(s->flux0base, s->flux0, sizeof(double) * gc));
-        if (nm1) SNAP_TRY(s->copy(s->fluxmbase, s->fluxm, sizeof(double) * gc * nm1));
+  bool all_converged = true;
+  const double dt = s->prob.tf / (double)s->prob.nsteps;
+  for (int cy = 0; cy < s->prob.nsteps; cy++) {
+    s->step = cy;
+    // MMSScale (snap.cc:452-467): the source at the step's mid-time
+    if (s->prob.mms)
+      SNAP_TRY(launch_scale(s->qim.as<double>(), (int64_t)ta,
+                            cy == 0 ? dt * 0.5 : double(2 * cy + 1) / double(2 * cy - 1), stream));
+    bool inner_converged = false, converged = false;
+    for (int otno = 0; otno < s->prob.oitm; otno++) {
+      SNAP_TRY(s->outer_source());
+      SNAP_TRY(s->copy(s->flux0po, s->flux0, sizeof(double) * gc));
+      st.outer_total++;
+      inner_converged = false;
+      for (int inno = 0; inno < s->prob.iitm; inno++) {
+        SNAP_TRY(s->inner_source());
+        SNAP_TRY(s->copy(s->flux0pi, s->flux0, sizeof(double) * gc));
+        const bool refine = mixed && s->opt.refine;
+        const bool fp32 = mixed && !(s->opt.first_sweep_fp64 && st.inner_total == 0) &&
+                          !(refine && inno == 0);
+        if (refine && fp32)  // FP64 defect of the source against the outer's base
+          SNAP_TRY(launch_subtract(s->qtot.as<double>(), s->qbase.as<double>(),
+                                   (int64_t)gc * s->cmom, stream));
+        SNAP_HIP_TRY(hipEventRecord(s->ev[0], stream));
+        SNAP_TRY(s->sweep(fp32 ? SNAP_PRECISION_MIXED
+                               : (mixed ? SNAP_PRECISION_FP64 : s->opt.precision),
+                          nullptr, !(refine && fp32)));
+        SNAP_HIP_TRY(hipEventRecord(s->ev[1], stream));
+        SNAP_TRY(s->reduce_octants(refine && fp32));
+        if (refine && !fp32) {  // keep the FP64 base of this outer
+          SNAP_TRY(s->copy(s->qbase, s->qtot, sizeof(double) * gc * s->cmom));
+          SNAP_TRY(s->copy(s->flux0base, s->flux0, sizeof(double) * gc));
+          if (nm1) SNAP_TRY(s->copy(s->fluxmbase, s->fluxm, sizeof(double) * gc * nm1));
+        }
+        if (fp32)
+          st.sweeps_fp32++;
+        else
+          st.sweeps_fp64++;
+        st.inner_total++;
+        SNAP_TRY(s->test(s->flux0pi, s->prob.epsi, &inner_converged));
+        float ms = 0.f;
+        SNAP_HIP_TRY(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+        sweep_ms += ms;
+        if (inner_converged) break;
       }
-      if (fp32)
-        st.sweeps_fp32++;
-      else
-        st.sweeps_fp64++;
-      st.inner_total++;
-      SNAP_TRY(s->test(s->flux0pi, s->prob.epsi, &inner_converged));
-      float ms = 0.f;
-      SNAP_HIP_TRY(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
-      sweep_ms += ms;
-      if (inner_converged) break;
+      if (otno == 0) continue;
+      if (!inner_converged) continue;
+      bool outer_converged = false;
+      SNAP_TRY(s->test(s->flux0po, 100.0 * s->prob.epsi, &outer_converged));
+      if (outer_converged) {
+        converged = true;
+        break;
+      }
     }
-    if (otno == 0) continue;
-    if (!inner_converged) continue;
-    bool outer_converged = false;
-    SNAP_TRY(s->test(s->flux0po, 100.0 * s->prob.epsi, &outer_converged));
-    if (outer_converged) {
-      st.converged = 1;
-      break;
-    }
+    all_converged = all_converged && converged;
+    st.time_steps++;
   }
+  st.converged = all_converged ? 1 : 0;
   SNAP_HIP_TRY(hipEventRecord(s->ev[3], stream));
   SNAP_HIP_TRY(hipEventSynchronize(s->ev[3]));
   float solve_ms = 0.f;
@@ -353,6 +423,24 @@ extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) 
   st.units_per_sweep = (double)s->cells * 8.0 * s->nang * s->ng;
   st.grind_ns = st.inner_total ? (sweep_ms * 1e6) / (st.units_per_sweep * st.inner_total) : 0.0;
   if (stats) *stats = st;
+  return SNAP_OK;
+}
+
+extern "C" int snap_hip_solver_mms_compare(snap_solver_t *s, double result[3]) {
+  if (!s || !result) return fail(SNAP_ERR_INVALID, "NULL argument");
+  if (!s->prob.mms) return fail(SNAP_ERR_INVALID, "the problem has no manufactured solution");
+  const size_t gc = (size_t)s->ng * s->cells;
+  std::vector<double> f(gc);
+  SNAP_HIP_TRY(hipStreamSynchronize(s->stream));
+  SNAP_HIP_TRY(hipMemcpy(f.data(), s->flux0.p, sizeof(double) * gc, hipMemcpyDeviceToHost));
+  mms_compare(f.data(), s->mms.ref.data(), gc, result);
+  return SNAP_OK;
+}
+
+extern "C" int snap_hip_solver_get_mms_reference(snap_solver_t *s, double *ref_flux) {
+  if (!s || !ref_flux) return fail(SNAP_ERR_INVALID, "NULL argument");
+  if (!s->prob.mms) return fail(SNAP_ERR_INVALID, "the problem has no manufactured solution");
+  memcpy(ref_flux, s->mms.ref.data(), sizeof(double) * s->mms.ref.size());
   return SNAP_OK;
 }
 
@@ -401,6 +489,7 @@ extern "C" int snap_hip_solver_gradient(snap_solver_t *s, const double *resp, in
                                         int *iterations) {
   if (!s || !resp || !grad) return fail(SNAP_ERR_INVALID, "NULL argument");
   if (s->nmom != 1) return fail(SNAP_ERR_UNSUPPORTED, "the adjoint handles nmom = 1 (isotropic scattering)");
+  if (s->td) return fail(SNAP_ERR_UNSUPPORTED, "the adjoint is of the steady-state solve");
   if (precision != SNAP_PRECISION_FP64 && precision != SNAP_PRECISION_MIXED)
     return fail(SNAP_ERR_INVALID, "adjoint precision must be FP64 or MIXED (FP32 sweeps)");
   if (!(tol > 0.0) || maxit < 1) return fail(SNAP_ERR_INVALID, "bad tolerance or iteration cap");

@@ -53,6 +53,11 @@ __host__ __device__ constexpr int max_steps(int apl) { return apl == 1 ? 4 : 2; 
 // Workgroup size bound: two angles per lane need more than the 128 registers
 // a 16-wave workgroup leaves each lane, so those workgroups stop at 8 waves.
 __host__ __device__ constexpr int max_threads(int apl) { return apl == 1 ? 1024 : 512; }
+// The time-dependent kernels carry the time-edge inflow and MMS source of
+// every prefetched cell as well: they stop at 8 waves (256 registers a lane).
+__host__ __device__ constexpr int max_threads(int apl, bool td) {
+  return td ? 512 : max_threads(apl);
+}
 
 // Row strides of the two ec tables: the per-angle table is read by lane
 // (angle) with the moment as the row, the per-moment table by (cell, moment)
@@ -144,7 +149,7 @@ static int tile_maxdiag(int TX, int TY, int TZ) {
 }
 
 void plan_choose(Extent3 e, int nang, int cmom, int real_bytes, int *TX, int *TY, int *TZ,
-                 int *waves, int *apl) {
+                 int *waves, int *apl, bool td) {
   // angles per lane: one at FP64 (two would exceed 128 registers per lane
   // and halve the resident waves), the better-filled wavefront at FP32, two
   // whenever one cell's angles exceed a wavefront
@@ -152,7 +157,7 @@ void plan_choose(Extent3 e, int nang, int cmom, int real_bytes, int *TX, int *TY
   if (*apl < 1 || *apl > 2) return;  // plan_create reports it
   const int lpc = (nang + *apl - 1) / *apl;
   const int cpw = std::max(1, pick_cpw(lpc));
-  const int ms = max_steps(*apl), wmax = max_threads(*apl) / 64;
+  const int ms = max_steps(*apl), wmax = max_threads(*apl, td) / 64;
   if (*TX <= 0 || *TY <= 0 || *TZ <= 0) {
     // largest candidate tile whose biggest cell diagonal the workgroup
     // covers and whose LDS fits; measured best first (8x8x4 at FP64, 8x8x8
@@ -188,7 +193,7 @@ void plan_free(SweepPlan *p) {
 }
 
 int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int TZ, int waves,
-                int apl) {
+                int apl, bool td) {
   plan_free(p);
   if (e.nx < 1 || e.ny < 1 || e.nz < 1) return fail(SNAP_ERR_INVALID, "empty chunk");
   if (nang < 1 || nang > 128)
@@ -217,6 +222,7 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
   // with two angles per lane, where ~150 registers leave 12 waves per CU
   if (waves == 0) waves = (apl == 1) ? 8 : 6;
   p->waves = waves;
+  p->td = td;
   p->apl = apl;
   p->lpc = (nang + apl - 1) / apl;
   p->cpw = pick_cpw(p->lpc);
@@ -259,12 +265,13 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
   const int ms = max_steps(apl);
   const int need_waves = (p->maxdiag + ms * p->cpw - 1) / (ms * p->cpw);
   if (p->waves < need_waves) p->waves = need_waves;
-  if (p->waves * 64 > max_threads(apl)) {
-    if (waves * 64 <= max_threads(apl) && need_waves * 64 <= max_threads(apl))
-      p->waves = max_threads(apl) / 64;
+  const int mt = max_threads(apl, td);
+  if (p->waves * 64 > mt) {
+    if (waves * 64 <= mt && need_waves * 64 <= mt)
+      p->waves = mt / 64;
     else
       return fail(SNAP_ERR_INVALID, "tile %dx%dx%d needs %d waves per workgroup (> %d)", TX, TY,
-                  TZ, std::max(waves, need_waves), max_threads(apl) / 64);
+                  TZ, std::max(waves, need_waves), mt / 64);
   }
   if (p->lds_bytes(8, false) > 160 * 1024)
     return fail(SNAP_ERR_INVALID, "tile %dx%dx%d with %d angles needs %zu B of LDS (> 160 KiB)",
@@ -309,6 +316,13 @@ struct SweepArgs {
   const BatchDesc *descs;
   const int *active, *done;
   int64_t total_cells;
+  // time-dependent kernels (TD): vdelt per job group, or vdelt0 for the
+  // general variant; time-edge fluxes and MMS source [8][ng_job][cells][nang]
+  const double *vdelt;
+  double vdelt0;
+  const double *tin;
+  double *tout;
+  const double *qim;
 };
 
 // Arguments of the general variant (the drop-in entry point): Realm affine
@@ -316,6 +330,7 @@ struct SweepArgs {
 struct SweepGeneral {
   Acc3 qtot, dinv, txs, flux, fluxm;
   Acc2 gx, gy, gz;
+  Acc3 tin, tout, qim;  // TD: nang doubles per cell (base NULL = none)
 };
 
 __device__ inline void wave_sync() {
@@ -367,9 +382,14 @@ __device__ inline unsigned seg_count(const unsigned long long (&b)[NB], unsigned
 // ghost faces and atomic accumulation (the drop-in entry point).  BATCH:
 // each workgroup's (problem, tile) comes from the launch's table, arrays are
 // the batch's concatenation ([g][all cells]), and a workgroup whose problem
-// is finished or has converged its inner iteration exits at once.
-template <typename Real, int CPW, int APL, bool EXACT, bool REC, bool GEN, bool BATCH = false>
-__global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, SweepGeneral G, int plane_off) {
+// is finished or has converged its inner iteration exits at once.  TD: the
+// time-dependent terms (sweep.cc:414-534; gpu_sweep.cu:738-1113) -- the
+// time-edge inflow vdelt*psi_t,in, a fourth outflow edge in the fixup and
+// psi_t,out -- and the MMS angular source (sweep.cc:347-351), each switched
+// on at run time by vdelt != 0 and a non-NULL source.
+template <typename Real, int CPW, int APL, bool EXACT, bool REC, bool GEN, bool BATCH = false,
+          bool TD = false>
+__global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs A, SweepGeneral G, int plane_off) {
   constexpr bool kFP64 = std::is_same<Real, double>::value;
   constexpr int MS = max_steps(APL);  // cell groups per wave and diagonal
   constexpr int QS = APL;             // source moments held per lane
@@ -437,6 +457,26 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
   double *pfg = A.part_flux + ((size_t)corner * A.ng_job + g) * gstride + coff;
   double *pmg =
       A.part_fluxm + (((size_t)corner * A.ng_job + g) * gstride + coff) * (size_t)(cmom - 1);
+  // time-dependent terms of this sweep (uniform over the workgroup)
+  double vd = 0.0;
+  bool has_time = false, has_qim = false;
+  const double *tig = nullptr, *qmg = nullptr;
+  double *tog = nullptr;
+  if constexpr (TD) {
+    vd = GEN ? A.vdelt0 : A.vdelt[g];
+    has_time = vd != 0.0;
+    if constexpr (GEN) {
+      has_qim = G.qim.base != nullptr;
+    } else {
+      const size_t tb = (((size_t)corner * A.ng_job + g) * gstride + coff) * (size_t)nang;
+      has_qim = A.qim != nullptr;
+      if (has_time) {
+        tig = A.tin + tb;
+        tog = A.tout + tb;
+      }
+      if (has_qim) qmg = A.qim + tb;
+    }
+  }
 
   // ---- per-lane constants: lane = cell slot * LPC + la, angles la + r*LPC
   const int cl = (CPW == 1) ? 0 : lane / LPC;
@@ -563,10 +603,27 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
   __syncthreads();
 
   // ---- register prefetch of this wave's cells on the next diagonal: dinv of
-  // the lane's angles, its source moments and t_xs.
-  double n_dv[MS][APL], n_q[MS][QS], n_tx[MS];
+  // the lane's angles, its source moments and t_xs (TD: and the lane's
+  // time-edge inflows and MS sources).
+  double n_dv[MS][APL], n_q[MS][QS], n_tx[MS], n_ti[MS][APL], n_qm[MS][APL];
   int n_cell[MS];
   auto load_cell = [&](int cell, int st) {
+    if constexpr (TD) {
+      const double *tp = nullptr, *qp = nullptr;
+      if constexpr (GEN) {
+        const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
+        if (has_time) tp = (const double *)G.tin.at(px, py, pz, 0);
+        if (has_qim) qp = (const double *)G.qim.at(px, py, pz, 0);
+      } else {
+        if (has_time) tp = tig + (size_t)cell * nang;
+        if (has_qim) qp = qmg + (size_t)cell * nang;
+      }
+#pragma unroll
+      for (int r = 0; r < APL; r++) {
+        if (has_time) n_ti[st][r] = tp[an[r]];
+        if (has_qim) n_qm[st][r] = qp[an[r]];
+      }
+    }
     if constexpr (GEN) {
       const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
       const Real *dv = (const Real *)G.dinv.at(px, py, pz, 0);
@@ -599,6 +656,10 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
       for (int r = 0; r < APL; r++) n_dv[st][r] = 0.0;
 #pragma unroll
       for (int m = 0; m < QS; m++) n_q[st][m] = 0.0;
+      if constexpr (TD) {
+#pragma unroll
+        for (int r = 0; r < APL; r++) n_ti[st][r] = n_qm[st][r] = 0.0;
+      }
       n_tx[st] = 0.0;
       n_cell[st] = -1;
       if (lane_ok && my < c1) {
@@ -656,13 +717,20 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
   for (int d = 0; d < ncp; ++d) {
     const int c0 = toff[d], c1 = toff[d + 1];
     const int nsteps = (c1 - c0 + per_step - 1) / per_step;
-    double c_dv[MS][APL], c_tx[MS];
+    double c_dv[MS][APL], c_tx[MS], c_ti[MS][APL], c_qm[MS][APL];
     int c_cell[MS];
     // source moments of this diagonal's cells to the wave's LDS slots
 #pragma unroll
     for (int st = 0; st < MS; st++) {
 #pragma unroll
       for (int r = 0; r < APL; r++) c_dv[st][r] = n_dv[st][r];
+      if constexpr (TD) {
+#pragma unroll
+        for (int r = 0; r < APL; r++) {
+          c_ti[st][r] = n_ti[st][r];
+          c_qm[st][r] = n_qm[st][r];
+        }
+      }
       c_tx[st] = n_tx[st];
       c_cell[st] = n_cell[st];
 #pragma unroll
@@ -717,6 +785,12 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
 #pragma unroll
             for (int r = 0; r < APL; r++) ps[r] = madd<EXACT>(ecs[l * ECR + an[r]], ql, ps[r]);
           }
+          if constexpr (TD) {
+            if (has_qim) {
+#pragma unroll
+              for (int r = 0; r < APL; r++) ps[r] += c_qm[st][r];
+            }
+          }
 #pragma unroll
           for (int r = 0; r < APL; r++) psi[r] = ps[r];
         } else {
@@ -728,6 +802,12 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
 #pragma unroll
             for (int r = 0; r < APL; r++)
               pf[r] = maddf((float)ecs[l * ECR + an[r]], ql, pf[r]);
+          }
+          if constexpr (TD) {
+            if (has_qim) {
+#pragma unroll
+              for (int r = 0; r < APL; r++) pf[r] += (float)c_qm[st][r];
+            }
           }
 #pragma unroll
           for (int r = 0; r < APL; r++) psi[r] = pf[r];
@@ -751,12 +831,14 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
         Real *sx = fx + rec.y;
         Real *sy = fy + rec.z;
         Real *sz = fz + rec.w;
-        Real psii[APL], psij[APL], psik[APL], pc[APL];
+        Real psii[APL], psij[APL], psik[APL], pc[APL], ti[APL];
+        const Real vdr = (Real)vd;
 #pragma unroll
         for (int r = 0; r < APL; r++) {
           psii[r] = sx[an[r]];
           psij[r] = sy[an[r]];
           psik[r] = sz[an[r]];
+          ti[r] = TD ? (Real)c_ti[st][r] : Real(0);
           // initial diamond-difference solution (sweep.cc:352-400)
           Real p = psi[r];
           if constexpr (kFP64 && EXACT) {
@@ -772,22 +854,35 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
             p = maddf(psij[r], cy[r], p);
             p = maddf(psik[r], cz[r], p);
           }
+          // time-edge inflow (sweep.cc:414-419)
+          if constexpr (TD) {
+            if (has_time) {
+              if constexpr (kFP64 && EXACT)
+                p += vdr * ti[r];
+              else if constexpr (kFP64)
+                p = __builtin_fma(vdr, ti[r], p);
+              else
+                p = maddf(vdr, ti[r], p);
+            }
+          }
           pc[r] = p * dv[r];
         }
         // outflow factors: 1 while the face keeps its diamond-difference
         // value, 0 once the fixup has set it to zero
-        bool hx[APL], hy[APL], hz[APL];
+        bool hx[APL], hy[APL], hz[APL], ht[APL];
 #pragma unroll
-        for (int r = 0; r < APL; r++) hx[r] = hy[r] = hz[r] = true;
+        for (int r = 0; r < APL; r++) hx[r] = hy[r] = hz[r] = ht[r] = true;
         if (A.fixup) {
           // negative-flux fixup (sweep.cc:402-505; gpu_sweep.cu:1556-1630).
           // The count of negative outflows is cell-global over angles.  A
           // pass that clears no new flag leaves the next pc equal to this
           // one, so the pass after it repeats the count and ends the loop:
-          // with 3*nang flags, 3*nang+2 passes always reach the break, and
-          // the outflows are 2*pc - psi_in of the final pc.
+          // with 3*nang flags (4*nang with the time edge), 3*nang+2 passes
+          // always reach the break, and the outflows are 2*pc - psi_in of
+          // the final pc.
+          constexpr int NE = TD ? 4 : 3;
           auto count_negative = [&]() -> unsigned {
-            unsigned long long bal[3 * APL];
+            unsigned long long bal[NE * APL];
 #pragma unroll
             for (int r = 0; r < APL; r++) {
               const Real p2 = Real(2) * pc[r];
@@ -797,31 +892,54 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
               hx[r] = hx[r] && !nxn;
               hy[r] = hy[r] && !nyn;
               hz[r] = hz[r] && !nzn;
-              bal[3 * r] = __builtin_amdgcn_ballot_w64(nxn);
-              bal[3 * r + 1] = __builtin_amdgcn_ballot_w64(nyn);
-              bal[3 * r + 2] = __builtin_amdgcn_ballot_w64(nzn);
+              bal[NE * r] = __builtin_amdgcn_ballot_w64(nxn);
+              bal[NE * r + 1] = __builtin_amdgcn_ballot_w64(nyn);
+              bal[NE * r + 2] = __builtin_amdgcn_ballot_w64(nzn);
+              if constexpr (TD) {
+                const bool ntn = has_time && aok[r] && p2 - ti[r] < Real(0);
+                ht[r] = ht[r] && !ntn;
+                bal[NE * r + 3] = __builtin_amdgcn_ballot_w64(ntn);
+              }
             }
             return seg_count<CPW>(bal, seg0, LPC, cl);
           };
           unsigned old_neg = count_negative();
           if (old_neg != 0) {
-            for (int it = 1; it < 3 * nang + 2; ++it) {
+            for (int it = 1; it < NE * nang + 2; ++it) {
 #pragma unroll
               for (int r = 0; r < APL; r++) {
                 const Real vx = hx[r] ? Real(1) : Real(0), vy = hy[r] ? Real(1) : Real(0),
-                           vz = hz[r] ? Real(1) : Real(0);
+                           vz = hz[r] ? Real(1) : Real(0), vt = ht[r] ? Real(1) : Real(0);
                 Real den, p;
                 if constexpr (kFP64 && EXACT) {
-                  p = psi[r] + 0.5 * (psii[r] * cx[r] * hi * (1.0 + vx) +
-                                      psij[r] * cy[r] * hj * (1.0 + vy) +
-                                      psik[r] * cz[r] * hk * (1.0 + vz));
-                  den = (p <= 0.0) ? 0.0
-                                   : (txs + cx[r] * hi * vx + cy[r] * hj * vy + cz[r] * hk * vz);
+                  if (TD && has_time) {
+                    // sweep.cc:481-494
+                    p = psi[r] + 0.5 * (psii[r] * cx[r] * hi * (1.0 + vx) +
+                                        psij[r] * cy[r] * hj * (1.0 + vy) +
+                                        psik[r] * cz[r] * hk * (1.0 + vz) +
+                                        ti[r] * vdr * (1.0 + vt));
+                    den = (p <= 0.0) ? 0.0
+                                     : (txs + cx[r] * hi * vx + cy[r] * hj * vy + cz[r] * hk * vz +
+                                        vdr * vt);
+                  } else {
+                    p = psi[r] + 0.5 * (psii[r] * cx[r] * hi * (1.0 + vx) +
+                                        psij[r] * cy[r] * hj * (1.0 + vy) +
+                                        psik[r] * cz[r] * hk * (1.0 + vz));
+                    den = (p <= 0.0) ? 0.0
+                                     : (txs + cx[r] * hi * vx + cy[r] * hj * vy + cz[r] * hk * vz);
+                  }
                 } else {
-                  const Real t = psii[r] * cx[r] * (Real(1) + vx) +
-                                 psij[r] * cy[r] * (Real(1) + vy) + psik[r] * cz[r] * (Real(1) + vz);
+                  Real t = psii[r] * cx[r] * (Real(1) + vx) +
+                           psij[r] * cy[r] * (Real(1) + vy) + psik[r] * cz[r] * (Real(1) + vz);
+                  Real dd = txs + cx[r] * vx + cy[r] * vy + cz[r] * vz;
+                  if constexpr (TD) {
+                    if (has_time) {
+                      t += ti[r] * vdr * (Real(1) + vt);
+                      dd += vdr * vt;
+                    }
+                  }
                   p = psi[r] + Real(0.5) * t;
-                  den = (p <= Real(0)) ? Real(0) : (txs + cx[r] * vx + cy[r] * vy + cz[r] * vz);
+                  den = (p <= Real(0)) ? Real(0) : dd;
                 }
                 pc[r] = (den < tolr) ? Real(0) : p / den;
               }
@@ -843,6 +961,21 @@ __global__ void __launch_bounds__(max_threads(APL)) k_sweep_tiles(SweepArgs A, S
             sz[an[r]] = oz;
           }
           pcw[r] = w_d[r] * (double)pc[r];
+        }
+        // time-edge outflow (sweep.cc:512-519, 529-533)
+        if constexpr (TD) {
+          if (has_time) {
+            double *to;
+            if constexpr (GEN) {
+              const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
+              to = (double *)G.tout.at(px, py, pz, 0);
+            } else {
+              to = tog + (size_t)cell * nang;
+            }
+#pragma unroll
+            for (int r = 0; r < APL; r++)
+              if (aok[r]) to[an[r]] = (double)((Real(2) * pc[r] - ti[r]) * (ht[r] ? Real(1) : Real(0)));
+          }
         }
       }
       // ---- batch this step's cells for the moment sums
@@ -942,6 +1075,30 @@ static SweepKernel kernel_for_batch(int cpw, int apl) {
   return nullptr;
 }
 
+// Time-dependent kernels: dense with dinv streamed (FP64 fused or exact, or
+// FP32), and the general variant in the exact order for the drop-in.
+template <typename Real, bool EXACT>
+static SweepKernel kernel_for_td(int cpw, int apl, bool gen) {
+#define SNAP_TD_CASE(C, P) \
+  if (cpw == C && apl == P) return k_sweep_tiles<Real, C, P, EXACT, false, false, false, true>;
+#define SNAP_TDG_CASE(C, P) \
+  if (cpw == C && apl == P) return k_sweep_tiles<Real, C, P, EXACT, false, true, false, true>;
+  if (gen) {
+    if constexpr (EXACT && std::is_same<Real, double>::value) {
+      SNAP_TDG_CASE(1, 1) SNAP_TDG_CASE(2, 1) SNAP_TDG_CASE(3, 1) SNAP_TDG_CASE(4, 1)
+      SNAP_TDG_CASE(5, 1) SNAP_TDG_CASE(6, 1) SNAP_TDG_CASE(8, 1) SNAP_TDG_CASE(1, 2)
+    }
+    return nullptr;
+  }
+  SNAP_TD_CASE(1, 1) SNAP_TD_CASE(2, 1) SNAP_TD_CASE(3, 1) SNAP_TD_CASE(4, 1)
+  SNAP_TD_CASE(5, 1) SNAP_TD_CASE(6, 1) SNAP_TD_CASE(8, 1)
+  SNAP_TD_CASE(1, 2) SNAP_TD_CASE(2, 2) SNAP_TD_CASE(3, 2) SNAP_TD_CASE(4, 2)
+  SNAP_TD_CASE(5, 2) SNAP_TD_CASE(6, 2) SNAP_TD_CASE(8, 2)
+#undef SNAP_TD_CASE
+#undef SNAP_TDG_CASE
+  return nullptr;
+}
+
 static int ensure_lds_attr(SweepKernel k, size_t bytes) {
   // dynamic LDS above 64 KiB must be requested per kernel
   static std::mutex mu;
@@ -966,6 +1123,9 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
     return fail(SNAP_ERR_UNSUPPORTED, "chunks of 2^31 cells or more");
   if (L.batch && (L.general || L.recompute_dinv))
     return fail(SNAP_ERR_INVALID, "batch sweeps are dense with dinv streamed");
+  if (L.td && (L.batch || L.recompute_dinv))
+    return fail(SNAP_ERR_INVALID, "time-dependent sweeps read dinv and run one problem");
+  if (L.td && !p.td) return fail(SNAP_ERR_INVALID, "the plan is not one for time-dependent sweeps");
   SweepArgs A;
   memset(&A, 0, sizeof(A));
   SweepGeneral G;
@@ -1012,6 +1172,19 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
     A.done = L.batch->done;
     A.total_cells = L.batch->total_cells;
   }
+  if (L.td) {
+    A.vdelt = L.vdelt;
+    A.vdelt0 = L.vdelt0;
+    A.tin = L.tin;
+    A.tout = L.tout;
+    A.qim = L.qim;
+    G.tin = L.tin_acc;
+    G.tout = L.tout_acc;
+    G.qim = L.qim_acc;
+    // dense: the caller gives tin/tout whenever a vdelt it passes is nonzero
+    if (L.general ? (L.vdelt0 != 0.0 && (!G.tin.base || !G.tout.base)) : !L.vdelt)
+      return fail(SNAP_ERR_INVALID, "time-dependent sweep without vdelt or time-edge fluxes");
+  }
   if (L.general) {
     G.qtot = L.qtot;
     G.dinv = L.dinv;
@@ -1029,7 +1202,10 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
   }
   const size_t lds = p.lds_bytes(L.fp32 ? 4 : 8, L.recompute_dinv);
   const SweepKernel k =
-      L.batch ? (L.fp32 ? kernel_for_batch<float, false>(p.cpw, p.apl)
+      L.td ? (L.fp32 ? kernel_for_td<float, false>(p.cpw, p.apl, L.general)
+                     : (L.exact ? kernel_for_td<double, true>(p.cpw, p.apl, L.general)
+                                : kernel_for_td<double, false>(p.cpw, p.apl, L.general)))
+      : L.batch ? (L.fp32 ? kernel_for_batch<float, false>(p.cpw, p.apl)
                         : (L.exact ? kernel_for_batch<double, true>(p.cpw, p.apl)
                                    : kernel_for_batch<double, false>(p.cpw, p.apl)))
       : L.fp32 ? kernel_for<float, false>(p.cpw, p.apl, L.recompute_dinv, L.general)
@@ -1128,8 +1304,9 @@ extern "C" int snap_hip_initialize_gpu_context(const double *ec, const double *m
 extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream_t stream) {
   if (!args) return fail(SNAP_ERR_INVALID, "NULL sweep arguments");
   const snap_sweep_args_t &a = *args;
-  if (a.mms_source) return fail(SNAP_ERR_UNSUPPORTED, "MMS sources are not built (steady fixed-source path)");
-  if (a.vdelt != 0.0) return fail(SNAP_ERR_UNSUPPORTED, "time-dependent sweeps (vdelt != 0) are not built");
+  if (a.mms_source && !a.qim.base) return fail(SNAP_ERR_INVALID, "mms_source set without qim");
+  if (a.vdelt != 0.0 && (!a.time_flux_in.base || !a.time_flux_out.base))
+    return fail(SNAP_ERR_INVALID, "vdelt != 0 needs time_flux_in and time_flux_out");
   if (a.stride_x_positive != ((a.corner & 1) != 0) || a.stride_y_positive != ((a.corner & 2) != 0) ||
       a.stride_z_positive != ((a.corner & 4) != 0) || a.corner < 0 || a.corner > 7)
     return fail(SNAP_ERR_INVALID, "corner %d does not match the stride directions", a.corner);
@@ -1142,15 +1319,17 @@ extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream
     return fail(SNAP_ERR_INVALID, "angles/moments differ from the context");
   const Extent3 e{a.x_range, a.y_range, a.z_range};
   if (e.nx < 1 || e.ny < 1 || e.nz < 1) return fail(SNAP_ERR_INVALID, "empty chunk");
-  // plan per chunk shape
+  // plan per chunk shape, one for steady and one for time-dependent sweeps
+  const bool td = a.vdelt != 0.0 || a.mms_source;
   SweepPlan *plan = nullptr;
   for (auto &pp : c.plans)
-    if (pp.first.nx == e.nx && pp.first.ny == e.ny && pp.first.nz == e.nz) plan = pp.second;
+    if (pp.first.nx == e.nx && pp.first.ny == e.ny && pp.first.nz == e.nz && pp.second->td == td)
+      plan = pp.second;
   if (!plan) {
     int TX = 0, TY = 0, TZ = 0, waves = 0, apl = 0;
-    plan_choose(e, c.quad.nang, c.quad.cmom, 8, &TX, &TY, &TZ, &waves, &apl);
+    plan_choose(e, c.quad.nang, c.quad.cmom, 8, &TX, &TY, &TZ, &waves, &apl, td);
     plan = new SweepPlan();
-    int rc = plan_create(plan, e, c.quad.nang, c.quad.cmom, TX, TY, TZ, waves, apl);
+    int rc = plan_create(plan, e, c.quad.nang, c.quad.cmom, TX, TY, TZ, waves, apl, td);
     if (rc != SNAP_OK) {
       delete plan;
       return rc;
@@ -1216,5 +1395,16 @@ extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream
   L.faces = wk->faces;
   L.fp32 = false;
   L.exact = true;
+  // time-dependent terms and the MMS source (gpu_sweep.cu:2102-2140 picks the
+  // time-dependent kernel when vdelt != 0)
+  if (td) {
+    L.td = true;
+    L.vdelt0 = a.vdelt;
+    if (a.vdelt != 0.0) {
+      SNAP_TRY(single_acc3(a.time_flux_in, b, &L.tin_acc, "time_flux_in"));
+      SNAP_TRY(single_acc3(a.time_flux_out, b, &L.tout_acc, "time_flux_out"));
+    }
+    if (a.mms_source) SNAP_TRY(single_acc3(a.qim, b, &L.qim_acc, "qim"));
+  }
   return sweep_enqueue(L, st);
 }

@@ -71,7 +71,8 @@ class Problem(ctypes.Structure):
         ("nmat", ctypes.c_int),
         ("mat", _ip), ("sigt", _dp), ("slgg", _dp), ("qi", _dp), ("sigt_cell", _dp),
         ("epsi", ctypes.c_double), ("iitm", ctypes.c_int), ("oitm", ctypes.c_int),
-        ("fixup", ctypes.c_int),
+        ("fixup", ctypes.c_int), ("timedep", ctypes.c_int), ("tf", ctypes.c_double),
+        ("nsteps", ctypes.c_int), ("mms", ctypes.c_int),
     ]
 
 
@@ -109,6 +110,7 @@ class SolveStats(ctypes.Structure):
         ("sweeps_fp32", ctypes.c_int),
         ("sweep_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
         ("units_per_sweep", ctypes.c_double), ("grind_ns", ctypes.c_double),
+        ("time_steps", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -157,6 +159,8 @@ SIGNATURES = {
     "snap_hip_solver_time_sweeps": (
         [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "snap_hip_solver_plan": ([_vp, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+    "snap_hip_solver_mms_compare": ([_vp, _dp], ctypes.c_int),
+    "snap_hip_solver_get_mms_reference": ([_vp, _dp], ctypes.c_int),
     "snap_hip_batch_create": (
         [ctypes.POINTER(Problem), ctypes.c_int, ctypes.POINTER(BatchOptions), ctypes.POINTER(_vp)],
         ctypes.c_int),

@@ -40,6 +40,7 @@ def problem_struct(p: Problem) -> "_lib.Problem":
     s.qi = _ptr(p.qi)
     s.sigt_cell = _ptr(p.sigt_cell)
     s.epsi, s.iitm, s.oitm, s.fixup = p.epsi, p.iitm, p.oitm, int(bool(p.fixup))
+    s.timedep, s.tf, s.nsteps, s.mms = int(bool(p.timedep)), p.tf, p.nsteps, int(bool(p.mms))
     return s
 
 
@@ -119,6 +120,22 @@ class Solver:
                                                      ctypes.byref(R), ctypes.byref(it)),
                    "snap_hip_solver_gradient")
         return grad, R.value, it.value
+
+    def mms_compare(self):
+        """MMSCompare of the current flux (mms.cc:497-541): (max, min, mean)
+        of |phi/ref - 1| over all cells and groups (mms problems only)."""
+        out = np.zeros(3)
+        _lib.check(self.lib.snap_hip_solver_mms_compare(self._h, _ptr(out)),
+                   "snap_hip_solver_mms_compare")
+        return float(out[0]), float(out[1]), float(out[2])
+
+    def mms_reference(self) -> np.ndarray:
+        """The manufactured scalar flux [ng, nz, ny, nx] (mms problems only)."""
+        p = self.problem
+        ref = np.zeros((p.ng, p.nz, p.ny, p.nx))
+        _lib.check(self.lib.snap_hip_solver_get_mms_reference(self._h, _ptr(ref)),
+                   "snap_hip_solver_get_mms_reference")
+        return ref
 
     def plan(self) -> dict:
         """The sweep plan the solver chose (tile edges, waves per tile,

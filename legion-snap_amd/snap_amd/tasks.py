@@ -85,9 +85,13 @@ def initialize_gpu_context(quad, nang: int, nmom: int, chunk) -> None:
 
 
 def mini_kba(corner: int, groups: Sequence[int], qtot, flux, fluxm, dinv, t_xs,
-             ghosts, hijk, nang: int, nmom: int, fixup: bool = True, origin=(0, 0, 0)) -> None:
+             ghosts, hijk, nang: int, nmom: int, fixup: bool = True, origin=(0, 0, 0),
+             vdelt=None, time_flux_in=None, time_flux_out=None, qim=None) -> None:
     """One corner over a group range: run_gpu_sweep once per group.
-    ghosts: (gx [ng, nz, ny, nang], gy [ng, nz, nx, nang], gz [ng, ny, nx, nang]) or None."""
+    ghosts: (gx [ng, nz, ny, nang], gy [ng, nz, nx, nang], gz [ng, ny, nx, nang]) or None.
+    Time dependence (MiniKBATask regions 5, 6, 11, sweep.cc:45-84): vdelt
+    [ng] host values, time_flux_in / time_flux_out [ng, nz, ny, nx, nang] of
+    this corner; qim [ng, nz, ny, nx, nang] is the corner's MMS source."""
     lib = _lib.load()
     nz, ny, nx = flux.shape[1:4]
     for g in groups:
@@ -115,7 +119,13 @@ def mini_kba(corner: int, groups: Sequence[int], qtot, flux, fluxm, dinv, t_xs,
         a.stride_z_positive = int(bool(corner & 4))
         a.num_moments = nmom
         a.hi, a.hj, a.hk = hijk
-        a.vdelt = 0.0
+        a.vdelt = 0.0 if vdelt is None else float(vdelt[g])
+        if a.vdelt != 0.0:
+            a.time_flux_in = acc3(time_flux_in, g)
+            a.time_flux_out = acc3(time_flux_out, g)
+        if qim is not None:
+            a.qim = acc3(qim, g)
+            a.mms_source = 1
         a.num_angles = nang
         a.fixup = int(bool(fixup))
         _lib.check(lib.snap_hip_run_gpu_sweep(ctypes.byref(a), _stream()), "snap_hip_run_gpu_sweep")

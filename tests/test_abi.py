@@ -47,14 +47,14 @@ def test_solver_rejects_null_problem():
     assert b"NULL" in lib.snap_hip_last_error()
 
 
-def test_sweep_rejects_unsupported_modes():
+def test_sweep_rejects_incomplete_arguments():
     lib = _lib.load()
     a = _lib.SweepArgs()
-    a.mms_source = 1
-    assert lib.snap_hip_run_gpu_sweep(ctypes.byref(a), None) == _lib.SNAP_ERR_UNSUPPORTED
+    a.mms_source = 1  # without a qim accessor
+    assert lib.snap_hip_run_gpu_sweep(ctypes.byref(a), None) == _lib.SNAP_ERR_INVALID
     a.mms_source = 0
-    a.vdelt = 0.5
-    assert lib.snap_hip_run_gpu_sweep(ctypes.byref(a), None) == _lib.SNAP_ERR_UNSUPPORTED
+    a.vdelt = 0.5  # without time-edge flux accessors
+    assert lib.snap_hip_run_gpu_sweep(ctypes.byref(a), None) == _lib.SNAP_ERR_INVALID
     a.vdelt = 0.0
     a.corner = 3
     a.stride_x_positive = 0  # corner 3 sweeps +x
