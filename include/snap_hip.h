@@ -246,6 +246,8 @@ typedef struct {
                                sweeps of the whole source */
   int outer_ordered;        /* 1 = outer source in the CPU's summation order even
                                when not FP64_EXACT (default: FP64 matrix cores) */
+  int wide_sweep;           /* 1 = one workgroup per cell and sweep (the only form past
+                               128 angles per octant; up to 2048); 0 = tiled when it fits */
 } snap_solve_options_t;
 
 /* MIXED with refine = 1 (iterative refinement of each outer iteration): the
@@ -310,7 +312,7 @@ int snap_hip_solver_gradient(snap_solver_t *solver, const double *resp, int prec
                              int *iterations);
 /* The sweep plan the solver chose: plan[0..2] tile edges (x,y,z), plan[3]
  * waves per tile workgroup, plan[4] angles per lane, plan[5] cells per
- * wavefront. */
+ * wavefront (0 for the one-cell-per-workgroup sweep, whose tiles are 1x1x1). */
 int snap_hip_solver_plan(snap_solver_t *solver, int32_t plan[6]);
 void snap_hip_solver_destroy(snap_solver_t *solver);
 

@@ -161,6 +161,9 @@ struct SweepPlan {
   int sb = 0;                  // cells per per-wave moment batch
   int maxdiag = 0;             // most cells on one cell diagonal of a tile
   bool td = false;             // for the time-dependent kernels (512 threads at most)
+  // one workgroup per cell (any angle count): tiles are single cells, the
+  // faces are in-place planes per sweep and `waves` spans the cell's angles
+  bool wide = false;
   int nplanes = 0;             // tile diagonals
   int ncellplanes = 0;         // cell diagonals inside one tile
   std::vector<int> plane_off;  // host: offset of each tile diagonal in plane_tiles
@@ -176,6 +179,9 @@ struct SweepPlan {
 int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int TZ, int waves,
                 int apl = 0, bool td = false);
 void plan_free(SweepPlan *p);
+// The plan of the one-cell-per-workgroup sweep (angle counts past 128, or on
+// request): cell diagonals of the chunk, angles across up to 1024 lanes.
+int plan_create_wide(SweepPlan *p, Extent3 e, int nang, int cmom, bool td = false);
 // Fill in the tile edges, waves per workgroup and angles per lane left at 0
 // for a sweep whose faces are real_bytes wide (8 FP64, 4 FP32).
 void plan_choose(Extent3 e, int nang, int cmom, int real_bytes, int *TX, int *TY, int *TZ,

@@ -231,16 +231,24 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
   SNAP_TRY(quad_upload(&s->quad, ec.data(), mu.data(), eta.data(), xi.data(), w.data(), p->nang, p->nmom));
   int TX = s->opt.tile[0], TY = s->opt.tile[1], TZ = s->opt.tile[2], waves = s->opt.waves_per_tile;
   int apl = s->opt.angles_per_lane;
-  plan_choose(s->e, s->nang, s->cmom, s->opt.precision == SNAP_PRECISION_MIXED ? 4 : 8, &TX, &TY,
-              &TZ, &waves, &apl, s->td);
-  SNAP_TRY(plan_create(&s->plan, s->e, s->nang, s->cmom, TX, TY, TZ, waves, apl, s->td));
-  if (s->opt.precision == SNAP_PRECISION_MIXED) {
-    // the FP64 sweeps of a mixed solve (first sweep, refinement bases) get
-    // their own plan unless the caller fixed the tiling
-    int tx = s->opt.tile[0], ty = s->opt.tile[1], tz = s->opt.tile[2];
-    int w = s->opt.waves_per_tile, a = s->opt.angles_per_lane;
-    plan_choose(s->e, s->nang, s->cmom, 8, &tx, &ty, &tz, &w, &a, s->td);
-    SNAP_TRY(plan_create(&s->plan64, s->e, s->nang, s->cmom, tx, ty, tz, w, a, s->td));
+  // more angles than a wavefront's lanes hold twice over: one workgroup per cell
+  const bool wide = s->nang > 128 || s->opt.wide_sweep;
+  if (wide) {
+    SNAP_TRY(plan_create_wide(&s->plan, s->e, s->nang, s->cmom, s->td));
+    if (s->opt.precision == SNAP_PRECISION_MIXED)
+      SNAP_TRY(plan_create_wide(&s->plan64, s->e, s->nang, s->cmom, s->td));
+  } else {
+    plan_choose(s->e, s->nang, s->cmom, s->opt.precision == SNAP_PRECISION_MIXED ? 4 : 8, &TX,
+                &TY, &TZ, &waves, &apl, s->td);
+    SNAP_TRY(plan_create(&s->plan, s->e, s->nang, s->cmom, TX, TY, TZ, waves, apl, s->td));
+    if (s->opt.precision == SNAP_PRECISION_MIXED) {
+      // the FP64 sweeps of a mixed solve (first sweep, refinement bases) get
+      // their own plan unless the caller fixed the tiling
+      int tx = s->opt.tile[0], ty = s->opt.tile[1], tz = s->opt.tile[2];
+      int w = s->opt.waves_per_tile, a = s->opt.angles_per_lane;
+      plan_choose(s->e, s->nang, s->cmom, 8, &tx, &ty, &tz, &w, &a, s->td);
+      SNAP_TRY(plan_create(&s->plan64, s->e, s->nang, s->cmom, tx, ty, tz, w, a, s->td));
+    }
   }
 
   SNAP_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -490,6 +498,7 @@ extern "C" int snap_hip_solver_gradient(snap_solver_t *s, const double *resp, in
   if (!s || !resp || !grad) return fail(SNAP_ERR_INVALID, "NULL argument");
   if (s->nmom != 1) return fail(SNAP_ERR_UNSUPPORTED, "the adjoint handles nmom = 1 (isotropic scattering)");
   if (s->td) return fail(SNAP_ERR_UNSUPPORTED, "the adjoint is of the steady-state solve");
+  if (s->plan.wide) return fail(SNAP_ERR_UNSUPPORTED, "the adjoint sweep is tiled (up to 64 angles)");
   if (precision != SNAP_PRECISION_FP64 && precision != SNAP_PRECISION_MIXED)
     return fail(SNAP_ERR_INVALID, "adjoint precision must be FP64 or MIXED (FP32 sweeps)");
   if (!(tol > 0.0) || maxit < 1) return fail(SNAP_ERR_INVALID, "bad tolerance or iteration cap");

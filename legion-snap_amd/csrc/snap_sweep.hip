@@ -96,13 +96,22 @@ __host__ __device__ inline SweepLds sweep_lds(int TX, int TY, int TZ, int nang, 
 // ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
+// LDS of the one-cell workgroup: fixup counts [2][16] | per-wave moment
+// partials [16][16] | w*pc of every angle (the ordered moment sums)
+__host__ __device__ inline size_t wide_lds(int nang) {
+  return 2 * 16 * 4 + 16 * 16 * 8 + (size_t)nang * 8;
+}
+
 size_t SweepPlan::lds_bytes(int real_bytes, bool recompute_dinv) const {
   (void)recompute_dinv;
+  if (wide) return wide_lds(nang);
   return sweep_lds(TX, TY, TZ, nang, cmom, waves, sb, TX * TY * TZ, ncellplanes, cpw, apl,
                    real_bytes).total;
 }
 
 size_t SweepPlan::face_doubles_per_sweep() const {
+  if (wide)  // in-place x, y and z face planes
+    return ((size_t)ext.ny * ext.nz + (size_t)ext.nx * ext.nz + (size_t)ext.nx * ext.ny) * nang;
   return (size_t)ntiles * (TY * TZ + TX * TZ + TX * TY) * nang;
 }
 
@@ -285,6 +294,56 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
   SNAP_HIP_TRY(hipMemcpy(p->cell_off, coff.data(), sizeof(int) * coff.size(),
                          hipMemcpyHostToDevice));
   SNAP_HIP_TRY(hipMemcpy(p->cells, cells.data(), sizeof(int16_t) * cells.size(),
+                         hipMemcpyHostToDevice));
+  return SNAP_OK;
+}
+
+// Most angles the one-cell workgroup holds: 1024 lanes with two angles each.
+constexpr int kWideMaxAngles = 2048;
+
+int plan_create_wide(SweepPlan *p, Extent3 e, int nang, int cmom, bool td) {
+  plan_free(p);
+  if (e.nx < 1 || e.ny < 1 || e.nz < 1) return fail(SNAP_ERR_INVALID, "empty chunk");
+  if (nang < 1 || nang > kWideMaxAngles)
+    return fail(SNAP_ERR_UNSUPPORTED, "the sweep handles 1..%d angles per octant, got %d",
+                kWideMaxAngles, nang);
+  if (cmom < 1 || cmom > 16) return fail(SNAP_ERR_INVALID, "cmom must be 1..16, got %d", cmom);
+  if (e.nx > 1023 || e.ny > 1023 || e.nz > 1023)
+    return fail(SNAP_ERR_INVALID, "chunk edges above 1023 cells");
+  p->ext = e;
+  p->TX = p->TY = p->TZ = 1;
+  p->nTx = e.nx;
+  p->nTy = e.ny;
+  p->nTz = e.nz;
+  p->ntiles = e.nx * e.ny * e.nz;
+  p->nang = nang;
+  p->cmom = cmom;
+  p->wide = true;
+  p->td = td;
+  // one angle per lane up to 1024 angles, two beyond; whole wavefronts
+  p->apl = nang > 1024 ? 2 : 1;
+  const int lanes = (nang + p->apl - 1) / p->apl;
+  p->waves = (lanes + 63) / 64;
+  p->lpc = lanes;
+  p->cpw = 0;
+  p->sb = 0;
+  p->maxdiag = 0;
+  p->ncellplanes = 1;
+  p->nplanes = e.nx + e.ny + e.nz - 2;
+  std::vector<int> cells;
+  p->plane_off.assign(p->nplanes + 1, 0);
+  for (int D = 0; D < p->nplanes; D++) {
+    p->plane_off[D] = (int)cells.size();
+    for (int k = 0; k < e.nz; k++)
+      for (int j = 0; j < e.ny; j++) {
+        const int i = D - j - k;
+        if (i >= 0 && i < e.nx) cells.push_back(i | (j << 10) | (k << 20));
+      }
+    p->maxdiag = std::max(p->maxdiag, (int)cells.size() - p->plane_off[D]);
+  }
+  p->plane_off[p->nplanes] = (int)cells.size();
+  SNAP_HIP_TRY(hipMalloc((void **)&p->plane_tiles, sizeof(int) * cells.size()));
+  SNAP_HIP_TRY(hipMemcpy(p->plane_tiles, cells.data(), sizeof(int) * cells.size(),
                          hipMemcpyHostToDevice));
   return SNAP_OK;
 }
@@ -1025,6 +1084,318 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
 }
 
 // ---------------------------------------------------------------------------
+// one cell per workgroup (any angle count)
+// ---------------------------------------------------------------------------
+// Each workgroup sweeps one cell of one (corner, group) sweep; launch D covers
+// the cells on chunk diagonal i+j+k = D of every sweep of the job.  Thread t
+// holds angles t + r*blockDim (r < APL), so per-angle arrays (dinv, time-edge
+// fluxes, MMS source, faces) are read and written in contiguous runs.  The
+// inflow faces come from per-sweep face planes updated in place, as the
+// reference's GPU sweep keeps its x/y/z flux planes (gpu_sweep.cu:1470-1555):
+// the cell at (i,j,k) reads the x plane at (j,k) written by (i-1,j,k) on the
+// previous diagonal and overwrites it with its outflow.  The fixup count is
+// cell-global: wave ballots, then a sum over the workgroup's waves in LDS
+// (gpu_sweep.cu:1596-1614 reduces over the CTA the same way).
+template <typename Real, int APL, bool EXACT, bool GEN, bool TD>
+__global__ void __launch_bounds__(1024) k_sweep_wide(SweepArgs A, SweepGeneral G, int plane_off) {
+  constexpr bool kFP64 = std::is_same<Real, double>::value;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  int *cnt = (int *)smem_raw;                       // [2][16] per-wave negative counts
+  double *part = (double *)(smem_raw + 2 * 16 * 4);  // [16 waves][16 moments]
+  double *wpc = part + 16 * 16;                      // [nang] w*pc
+  const int nang = A.nang, cmom = A.cmom;
+  const int tid = threadIdx.x, T = blockDim.x, lane = tid & 63, wave = tid >> 6, W = T >> 6;
+  const int s = blockIdx.y;
+  const int corner = A.corner0 + s / A.ng_job;
+  const int g = s % A.ng_job;
+  const int tp = A.plane_tiles[plane_off + blockIdx.x];
+  const int i = tp & 1023, j = (tp >> 10) & 1023, k = (tp >> 20) & 1023;
+  const int nx = A.ext.nx, ny = A.ext.ny, nz = A.ext.nz;
+  const bool sxp = (corner & 1) != 0, syp = (corner & 2) != 0, szp = (corner & 4) != 0;
+  const int px = sxp ? i : nx - 1 - i, py = syp ? j : ny - 1 - j, pz = szp ? k : nz - 1 - k;
+  const int cell = (pz * ny + py) * nx + px;
+  const size_t cells = (size_t)nx * ny * nz;
+  Real *fxp = (Real *)A.faces +
+              (size_t)s * ((size_t)ny * nz + (size_t)nx * nz + (size_t)nx * ny) * nang;
+  Real *fyp = fxp + (size_t)ny * nz * nang;
+  Real *fzp = fyp + (size_t)nx * nz * nang;
+  Real *sxf = fxp + (size_t)(k * ny + j) * nang;
+  Real *syf = fyp + (size_t)(k * nx + i) * nang;
+  Real *szf = fzp + (size_t)(j * nx + i) * nang;
+
+  // per-angle constants of this thread's angles
+  int an[APL];
+  bool aok[APL];
+#pragma unroll
+  for (int r = 0; r < APL; r++) {
+    const int a = tid + r * T;
+    aok[r] = a < nang;
+    an[r] = aok[r] ? a : 0;
+  }
+  const double dhi = A.hi, dhj = A.hj, dhk = A.hk;
+  const Real hi = (Real)dhi, hj = (Real)dhj, hk = (Real)dhk;
+  const Real tolr = (Real)1.0e-12;
+  const double *ecc = A.ec + (size_t)corner * cmom * nang;
+
+  // the cell's source moments, t_xs and (TD) vdelt are uniform over the workgroup
+  const double *qc;
+  Real txs;
+  if constexpr (GEN) {
+    qc = (const double *)G.qtot.at(px, py, pz, 0);
+    txs = *(const Real *)G.txs.at(px, py, pz, 0);
+  } else {
+    qc = A.q + ((size_t)g * cells + cell) * cmom;
+    txs = ((const Real *)A.txs)[(size_t)g * cells + cell];
+  }
+  double vd = 0.0;
+  bool has_time = false, has_qim = false;
+  if constexpr (TD) {
+    vd = GEN ? A.vdelt0 : A.vdelt[g];
+    has_time = vd != 0.0;
+    has_qim = GEN ? G.qim.base != nullptr : A.qim != nullptr;
+  }
+  const size_t ab = (((size_t)corner * A.ng_job + g) * cells + cell) * (size_t)nang;  // dense angle base
+  const Real vdr = (Real)vd;
+
+  Real psi[APL], psii[APL], psij[APL], psik[APL], pc[APL], ti[APL], cx[APL], cy[APL], cz[APL];
+  double w_d[APL];
+#pragma unroll
+  for (int r = 0; r < APL; r++) {
+    const int a = an[r];
+    const double mu_d = A.mu[a], eta_d = A.eta[a], xi_d = A.xi[a];
+    w_d[r] = A.w[a];
+    if constexpr (EXACT) {
+      cx[r] = (Real)mu_d;
+      cy[r] = (Real)eta_d;
+      cz[r] = (Real)xi_d;
+    } else {
+      cx[r] = (Real)(mu_d * dhi);
+      cy[r] = (Real)(eta_d * dhj);
+      cz[r] = (Real)(xi_d * dhk);
+    }
+    // angular source (sweep.cc:331-351)
+    if constexpr (kFP64) {
+      double ps = qc[0];
+      for (int l = 1; l < cmom; l++) ps = madd<EXACT>(ecc[(size_t)l * nang + a], qc[l], ps);
+      if constexpr (TD) {
+        if (has_qim)
+          ps += GEN ? ((const double *)G.qim.at(px, py, pz, 0))[a] : A.qim[ab + a];
+      }
+      psi[r] = ps;
+    } else {
+      float pf = (float)qc[0];
+      for (int l = 1; l < cmom; l++) pf = maddf((float)ecc[(size_t)l * nang + a], (float)qc[l], pf);
+      if constexpr (TD) {
+        if (has_qim)
+          pf += (float)(GEN ? ((const double *)G.qim.at(px, py, pz, 0))[a] : A.qim[ab + a]);
+      }
+      psi[r] = pf;
+    }
+    ti[r] = Real(0);
+    if constexpr (TD) {
+      if (has_time)
+        ti[r] = (Real)(GEN ? ((const double *)G.tin.at(px, py, pz, 0))[a] : A.tin[ab + a]);
+    }
+    // inflow faces: the upstream cell's outflow, the ghost array at the
+    // chunk edge, or vacuum
+    Real vx = Real(0), vy = Real(0), vz = Real(0);
+    if (i > 0)
+      vx = sxf[a];
+    else if (GEN && G.gx.base)
+      vx = (Real)((const double *)G.gx.at(py, pz, 0))[a];
+    if (j > 0)
+      vy = syf[a];
+    else if (GEN && G.gy.base)
+      vy = (Real)((const double *)G.gy.at(px, pz, 0))[a];
+    if (k > 0)
+      vz = szf[a];
+    else if (GEN && G.gz.base)
+      vz = (Real)((const double *)G.gz.at(px, py, 0))[a];
+    psii[r] = vx;
+    psij[r] = vy;
+    psik[r] = vz;
+    const Real dv = GEN ? ((const Real *)G.dinv.at(px, py, pz, 0))[a]
+                        : ((const Real *)A.dinv)[((size_t)g * cells + cell) * nang + a];
+    // initial diamond-difference solution (sweep.cc:352-423)
+    Real p = psi[r];
+    if constexpr (kFP64 && EXACT) {
+      p += psii[r] * cx[r] * hi;
+      p += psij[r] * cy[r] * hj;
+      p += psik[r] * cz[r] * hk;
+      if (TD && has_time) p += vdr * ti[r];
+    } else if constexpr (kFP64) {
+      p = __builtin_fma(psii[r], cx[r], p);
+      p = __builtin_fma(psij[r], cy[r], p);
+      p = __builtin_fma(psik[r], cz[r], p);
+      if (TD && has_time) p = __builtin_fma(vdr, ti[r], p);
+    } else {
+      p = maddf(psii[r], cx[r], p);
+      p = maddf(psij[r], cy[r], p);
+      p = maddf(psik[r], cz[r], p);
+      if (TD && has_time) p = maddf(vdr, ti[r], p);
+    }
+    pc[r] = p * dv;
+  }
+
+  bool hx[APL], hy[APL], hz[APL], ht[APL];
+#pragma unroll
+  for (int r = 0; r < APL; r++) hx[r] = hy[r] = hz[r] = ht[r] = true;
+  if (A.fixup) {
+    // negative-flux fixup (sweep.cc:424-520), the count summed over the
+    // workgroup's waves; counts alternate between two LDS rows so one
+    // barrier per pass orders every write before every read
+    int buf = 0;
+    auto count_negative = [&]() -> int {
+      unsigned n = 0;
+#pragma unroll
+      for (int r = 0; r < APL; r++) {
+        const Real p2 = Real(2) * pc[r];
+        const bool nxn = aok[r] && p2 - psii[r] < Real(0);
+        const bool nyn = aok[r] && p2 - psij[r] < Real(0);
+        const bool nzn = aok[r] && p2 - psik[r] < Real(0);
+        const bool ntn = TD && has_time && aok[r] && p2 - ti[r] < Real(0);
+        hx[r] = hx[r] && !nxn;
+        hy[r] = hy[r] && !nyn;
+        hz[r] = hz[r] && !nzn;
+        ht[r] = ht[r] && !ntn;
+        n += (unsigned)__popcll(__builtin_amdgcn_ballot_w64(nxn)) +
+             (unsigned)__popcll(__builtin_amdgcn_ballot_w64(nyn)) +
+             (unsigned)__popcll(__builtin_amdgcn_ballot_w64(nzn)) +
+             (unsigned)__popcll(__builtin_amdgcn_ballot_w64(ntn));
+      }
+      if (lane == 0) cnt[buf * 16 + wave] = (int)n;
+      __syncthreads();
+      int total = 0;
+      for (int v = 0; v < W; v++) total += cnt[buf * 16 + v];
+      buf ^= 1;
+      return total;
+    };
+    int old_neg = count_negative();
+    if (old_neg != 0) {
+      const int ne = (TD && has_time) ? 4 : 3;
+      for (int it = 1; it < ne * nang + 2; ++it) {
+#pragma unroll
+        for (int r = 0; r < APL; r++) {
+          const Real vx = hx[r] ? Real(1) : Real(0), vy = hy[r] ? Real(1) : Real(0),
+                     vz = hz[r] ? Real(1) : Real(0), vt = ht[r] ? Real(1) : Real(0);
+          Real den, p;
+          if constexpr (kFP64 && EXACT) {
+            if (TD && has_time) {
+              p = psi[r] + 0.5 * (psii[r] * cx[r] * hi * (1.0 + vx) +
+                                  psij[r] * cy[r] * hj * (1.0 + vy) +
+                                  psik[r] * cz[r] * hk * (1.0 + vz) + ti[r] * vdr * (1.0 + vt));
+              den = (p <= 0.0) ? 0.0
+                               : (txs + cx[r] * hi * vx + cy[r] * hj * vy + cz[r] * hk * vz +
+                                  vdr * vt);
+            } else {
+              p = psi[r] + 0.5 * (psii[r] * cx[r] * hi * (1.0 + vx) +
+                                  psij[r] * cy[r] * hj * (1.0 + vy) +
+                                  psik[r] * cz[r] * hk * (1.0 + vz));
+              den = (p <= 0.0) ? 0.0 : (txs + cx[r] * hi * vx + cy[r] * hj * vy + cz[r] * hk * vz);
+            }
+          } else {
+            Real t = psii[r] * cx[r] * (Real(1) + vx) + psij[r] * cy[r] * (Real(1) + vy) +
+                     psik[r] * cz[r] * (Real(1) + vz);
+            Real dd = txs + cx[r] * vx + cy[r] * vy + cz[r] * vz;
+            if (TD && has_time) {
+              t += ti[r] * vdr * (Real(1) + vt);
+              dd += vdr * vt;
+            }
+            p = psi[r] + Real(0.5) * t;
+            den = (p <= Real(0)) ? Real(0) : dd;
+          }
+          pc[r] = (den < tolr) ? Real(0) : p / den;
+        }
+        const int neg = count_negative();
+        if (neg == old_neg) break;
+        old_neg = neg;
+      }
+    }
+  }
+
+  // outflows: the face planes in place, the ghost arrays at the chunk's far
+  // side, the time edge (sweep.cc:506-566)
+  double pcw[APL];
+#pragma unroll
+  for (int r = 0; r < APL; r++) {
+    const int a = an[r];
+    const Real p2 = Real(2) * pc[r];
+    const Real ox = (p2 - psii[r]) * (hx[r] ? Real(1) : Real(0));
+    const Real oy = (p2 - psij[r]) * (hy[r] ? Real(1) : Real(0));
+    const Real oz = (p2 - psik[r]) * (hz[r] ? Real(1) : Real(0));
+    pcw[r] = w_d[r] * (double)pc[r];
+    if (!aok[r]) continue;
+    if (GEN && i == nx - 1 && G.gx.base)
+      ((double *)G.gx.at(py, pz, 0))[a] = (double)ox;
+    else
+      sxf[a] = ox;
+    if (GEN && j == ny - 1 && G.gy.base)
+      ((double *)G.gy.at(px, pz, 0))[a] = (double)oy;
+    else
+      syf[a] = oy;
+    if (GEN && k == nz - 1 && G.gz.base)
+      ((double *)G.gz.at(px, py, 0))[a] = (double)oz;
+    else
+      szf[a] = oz;
+    if constexpr (TD) {
+      if (has_time) {
+        const double ot = (double)((p2 - ti[r]) * (ht[r] ? Real(1) : Real(0)));
+        if constexpr (GEN)
+          ((double *)G.tout.at(px, py, pz, 0))[a] = ot;
+        else
+          A.tout[ab + a] = ot;
+      }
+    }
+  }
+
+  // flux moments phi_l = sum_a ec[l][a] * w[a] pc[a] (sweep.cc:568-591)
+  auto store = [&](int l, double sum) {
+    if constexpr (GEN) {
+      double *dst = (l == 0) ? (double *)G.flux.at(px, py, pz, 0)
+                             : (double *)G.fluxm.at(px, py, pz, 0) + (l - 1);
+      atomicAdd(dst, sum);
+    } else {
+      const size_t sc = ((size_t)corner * A.ng_job + g) * cells + cell;
+      if (l == 0)
+        A.part_flux[sc] = sum;
+      else
+        A.part_fluxm[sc * (size_t)(cmom - 1) + (l - 1)] = sum;
+    }
+  };
+  if constexpr (EXACT) {
+    // the CPU's ascending angle order: one lane per moment sums them all
+#pragma unroll
+    for (int r = 0; r < APL; r++)
+      if (aok[r]) wpc[an[r]] = pcw[r];
+    __syncthreads();
+    if (tid < cmom) {
+      const int l = tid;
+      const double *e = ecc + (size_t)l * nang;
+      double sum = 0.0;
+      for (int a = 0; a < nang; a++) sum = (l == 0) ? sum + wpc[a] : madd<true>(e[a], wpc[a], sum);
+      store(l, sum);
+    }
+  } else {
+    // lane partials, wave sums, then the waves in order
+    for (int l = 0; l < cmom; l++) {
+      double v = 0.0;
+#pragma unroll
+      for (int r = 0; r < APL; r++)
+        if (aok[r]) v = (l == 0) ? v + pcw[r] : __builtin_fma(ecc[(size_t)l * nang + an[r]], pcw[r], v);
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) part[wave * 16 + l] = v;
+    }
+    __syncthreads();
+    if (tid < cmom) {
+      double sum = 0.0;
+      for (int v = 0; v < W; v++) sum += part[v * 16 + tid];
+      store(tid, sum);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
 typedef void (*SweepKernel)(SweepArgs, SweepGeneral, int);
@@ -1099,6 +1470,22 @@ static SweepKernel kernel_for_td(int cpw, int apl, bool gen) {
   return nullptr;
 }
 
+// One-cell workgroup kernels: FP64 (fused or exact) or FP32, steady or
+// time-dependent, dense or general (the general ones in the exact order).
+template <typename Real, bool EXACT>
+static SweepKernel kernel_for_wide(int apl, bool gen, bool td) {
+#define SNAP_WIDE(P, GN, TDV) \
+  if (apl == P && gen == GN && td == TDV) return k_sweep_wide<Real, P, EXACT, GN, TDV>;
+  SNAP_WIDE(1, false, false) SNAP_WIDE(2, false, false) SNAP_WIDE(1, false, true)
+  SNAP_WIDE(2, false, true)
+  if constexpr (EXACT && std::is_same<Real, double>::value) {
+    SNAP_WIDE(1, true, false) SNAP_WIDE(2, true, false) SNAP_WIDE(1, true, true)
+    SNAP_WIDE(2, true, true)
+  }
+#undef SNAP_WIDE
+  return nullptr;
+}
+
 static int ensure_lds_attr(SweepKernel k, size_t bytes) {
   // dynamic LDS above 64 KiB must be requested per kernel
   static std::mutex mu;
@@ -1125,7 +1512,10 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
     return fail(SNAP_ERR_INVALID, "batch sweeps are dense with dinv streamed");
   if (L.td && (L.batch || L.recompute_dinv))
     return fail(SNAP_ERR_INVALID, "time-dependent sweeps read dinv and run one problem");
-  if (L.td && !p.td) return fail(SNAP_ERR_INVALID, "the plan is not one for time-dependent sweeps");
+  if (L.td && !p.td && !p.wide)
+    return fail(SNAP_ERR_INVALID, "the plan is not one for time-dependent sweeps");
+  if (p.wide && (L.batch || L.recompute_dinv))
+    return fail(SNAP_ERR_INVALID, "one-cell sweeps read dinv and run one problem");
   SweepArgs A;
   memset(&A, 0, sizeof(A));
   SweepGeneral G;
@@ -1202,7 +1592,10 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
   }
   const size_t lds = p.lds_bytes(L.fp32 ? 4 : 8, L.recompute_dinv);
   const SweepKernel k =
-      L.td ? (L.fp32 ? kernel_for_td<float, false>(p.cpw, p.apl, L.general)
+      p.wide ? (L.fp32 ? kernel_for_wide<float, false>(p.apl, L.general, L.td)
+                       : (L.exact ? kernel_for_wide<double, true>(p.apl, L.general, L.td)
+                                  : kernel_for_wide<double, false>(p.apl, L.general, L.td)))
+      : L.td ? (L.fp32 ? kernel_for_td<float, false>(p.cpw, p.apl, L.general)
                      : (L.exact ? kernel_for_td<double, true>(p.cpw, p.apl, L.general)
                                 : kernel_for_td<double, false>(p.cpw, p.apl, L.general)))
       : L.batch ? (L.fp32 ? kernel_for_batch<float, false>(p.cpw, p.apl)
@@ -1327,9 +1720,14 @@ extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream
       plan = pp.second;
   if (!plan) {
     int TX = 0, TY = 0, TZ = 0, waves = 0, apl = 0;
-    plan_choose(e, c.quad.nang, c.quad.cmom, 8, &TX, &TY, &TZ, &waves, &apl, td);
     plan = new SweepPlan();
-    int rc = plan_create(plan, e, c.quad.nang, c.quad.cmom, TX, TY, TZ, waves, apl, td);
+    int rc;
+    if (c.quad.nang > 128) {  // past two angles per lane of a wavefront
+      rc = plan_create_wide(plan, e, c.quad.nang, c.quad.cmom, td);
+    } else {
+      plan_choose(e, c.quad.nang, c.quad.cmom, 8, &TX, &TY, &TZ, &waves, &apl, td);
+      rc = plan_create(plan, e, c.quad.nang, c.quad.cmom, TX, TY, TZ, waves, apl, td);
+    }
     if (rc != SNAP_OK) {
       delete plan;
       return rc;

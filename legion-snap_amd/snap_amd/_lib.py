@@ -81,7 +81,7 @@ class SolveOptions(ctypes.Structure):
         ("precision", ctypes.c_int), ("tile", ctypes.c_int * 3),
         ("waves_per_tile", ctypes.c_int), ("first_sweep_fp64", ctypes.c_int),
         ("dinv_mode", ctypes.c_int), ("angles_per_lane", ctypes.c_int), ("refine", ctypes.c_int),
-        ("outer_ordered", ctypes.c_int),
+        ("outer_ordered", ctypes.c_int), ("wide_sweep", ctypes.c_int),
     ]
 
 

@@ -53,12 +53,14 @@ class Solver:
     refine (mixed only): defect correction per outer iteration -- the first
     inner sweep runs in FP64 and is kept as the base; later inners sweep the
     FP64 source defect in FP32 without the fixup and add it to the base.
+    wide_sweep: one workgroup per cell and sweep, with angles across up to
+    1024 lanes -- the form every solve past 128 angles per octant takes.
     """
 
     def __init__(self, problem: Problem, precision: str = FP64, tile=(0, 0, 0),
                  waves_per_tile: int = 0, first_sweep_fp64: bool = False,
                  dinv: str = "default", angles_per_lane: int = 0, refine: bool = False,
-                 outer_ordered: bool = False):
+                 outer_ordered: bool = False, wide_sweep: bool = False):
         problem.validate()
         self.problem = p = problem.contiguous()
         self.lib = _lib.load()
@@ -73,6 +75,7 @@ class Solver:
         o.angles_per_lane = angles_per_lane
         o.refine = int(bool(refine))
         o.outer_ordered = int(bool(outer_ordered))
+        o.wide_sweep = int(bool(wide_sweep))
         self.precision = precision
         h = ctypes.c_void_p()
         _lib.check(self.lib.snap_hip_solver_create(ctypes.byref(s), ctypes.byref(o), ctypes.byref(h)),
