@@ -83,6 +83,48 @@ def cpu_baseline(problem, budget_s: float = 25.0):
     }
 
 
+def deck_suite(names, cpu: bool) -> dict:
+    """Every reference input deck through the GPU solver (FP64, fused): the
+    decks' own sizes, time steps and iteration control.  The CPU column times
+    the oracle's steady sweeps of the same deck (one inner iteration, reduced
+    groups for the large ones) as the per-unit CPU cost."""
+    from oracle import oracle
+    from snap_amd import deck
+    from snap_amd.solver import Solver
+    import dataclasses
+    out = {}
+    for name in names:
+        path = os.path.join(REPO, "tests", "golden", "decks", name)
+        p, _ = deck.read_deck(path)
+        with Solver(p, precision="fp64") as s:
+            s.run()  # warm-up
+            st = s.run()
+            plan = s.plan()
+            mms = s.mms_compare() if p.mms else None
+        row = {"cells": p.ncells, "groups": p.ng, "angles": p.nang, "steps": p.nsteps,
+               "inner": st["inner_total"], "outer": st["outer_total"],
+               "converged": st["converged"], "grind_ns": st["grind_ns"],
+               "solve_s": st["solve_ms"] / 1e3, "sweep_s": st["sweep_ms"] / 1e3,
+               "sweep": "one cell per workgroup" if plan["cells_per_wave"] == 0 else "tiled"}
+        if mms is not None:
+            row["mms_max_diff"] = mms[0]
+        if cpu:
+            q = dataclasses.replace(p, timedep=False, mms=False,
+                                    qi=np.ones((p.ng, p.nz, p.ny, p.nx)))
+            units = p.ncells * 8 * p.nang * p.ng
+            if units > 4e8:  # fewer groups keep the sample near a second
+                g = max(1, int(p.ng * 4e8 / units))
+                q = dataclasses.replace(q, ng=g, sigt=q.sigt[:, :g], siga=q.siga[:, :g],
+                                        sigs=q.sigs[:, :g], slgg=q.slgg[:g, :, :g],
+                                        qi=q.qi[:g])
+            t = oracle.time_sweeps(q, 1)
+            row["cpu_grind_ns"] = t * 1e9 / (q.ncells * 8 * q.nang * q.ng)
+            row["speedup_vs_cpu"] = row["cpu_grind_ns"] / row["grind_ns"]
+        out[name] = row
+        print(name, json.dumps(row), file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +134,9 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the adjoint (config 3) and batch (config 4) measurements")
     ap.add_argument("--size", type=int, default=64, help="grid edge (default: the 64^3 workload)")
+    ap.add_argument("--decks", nargs="*", default=None,
+                    help="run the reference input decks (all when no names are given) and "
+                         "print their suite line instead of the default workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy and exit (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE traffic counters)")
@@ -108,6 +153,17 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         return 2
+    if args.decks is not None:
+        names = args.decks or sorted(os.listdir(os.path.join(REPO, "tests", "golden", "decks")))
+        names = [n for n in names if n.endswith(".in")]
+        rows = deck_suite(names, cpu=not args.no_cpu)
+        tot_units = sum(r["cells"] * 8 * r["angles"] * r["groups"] * r["inner"] for r in rows.values())
+        tot_sweep = sum(r["sweep_s"] for r in rows.values())
+        print(json.dumps({"metric": "reference input decks: FP64 sweep grind time and solve time",
+                          "value": tot_sweep * 1e9 / tot_units, "unit": "ns/(cell·angle·group)",
+                          "n_gpus": 1, "higher_is_better": False, "dtype": "f64",
+                          "data": "the reference's input decks (input/*.in)", "decks": rows}))
+        return 0
     if args.copy_probe:
         n = 1 << 27  # 1 GiB of doubles
         src = torch.ones(n, dtype=torch.float64, device="cuda")

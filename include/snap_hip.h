@@ -288,6 +288,30 @@ int snap_hip_solver_run(snap_solver_t *solver, snap_solve_stats_t *stats);
 int snap_hip_solver_mms_compare(snap_solver_t *solver, double result[3]);
 /* The manufactured scalar flux [ng][cells] the compare uses. */
 int snap_hip_solver_get_mms_reference(snap_solver_t *solver, double *ref_flux);
+
+/* One convergence test of the last run, in order: the events the reference's
+ * ConvergenceMonad binds (convergence.cc:153-236).  kind 0 is an inner test
+ * (after every inner iteration), kind 1 an outer test (every outer iteration
+ * after the first, snap.cc:519-525).  t_us is the host time at which the
+ * test's result was known, in microseconds from the start of the run. */
+typedef struct {
+  int kind;
+  int time_step, outer, inner;   /* 0-based loop indices */
+  int converged;
+  double t_us;
+} snap_iteration_record_t;
+/* Copy up to `capacity` records of the last run; *count gets the total. */
+int snap_hip_solver_get_log(snap_solver_t *solver, snap_iteration_record_t *records,
+                            int capacity, int *count);
+
+/* The reference mapper's tunables (Snap::SnapMapper::select_tunable_value,
+ * mapper.cc:101-172) for a deck: out[0] outer runahead (2), out[1] inner
+ * runahead (iitm), out[2] energy groups per sweep task, for `processors`
+ * GPUs (or CPUs), out[3] SMs per sweep for a device with `sms` SMs.  The
+ * solver issues all groups of an inner iteration as one job and breaks at
+ * the first converged test, which the predicated runahead reproduces. */
+int snap_hip_mapper_tunables(int num_groups, int max_inner_iters, int processors, int sms,
+                             int32_t out[4]);
 /* Copy the scalar flux [ng][cells] and flux moments [ng][cells][cmom-1]
  * (may be NULL) to host memory. */
 int snap_hip_solver_get_flux(snap_solver_t *solver, double *flux0, double *fluxm);

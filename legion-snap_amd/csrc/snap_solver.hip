@@ -22,6 +22,7 @@
 // carries over, and a manufactured-solution source is rescaled to the step's
 // mid-time before its outer iterations.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <memory>
 
@@ -79,6 +80,7 @@ struct snap_solver {
   DeviceBuffer tflux[2];            // time-edge fluxes [8][ng][cells][nang], even / odd
   int step = 0;                     // time step in progress: tflux[step & 1] is the inflow
   MmsState mms;
+  std::vector<snap_iteration_record_t> log;  // convergence tests of the last run
 
   ~snap_solver() {
     mms_free(&mms);
@@ -366,6 +368,18 @@ extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) 
   float sweep_ms = 0.f;
   bool all_converged = true;
   const double dt = s->prob.tf / (double)s->prob.nsteps;
+  s->log.clear();
+  const auto t0 = std::chrono::steady_clock::now();
+  auto record = [&](int kind, int cy, int otno, int inno, bool conv) {
+    snap_iteration_record_t r;
+    r.kind = kind;
+    r.time_step = cy;
+    r.outer = otno;
+    r.inner = inno;
+    r.converged = conv ? 1 : 0;
+    r.t_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+    s->log.push_back(r);
+  };
   for (int cy = 0; cy < s->prob.nsteps; cy++) {
     s->step = cy;
     // MMSScale (snap.cc:452-467): the source at the step's mid-time
@@ -404,15 +418,16 @@ extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) 
           st.sweeps_fp64++;
         st.inner_total++;
         SNAP_TRY(s->test(s->flux0pi, s->prob.epsi, &inner_converged));
+        record(0, cy, otno, inno, inner_converged);
         float ms = 0.f;
         SNAP_HIP_TRY(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
         sweep_ms += ms;
         if (inner_converged) break;
       }
       if (otno == 0) continue;
-      if (!inner_converged) continue;
       bool outer_converged = false;
-      SNAP_TRY(s->test(s->flux0po, 100.0 * s->prob.epsi, &outer_converged));
+      if (inner_converged) SNAP_TRY(s->test(s->flux0po, 100.0 * s->prob.epsi, &outer_converged));
+      record(1, cy, otno, -1, outer_converged);
       if (outer_converged) {
         converged = true;
         break;
@@ -442,6 +457,32 @@ extern "C" int snap_hip_solver_mms_compare(snap_solver_t *s, double result[3]) {
   SNAP_HIP_TRY(hipStreamSynchronize(s->stream));
   SNAP_HIP_TRY(hipMemcpy(f.data(), s->flux0.p, sizeof(double) * gc, hipMemcpyDeviceToHost));
   mms_compare(f.data(), s->mms.ref.data(), gc, result);
+  return SNAP_OK;
+}
+
+extern "C" int snap_hip_solver_get_log(snap_solver_t *s, snap_iteration_record_t *records,
+                                       int capacity, int *count) {
+  if (!s || !count || (capacity > 0 && !records)) return fail(SNAP_ERR_INVALID, "NULL argument");
+  const int n = (int)s->log.size();
+  *count = n;
+  for (int i = 0; i < n && i < capacity; i++) records[i] = s->log[i];
+  return SNAP_OK;
+}
+
+extern "C" int snap_hip_mapper_tunables(int num_groups, int max_inner_iters, int processors,
+                                        int sms, int32_t out[4]) {
+  if (!out || num_groups < 1 || max_inner_iters < 1 || processors < 1 || sms < 1)
+    return fail(SNAP_ERR_INVALID, "bad tunable query");
+  out[0] = 2;                // OUTER_RUNAHEAD_TUNABLE
+  out[1] = max_inner_iters;  // INNER_RUNAHEAD_TUNABLE: the whole inner loop
+  // SWEEP_ENERGY_CHUNKS_TUNABLE: 8 directions x groups over the processors,
+  // a quarter of that when it is at least 4, at most all groups
+  int chunk = 8 * num_groups / processors;
+  if (chunk >= 4) chunk /= 4;
+  if (chunk > num_groups) chunk = num_groups;
+  out[2] = chunk;
+  // GPU_SMS_PER_SWEEP_TUNABLE
+  out[3] = sms <= 4 ? sms : (sms <= 16 ? 4 : 8);
   return SNAP_OK;
 }
 

@@ -103,6 +103,11 @@ class BatchStats(ctypes.Structure):
                 ("angles_per_lane", ctypes.c_int), ("autotune_ms", ctypes.c_double)]
 
 
+class IterationRecord(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("time_step", ctypes.c_int), ("outer", ctypes.c_int),
+                ("inner", ctypes.c_int), ("converged", ctypes.c_int), ("t_us", ctypes.c_double)]
+
+
 class SolveStats(ctypes.Structure):
     _fields_ = [
         ("inner_total", ctypes.c_int), ("outer_total", ctypes.c_int),
@@ -160,6 +165,10 @@ SIGNATURES = {
         [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "snap_hip_solver_plan": ([_vp, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     "snap_hip_solver_mms_compare": ([_vp, _dp], ctypes.c_int),
+    "snap_hip_solver_get_log": ([_vp, ctypes.POINTER(IterationRecord), ctypes.c_int,
+                                 ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "snap_hip_mapper_tunables": ([ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int32)],
+                                 ctypes.c_int),
     "snap_hip_solver_get_mms_reference": ([_vp, _dp], ctypes.c_int),
     "snap_hip_batch_create": (
         [ctypes.POINTER(Problem), ctypes.c_int, ctypes.POINTER(BatchOptions), ctypes.POINTER(_vp)],

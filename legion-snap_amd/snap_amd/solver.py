@@ -132,6 +132,19 @@ class Solver:
                    "snap_hip_solver_mms_compare")
         return float(out[0]), float(out[1]), float(out[2])
 
+    def iteration_log(self) -> list:
+        """The convergence tests of the last run as dicts (kind "inner" or
+        "outer", time_step, outer, inner, converged, t_us)."""
+        n = ctypes.c_int()
+        _lib.check(self.lib.snap_hip_solver_get_log(self._h, None, 0, ctypes.byref(n)),
+                   "snap_hip_solver_get_log")
+        recs = (_lib.IterationRecord * max(1, n.value))()
+        _lib.check(self.lib.snap_hip_solver_get_log(self._h, recs, n.value, ctypes.byref(n)),
+                   "snap_hip_solver_get_log")
+        return [{"kind": "inner" if r.kind == 0 else "outer", "time_step": r.time_step,
+                 "outer": r.outer, "inner": r.inner, "converged": bool(r.converged),
+                 "t_us": r.t_us} for r in recs[:n.value]]
+
     def mms_reference(self) -> np.ndarray:
         """The manufactured scalar flux [ng, nz, ny, nx] (mms problems only)."""
         p = self.problem

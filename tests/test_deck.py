@@ -86,3 +86,41 @@ def test_base_deck_solves(oracle_mod):
     p, _ = deck.read_deck(os.path.join(DECKS, "base.in"))
     flux, _, st = oracle_mod.solve(p)
     assert st["converged"] and np.all(flux > 0)
+
+
+def test_mapper_tunables():
+    """select_tunable_value (mapper.cc:101-172): runahead 2 / iitm; energy
+    chunk = 8*ng/processors, quartered when >= 4, at most ng; SMs per sweep
+    = all up to 4, 4 up to 16, else 8."""
+    from snap_amd.run_deck import tunables
+    assert tunables(4, 5) == {"outer_runahead": 2, "inner_runahead": 5, "energy_chunk": 4,
+                              "sms_per_sweep": 8}
+    assert tunables(96, 5)["energy_chunk"] == 96          # 768 / 4 = 192, clamped to ng
+    assert tunables(32, 5, processors=128)["energy_chunk"] == 2    # 256/128 = 2 (< 4)
+    assert tunables(4, 5, processors=8)["energy_chunk"] == 1       # 32/8 = 4 -> 1
+    assert tunables(4, 5, sms=3)["sms_per_sweep"] == 3
+    assert tunables(4, 5, sms=12)["sms_per_sweep"] == 4
+
+
+def test_monad_report_counters():
+    """The ConvergenceMonad's counters (convergence.cc:153-236): the inner
+    count restarts after a converged inner and at each outer test; there is
+    no outer test in the first outer iteration of a step."""
+    from snap_amd.run_deck import monad_report
+    ev = [("inner", 0, False), ("inner", 1, True),            # outer 0: inners 0, 1
+          ("inner", 2, True), ("outer", 3, False),             # outer 1
+          ("inner", 4, False), ("inner", 5, True), ("outer", 6, True),   # outer 2, step done
+          ("inner", 7, True)]
+    log = [{"kind": k, "t_us": float(10 * t), "converged": c} for k, t, c in ev]
+    lines = monad_report(log)
+    assert lines[0] == "Inner loop 0 of outer loop 0 of time step 0 did not converge in 0 microseconds"
+    assert lines[1] == "Inner loop 1 of outer loop 0 of time step 0 CONVERGED in 10 microseconds"
+    assert lines[2].startswith("Inner loop 0 of outer loop 0 of time step 0 CONVERGED")
+    assert lines[3] == "Outer loop 0 of time step 0 did not converge in 30 microsecond"
+    assert lines[4].startswith("Inner loop 0 of outer loop 1 of time step 0 did not converge")
+    assert lines[5].startswith("Inner loop 1 of outer loop 1 of time step 0 CONVERGED")
+    assert lines[6] == "Outer loop 1 of time step 0 CONVERGED in 30 microseconds"
+    assert lines[7] == "Time step 0 took 60 microseconds"
+    assert lines[8].startswith("Inner loop 0 of outer loop 0 of time step 1 CONVERGED")
+    assert "  Total Time Steps: 1 (avg 60 us / iter)" in lines
+    assert "  Total Inner Loops: 6 (avg 11.666667 us / iter)" in lines
