@@ -61,6 +61,12 @@ typedef struct { int64_t lo[3]; int64_t hi[3]; } snap_rect3_t;
 const char *snap_hip_last_error(void);
 /* Library version string. */
 const char *snap_hip_version(void);
+/* The angular quadrature of Snap::compute_derived_globals (snap.cc:1171-1342,
+ * 3-D): mu, eta, xi, w [num_angles] and ec [8][cmom][num_angles], cmom =
+ * num_moments^2, host arrays.  What a caller passes to
+ * snap_hip_initialize_gpu_context. */
+int snap_hip_quadrature(int num_angles, int num_moments, double *mu, double *eta, double *xi,
+                        double *w, double *ec);
 
 /* ---------------------------------------------------------------------- */
 /* Drop-in task entry points                                               */

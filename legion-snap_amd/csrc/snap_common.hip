@@ -3,6 +3,8 @@
 #include <cstdarg>
 #include <cstring>
 
+#include <algorithm>
+
 #include "snap_internal.h"
 
 namespace snap {
@@ -171,3 +173,16 @@ void quad_free(QuadDev *q) {
 
 extern "C" const char *snap_hip_last_error(void) { return snap::g_last_error; }
 extern "C" const char *snap_hip_version(void) { return "snap_hip 0.1 (gfx950)"; }
+
+extern "C" int snap_hip_quadrature(int num_angles, int num_moments, double *mu, double *eta,
+                                   double *xi, double *w, double *ec) {
+  if (!mu || !eta || !xi || !w || !ec) return snap::fail(SNAP_ERR_INVALID, "NULL quadrature array");
+  std::vector<double> m, e, x, ww, c;
+  SNAP_TRY(snap::quad_compute(num_angles, num_moments, m, e, x, ww, c));
+  std::copy(m.begin(), m.end(), mu);
+  std::copy(e.begin(), e.end(), eta);
+  std::copy(x.begin(), x.end(), xi);
+  std::copy(ww.begin(), ww.end(), w);
+  std::copy(c.begin(), c.end(), ec);
+  return SNAP_OK;
+}

@@ -126,6 +126,7 @@ class SolveStats(ctypes.Structure):
 SIGNATURES = {
     "snap_hip_last_error": ([], ctypes.c_char_p),
     "snap_hip_version": ([], ctypes.c_char_p),
+    "snap_hip_quadrature": ([ctypes.c_int, ctypes.c_int] + [_dp] * 5, ctypes.c_int),
     "snap_hip_initialize_gpu_context": (
         [_dp, _dp, _dp, _dp, _dp] + [ctypes.c_int] * 6, ctypes.c_int),
     "snap_hip_run_gpu_sweep": ([ctypes.POINTER(SweepArgs), _vp], ctypes.c_int),

@@ -84,16 +84,30 @@ def initialize_gpu_context(quad, nang: int, nmom: int, chunk) -> None:
         "snap_hip_initialize_gpu_context")
 
 
+def quadrature(nang: int, nmom: int):
+    """compute_derived_globals (snap.cc:1171-1342): mu, eta, xi, w, ec[8][cmom][nang]."""
+    cmom = nmom * nmom
+    mu, eta, xi, w = (np.zeros(nang) for _ in range(4))
+    ec = np.zeros((8, cmom, nang))
+    dp = ctypes.POINTER(ctypes.c_double)
+    _lib.check(_lib.load().snap_hip_quadrature(nang, nmom, *(a.ctypes.data_as(dp) for a in
+                                                               (mu, eta, xi, w, ec))),
+               "snap_hip_quadrature")
+    return mu, eta, xi, w, ec
+
+
 def mini_kba(corner: int, groups: Sequence[int], qtot, flux, fluxm, dinv, t_xs,
              ghosts, hijk, nang: int, nmom: int, fixup: bool = True, origin=(0, 0, 0),
-             vdelt=None, time_flux_in=None, time_flux_out=None, qim=None) -> None:
+             vdelt=None, time_flux_in=None, time_flux_out=None, qim=None, extent=None) -> None:
     """One corner over a group range: run_gpu_sweep once per group.
-    ghosts: (gx [ng, nz, ny, nang], gy [ng, nz, nx, nang], gz [ng, ny, nx, nang]) or None.
+    ghosts: (gx [ng, nz, ny, nang], gy [ng, nz, nx, nang], gz [ng, ny, nx, nang]) or None,
+    indexed by absolute points.  extent: (nx, ny, nz) of the chunk at `origin`
+    (default: the whole of flux), whose arrays are indexed by absolute points.
     Time dependence (MiniKBATask regions 5, 6, 11, sweep.cc:45-84): vdelt
     [ng] host values, time_flux_in / time_flux_out [ng, nz, ny, nx, nang] of
     this corner; qim [ng, nz, ny, nx, nang] is the corner's MMS source."""
     lib = _lib.load()
-    nz, ny, nx = flux.shape[1:4]
+    nx, ny, nz = extent if extent is not None else flux.shape[1:4][::-1]
     for g in groups:
         a = _lib.SweepArgs()
         a.origin[0], a.origin[1], a.origin[2] = origin

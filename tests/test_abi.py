@@ -65,3 +65,14 @@ def test_check_raises_with_message():
     with pytest.raises(_lib.SnapError) as e:
         _lib.check(_lib.SNAP_ERR_INVALID, "x")
     assert "x failed" in str(e.value)
+
+
+def test_library_quadrature_is_the_oracles(oracle_mod):
+    """snap_hip_quadrature (host code, no GPU) against the oracle's
+    compute_derived_globals restatement, P0 to P3."""
+    import numpy as np
+    from snap_amd import tasks
+    for nang, nmom in ((10, 1), (36, 4), (7, 2), (1024, 1)):
+        a = tasks.quadrature(nang, nmom)
+        b = oracle_mod.quadrature(nang, nmom)
+        assert all(np.array_equal(x, y) for x, y in zip(a, b)), (nang, nmom)
