@@ -93,20 +93,20 @@ typedef struct {
   snap_accessor3_t flux;          /* RW  1 double per cell; the sweep ADDS its contribution */
   snap_accessor3_t fluxm;         /* RW  cmom-1 doubles per cell (base NULL when cmom == 1) */
   snap_accessor3_t dinv;          /* RO  num_angles doubles per cell */
-  snap_accessor3_t time_flux_in;  /* RO  unused: time-independent path only */
-  snap_accessor3_t time_flux_out; /* WO  unused: time-independent path only */
+  snap_accessor3_t time_flux_in;  /* RO  num_angles doubles per cell; read when vdelt != 0 */
+  snap_accessor3_t time_flux_out; /* WO  num_angles doubles per cell; written when vdelt != 0 */
   snap_accessor3_t t_xs;          /* RO  1 double per cell */
   snap_accessor2_t ghostx;        /* RW  num_angles doubles at point (y,z); base NULL = vacuum */
   snap_accessor2_t ghosty;        /* RW  num_angles doubles at point (x,z); base NULL = vacuum */
   snap_accessor2_t ghostz;        /* RW  num_angles doubles at point (x,y); base NULL = vacuum */
-  snap_accessor3_t qim;           /* RO  MMS angular source: must be unused (mms_source = 0) */
+  snap_accessor3_t qim;           /* RO  MMS angular source, num_angles doubles per cell (mms_source = 1) */
   int x_range, y_range, z_range;
   int corner;                     /* octant: bit0 +x, bit1 +y, bit2 +z */
   int stride_x_positive, stride_y_positive, stride_z_positive;
-  int mms_source;                 /* must be 0 */
+  int mms_source;                 /* 1: add qim to the angular source (sweep.cc:347-351) */
   int num_moments;                /* nmom; cmom = nmom^2 */
   double hi, hj, hk;              /* 2/dx, 2/dy, 2/dz */
-  double vdelt;                   /* must be 0.0 (steady state) */
+  double vdelt;                   /* 2/(dt v_g) of this group; 0.0 = steady state */
   int num_angles;
   int fixup;                      /* negative-flux fixup on/off */
 } snap_sweep_args_t;
