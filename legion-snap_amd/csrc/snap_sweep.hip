@@ -746,14 +746,29 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
       const double *e = ect + l;
       double sum = 0.0;
       int b = 0;
-      for (; b + 4 <= nang; b += 4) {
-        const double v0 = v[b], v1 = v[b + 1], v2 = v[b + 2], v3 = v[b + 3];
-        const double e0 = e[b * kEctRow], e1 = e[(b + 1) * kEctRow], e2 = e[(b + 2) * kEctRow],
-                     e3 = e[(b + 3) * kEctRow];
-        sum = madd<EXACT>(e0, v0, sum);
-        sum = madd<EXACT>(e1, v1, sum);
-        sum = madd<EXACT>(e2, v2, sum);
-        sum = madd<EXACT>(e3, v3, sum);
+      if constexpr (EXACT) {
+        for (; b + 4 <= nang; b += 4) {
+          const double v0 = v[b], v1 = v[b + 1], v2 = v[b + 2], v3 = v[b + 3];
+          const double e0 = e[b * kEctRow], e1 = e[(b + 1) * kEctRow], e2 = e[(b + 2) * kEctRow],
+                       e3 = e[(b + 3) * kEctRow];
+          sum = madd<EXACT>(e0, v0, sum);
+          sum = madd<EXACT>(e1, v1, sum);
+          sum = madd<EXACT>(e2, v2, sum);
+          sum = madd<EXACT>(e3, v3, sum);
+        }
+      } else {
+        // two independent chains (the order is free without EXACT)
+        double s1 = 0.0;
+        for (; b + 4 <= nang; b += 4) {
+          const double v0 = v[b], v1 = v[b + 1], v2 = v[b + 2], v3 = v[b + 3];
+          const double e0 = e[b * kEctRow], e1 = e[(b + 1) * kEctRow], e2 = e[(b + 2) * kEctRow],
+                       e3 = e[(b + 3) * kEctRow];
+          sum = __builtin_fma(e0, v0, sum);
+          s1 = __builtin_fma(e1, v1, s1);
+          sum = __builtin_fma(e2, v2, sum);
+          s1 = __builtin_fma(e3, v3, s1);
+        }
+        sum += s1;
       }
       for (; b < nang; b++) sum = madd<EXACT>(e[b * kEctRow], v[b], sum);
       if constexpr (GEN) {
@@ -839,10 +854,33 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
           double ps[APL];
 #pragma unroll
           for (int r = 0; r < APL; r++) ps[r] = q[0];
-          for (int l = 1; l < cmom; l++) {
-            const double ql = q[l];
+          if constexpr (EXACT) {
+            for (int l = 1; l < cmom; l++) {
+              const double ql = q[l];
 #pragma unroll
-            for (int r = 0; r < APL; r++) ps[r] = madd<EXACT>(ecs[l * ECR + an[r]], ql, ps[r]);
+              for (int r = 0; r < APL; r++) ps[r] = madd<EXACT>(ecs[l * ECR + an[r]], ql, ps[r]);
+            }
+          } else {
+            // two independent chains halve the dependent-FMA latency
+            double p1[APL];
+#pragma unroll
+            for (int r = 0; r < APL; r++) p1[r] = 0.0;
+            int l = 1;
+            for (; l + 1 < cmom; l += 2) {
+              const double q0 = q[l], q1 = q[l + 1];
+#pragma unroll
+              for (int r = 0; r < APL; r++) {
+                ps[r] = __builtin_fma(ecs[l * ECR + an[r]], q0, ps[r]);
+                p1[r] = __builtin_fma(ecs[(l + 1) * ECR + an[r]], q1, p1[r]);
+              }
+            }
+            if (l < cmom) {
+              const double q0 = q[l];
+#pragma unroll
+              for (int r = 0; r < APL; r++) ps[r] = __builtin_fma(ecs[l * ECR + an[r]], q0, ps[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < APL; r++) ps[r] += p1[r];
           }
           if constexpr (TD) {
             if (has_qim) {
@@ -853,15 +891,28 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
 #pragma unroll
           for (int r = 0; r < APL; r++) psi[r] = ps[r];
         } else {
-          float pf[APL];
+          float pf[APL], p1[APL];
 #pragma unroll
-          for (int r = 0; r < APL; r++) pf[r] = (float)q[0];
-          for (int l = 1; l < cmom; l++) {
-            const float ql = (float)q[l];
-#pragma unroll
-            for (int r = 0; r < APL; r++)
-              pf[r] = maddf((float)ecs[l * ECR + an[r]], ql, pf[r]);
+          for (int r = 0; r < APL; r++) {
+            pf[r] = (float)q[0];
+            p1[r] = 0.0f;
           }
+          int l = 1;
+          for (; l + 1 < cmom; l += 2) {
+            const float q0 = (float)q[l], q1 = (float)q[l + 1];
+#pragma unroll
+            for (int r = 0; r < APL; r++) {
+              pf[r] = maddf((float)ecs[l * ECR + an[r]], q0, pf[r]);
+              p1[r] = maddf((float)ecs[(l + 1) * ECR + an[r]], q1, p1[r]);
+            }
+          }
+          if (l < cmom) {
+            const float q0 = (float)q[l];
+#pragma unroll
+            for (int r = 0; r < APL; r++) pf[r] = maddf((float)ecs[l * ECR + an[r]], q0, pf[r]);
+          }
+#pragma unroll
+          for (int r = 0; r < APL; r++) pf[r] += p1[r];
           if constexpr (TD) {
             if (has_qim) {
 #pragma unroll

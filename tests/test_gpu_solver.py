@@ -160,7 +160,7 @@ def test_solver_rejects_bad_input(gpu):
     p.mat[0, 0, 0] = 5
     with pytest.raises(ValueError):
         Solver(p)
-    p = problems.config0(n=4, ng=2, nang=130)  # more angles than the tiled sweep handles
+    p = problems.config0(n=4, ng=2, nang=2049)  # more angles than a 1024-lane workgroup holds twice
     with pytest.raises(_lib.SnapError) as e:
         Solver(p)
     assert e.value.code == _lib.SNAP_ERR_UNSUPPORTED
