@@ -76,13 +76,13 @@ __host__ __device__ inline SweepLds sweep_lds(int TX, int TY, int TZ, int nang, 
   SweepLds L;
   size_t o = a16((size_t)(TY * TZ + TX * TZ + TX * TY) * nang * rb);
   L.ecs = o;
-  o += (size_t)cmom * ec_row(apl) * 8;
+  o += (size_t)cmom * ec_row(apl) * rb;
   L.ect = o;
   o += (size_t)nang * kEctRow * 8;
   L.pwb = o;
   o += (size_t)W * SB * nang * 8;
   L.qsl = o;
-  o += (size_t)W * max_steps(apl) * cpw * cmom * 8;
+  o += (size_t)W * max_steps(apl) * cpw * cmom * rb;
   L.cix = o;
   o = a16(o + (size_t)W * SB * 4);
   L.cells = o;
@@ -464,11 +464,11 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
   Real *fx = (Real *)smem_raw;
   Real *fy = fx + fxn;
   Real *fz = fy + fyn;
-  double *ecs = (double *)(smem_raw + lay.ecs);   // [cmom][ECR]: ec[l][a] at l*ECR + a
+  Real *ecs = (Real *)(smem_raw + lay.ecs);       // [cmom][ECR]: ec[l][a] at l*ECR + a
   double *ect = (double *)(smem_raw + lay.ect);   // [nang][kEctRow]: ec[l][a] at a*kEctRow + l
   double *pwb = (double *)(smem_raw + lay.pwb) + wave * SB * nang;  // this wave's w*pc batch
   int *cix = (int *)(smem_raw + lay.cix) + wave * SB;               // batch cell indices
-  double *qsl = (double *)(smem_raw + lay.qsl) + wave * MS * CPW * cmom;  // source moments
+  Real *qsl = (Real *)(smem_raw + lay.qsl) + wave * MS * CPW * cmom;  // source moments
   int4 *tcr = (int4 *)(smem_raw + lay.cells);  // per tile cell: chunk cell (or -1), face offsets
   int *toff = (int *)(smem_raw + lay.coff);    // offsets of the cell diagonals in tcr
 
@@ -569,7 +569,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
   for (int idx = threadIdx.x; idx < cmom * nang; idx += blockDim.x) {
     const double v = A.ec[(size_t)corner * cmom * nang + idx];
     const int l = idx / nang, b = idx - l * nang;
-    ecs[l * ECR + b] = v;
+    ecs[l * ECR + b] = (Real)v;
     ect[b * kEctRow + l] = v;
   }
   // Inflow slots of cells outside a ragged chunk edge are copied like the
@@ -664,7 +664,9 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
   // ---- register prefetch of this wave's cells on the next diagonal: dinv of
   // the lane's angles, its source moments and t_xs (TD: and the lane's
   // time-edge inflows and MS sources).
-  double n_dv[MS][APL], n_q[MS][QS], n_tx[MS], n_ti[MS][APL], n_qm[MS][APL];
+  // (dinv and t_xs in the sweep's type; moments in FP64 until they are staged)
+  Real n_dv[MS][APL], n_tx[MS];
+  double n_q[MS][QS], n_ti[MS][APL], n_qm[MS][APL];
   int n_cell[MS];
   auto load_cell = [&](int cell, int st) {
     if constexpr (TD) {
@@ -688,22 +690,22 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
       const Real *dv = (const Real *)G.dinv.at(px, py, pz, 0);
       const double *qc = (const double *)G.qtot.at(px, py, pz, 0);
 #pragma unroll
-      for (int r = 0; r < APL; r++) n_dv[st][r] = (double)dv[an[r]];
+      for (int r = 0; r < APL; r++) n_dv[st][r] = dv[an[r]];
 #pragma unroll
       for (int m = 0; m < QS; m++)
         if (q_in_lanes && la + m * LPC < cmom) n_q[st][m] = qc[la + m * LPC];
-      n_tx[st] = (double)*(const Real *)G.txs.at(px, py, pz, 0);
+      n_tx[st] = *(const Real *)G.txs.at(px, py, pz, 0);
     } else {
       if constexpr (!REC) {
         const Real *dv = dg + (size_t)cell * nang;
 #pragma unroll
-        for (int r = 0; r < APL; r++) n_dv[st][r] = (double)dv[an[r]];
+        for (int r = 0; r < APL; r++) n_dv[st][r] = dv[an[r]];
       }
       const double *qc = qg + (size_t)cell * cmom;
 #pragma unroll
       for (int m = 0; m < QS; m++)
         if (q_in_lanes && la + m * LPC < cmom) n_q[st][m] = qc[la + m * LPC];
-      n_tx[st] = (double)tg[cell];
+      n_tx[st] = tg[cell];
     }
   };
   auto prefetch = [&](int dd) {
@@ -712,14 +714,14 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
     for (int st = 0; st < MS; st++) {
       const int my = c0 + st * per_step + wave * CPW + cl;
 #pragma unroll
-      for (int r = 0; r < APL; r++) n_dv[st][r] = 0.0;
+      for (int r = 0; r < APL; r++) n_dv[st][r] = Real(0);
 #pragma unroll
       for (int m = 0; m < QS; m++) n_q[st][m] = 0.0;
       if constexpr (TD) {
 #pragma unroll
         for (int r = 0; r < APL; r++) n_ti[st][r] = n_qm[st][r] = 0.0;
       }
-      n_tx[st] = 0.0;
+      n_tx[st] = Real(0);
       n_cell[st] = -1;
       if (lane_ok && my < c1) {
         const int cell = tcr[my].x;
@@ -791,7 +793,8 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
   for (int d = 0; d < ncp; ++d) {
     const int c0 = toff[d], c1 = toff[d + 1];
     const int nsteps = (c1 - c0 + per_step - 1) / per_step;
-    double c_dv[MS][APL], c_tx[MS], c_ti[MS][APL], c_qm[MS][APL];
+    Real c_dv[MS][APL], c_tx[MS];
+    double c_ti[MS][APL], c_qm[MS][APL];
     int c_cell[MS];
     // source moments of this diagonal's cells to the wave's LDS slots
 #pragma unroll
@@ -810,7 +813,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
 #pragma unroll
       for (int m = 0; m < QS; m++)
         if (q_in_lanes && lane_ok && la + m * LPC < cmom)
-          qsl[(st * CPW + cl) * cmom + la + m * LPC] = n_q[st][m];
+          qsl[(st * CPW + cl) * cmom + la + m * LPC] = (Real)n_q[st][m];
     }
     if (!q_in_lanes) {
       // moments do not fit the cell's lanes: the wave copies its cells'
@@ -832,7 +835,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
             }
           }
         }
-        qsl[sc * cmom + l] = v;
+        qsl[sc * cmom + l] = (Real)v;
       }
     }
     if (d + 1 < ncp) prefetch(d + 1);
@@ -848,7 +851,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
       if (cell >= 0) {
         const int4 rec = tcr[my];
         // angular source (sweep.cc:331-345), moments from the LDS slots
-        const double *q = qsl + (st * CPW + cl) * cmom;
+        const Real *q = qsl + (st * CPW + cl) * cmom;
         Real psi[APL];
         if constexpr (kFP64) {
           double ps[APL];
@@ -922,7 +925,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
 #pragma unroll
           for (int r = 0; r < APL; r++) psi[r] = pf[r];
         }
-        const Real txs = (Real)c_tx[st];
+        const Real txs = c_tx[st];
         Real dv[APL];
 #pragma unroll
         for (int r = 0; r < APL; r++) {
@@ -934,7 +937,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
           else if constexpr (REC)
             dv[r] = 1.0f / ((float)txs + g_sum32[r]);
           else
-            dv[r] = (Real)c_dv[st][r];
+            dv[r] = c_dv[st][r];
         }
         // inflow faces; slots past the last angle read angle 0's entry and
         // are never stored
