@@ -139,3 +139,12 @@ def test_long_transient_settles_on_the_steady_solution(oracle_mod):
     # an early step is still far from it
     f_1, _, _ = oracle_mod.solve(dataclasses.replace(p, nsteps=2, tf=0.2))
     assert np.max(np.abs(f_1 / f_ss - 1.0)) > 1e-2
+
+
+def test_mms_compare_semantics(oracle_mod):
+    """MMSCompare (mms.cc:515-533): |phi/ref - 1|, or |phi| where ref < 1e-12."""
+    ref = np.array([2.0, 4.0, 1e-13, 0.0])
+    flux = np.array([2.2, 3.0, 0.25, -0.5])
+    mx, mn, mean = oracle_mod.mms_compare(flux, ref)
+    d = np.array([0.1, 0.25, 0.25, 0.5])
+    assert np.isclose(mx, 0.5) and np.isclose(mn, 0.1) and np.isclose(mean, d.mean())
