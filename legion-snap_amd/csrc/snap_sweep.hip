@@ -180,7 +180,7 @@ void plan_choose(Extent3 e, int nang, int cmom, int real_bytes, int *TX, int *TY
       const int need = (tile_maxdiag(tx, ty, tz) + ms * cpw - 1) / (ms * cpw);
       const int w = std::max(need, *waves > 0 ? *waves : (*apl == 1 ? 8 : 6));
       if (w > wmax) continue;
-      const int sb = cpw * std::min(std::max(1, 64 / (cmom * cpw)), std::max(1, 8 / cpw));
+      const int sb = cpw * std::min(std::max(1, 64 / (((cmom + 1) / 2) * cpw)), std::max(1, 8 / cpw));
       const size_t lds = sweep_lds(tx, ty, tz, nang, cmom, w, sb, tx * ty * tz, tx + ty + tz - 2,
                                    cpw, *apl, 8).total;
       if (lds > 160 * 1024) continue;
@@ -237,7 +237,7 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
   p->cpw = pick_cpw(p->lpc);
   // cells per moment batch: enough (cell, moment) sums to fill a wavefront
   {
-    int per = 64 / (cmom * p->cpw);
+    int per = 64 / (((cmom + 1) / 2) * p->cpw);  // lanes sum moment pairs
     if (per < 1) per = 1;
     p->sb = p->cpw * std::min(per, std::max(1, 8 / p->cpw));
   }
@@ -658,7 +658,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
   // la + m*LPC) when they fit; otherwise the wave copies them to LDS itself
   const bool q_in_lanes = cmom <= QS * LPC;
   // moment sums: lane -> (batched cell fk, moment fl) for the first 64 sums
-  const int fk = lane / cmom, fl = lane - fk * cmom;
+  const int pairs = (cmom + 1) / 2;  // moment pairs summed per lane in a flush
   __syncthreads();
 
   // ---- register prefetch of this wave's cells on the next diagonal: dinv of
@@ -734,56 +734,39 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
   int nbuf = 0;  // cells in this wave's moment batch (wave-uniform)
   auto flush = [&]() {
     // per-cell flux moments over angles in ascending angle order
-    // (sweep.cc:568-591): phi_l = sum_a ec[l][a] * (w[a]*pc[a])
+    // (sweep.cc:568-591): phi_l = sum_a ec[l][a] * (w[a]*pc[a]).  Each lane
+    // sums two adjacent moments of one cell, reading both coefficients of an
+    // angle with one 16-byte load from the angle-major table.
     wave_sync();
-    for (int o = lane; o < nbuf * cmom; o += 64) {
-      int kk = fk, l = fl;
-      if (o >= 64) {
-        kk = o / cmom;
-        l = o - kk * cmom;
-      }
+    for (int o = lane; o < nbuf * pairs; o += 64) {
+      const int kk = o / pairs, l0 = 2 * (o - kk * pairs);
       const int cell = cix[kk];
       if (cell < 0) continue;
       const double *v = pwb + kk * nang;
-      const double *e = ect + l;
-      double sum = 0.0;
-      int b = 0;
-      if constexpr (EXACT) {
-        for (; b + 4 <= nang; b += 4) {
-          const double v0 = v[b], v1 = v[b + 1], v2 = v[b + 2], v3 = v[b + 3];
-          const double e0 = e[b * kEctRow], e1 = e[(b + 1) * kEctRow], e2 = e[(b + 2) * kEctRow],
-                       e3 = e[(b + 3) * kEctRow];
-          sum = madd<EXACT>(e0, v0, sum);
-          sum = madd<EXACT>(e1, v1, sum);
-          sum = madd<EXACT>(e2, v2, sum);
-          sum = madd<EXACT>(e3, v3, sum);
-        }
-      } else {
-        // two independent chains (the order is free without EXACT)
-        double s1 = 0.0;
-        for (; b + 4 <= nang; b += 4) {
-          const double v0 = v[b], v1 = v[b + 1], v2 = v[b + 2], v3 = v[b + 3];
-          const double e0 = e[b * kEctRow], e1 = e[(b + 1) * kEctRow], e2 = e[(b + 2) * kEctRow],
-                       e3 = e[(b + 3) * kEctRow];
-          sum = __builtin_fma(e0, v0, sum);
-          s1 = __builtin_fma(e1, v1, s1);
-          sum = __builtin_fma(e2, v2, sum);
-          s1 = __builtin_fma(e3, v3, s1);
-        }
-        sum += s1;
+      const double *e = ect + l0;  // 16-byte aligned: l0 is even, rows are 128 B
+      double s0 = 0.0, s1 = 0.0;
+#pragma unroll 4
+      for (int b = 0; b < nang; b++) {
+        const double vb = v[b];
+        const double2 ee = *(const double2 *)(e + b * kEctRow);
+        s0 = madd<EXACT>(ee.x, vb, s0);
+        s1 = madd<EXACT>(ee.y, vb, s1);
       }
-      for (; b < nang; b++) sum = madd<EXACT>(e[b * kEctRow], v[b], sum);
-      if constexpr (GEN) {
-        const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
-        double *dst = (l == 0) ? (double *)G.flux.at(px, py, pz, 0)
-                               : (double *)G.fluxm.at(px, py, pz, 0) + (l - 1);
-        atomicAdd(dst, sum);
-      } else {
-        if (l == 0)
-          pfg[cell] = sum;
-        else
-          pmg[(size_t)cell * nm1 + (l - 1)] = sum;
-      }
+      auto store = [&](int l, double sum) {
+        if constexpr (GEN) {
+          const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
+          double *dst = (l == 0) ? (double *)G.flux.at(px, py, pz, 0)
+                                 : (double *)G.fluxm.at(px, py, pz, 0) + (l - 1);
+          atomicAdd(dst, sum);
+        } else {
+          if (l == 0)
+            pfg[cell] = sum;
+          else
+            pmg[(size_t)cell * nm1 + (l - 1)] = sum;
+        }
+      };
+      store(l0, s0);
+      if (l0 + 1 < cmom) store(l0 + 1, s1);
     }
     wave_sync();
     nbuf = 0;
