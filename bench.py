@@ -256,10 +256,16 @@ def main() -> int:
                 b.run()  # warm-up (captures the graph)
                 st, per = b.run()
             times = np.array([x["time_ms"] for x in per])
+            inners = [x["inner_total"] for x in per]
+            outers = [x["outer_total"] for x in per]
             details[f"batch_config4_{prec}"] = {
                 "problems": len(c4), "cells": int(sum(q.ncells for q in c4)),
                 "grind_ns": st["grind_ns"], "solve_s": st["solve_ms"] / 1e3,
                 "converged": st["converged"], "outer_iterations": st["outer_iterations"],
+                "per_problem_inner_min_max": [min(inners), max(inners)],
+                "per_problem_outer_min_max": [min(outers), max(outers)],
+                # a problem's time is read at the end of the (lockstep) outer
+                # iteration in which its outer test passed
                 "time_to_convergence_p50_ms": float(np.percentile(times, 50)),
                 "time_to_convergence_p99_ms": float(np.percentile(times, 99)),
                 "tile": list(st["tile"]), "angles_per_lane": st["angles_per_lane"],

@@ -76,3 +76,41 @@ def test_library_quadrature_is_the_oracles(oracle_mod):
         a = tasks.quadrature(nang, nmom)
         b = oracle_mod.quadrature(nang, nmom)
         assert all(np.array_equal(x, y) for x, y in zip(a, b)), (nang, nmom)
+
+
+# ctypes mirror -> C typedef of include/snap_hip.h
+MIRRORS = {
+    "Accessor1": "snap_accessor1_t", "Accessor2": "snap_accessor2_t",
+    "Accessor3": "snap_accessor3_t", "Rect3": "snap_rect3_t", "SweepArgs": "snap_sweep_args_t",
+    "Problem": "snap_problem_t", "SolveOptions": "snap_solve_options_t",
+    "SolveStats": "snap_solve_stats_t", "IterationRecord": "snap_iteration_record_t",
+    "BatchOptions": "snap_batch_options_t", "ProblemStats": "snap_problem_stats_t",
+    "BatchStats": "snap_batch_stats_t",
+}
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler
+    gives the header's struct (a C program built with gcc prints them)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "snap_hip.h"', "int main(void) {"]
+    for py, c in MIRRORS.items():
+        lines.append(f'  printf("{py} %zu\\n", sizeof({c}));')
+        for f, _ in getattr(_lib, py)._fields_:
+            lines.append(f'  printf("{py}.{f} %zu\\n", offsetof({c}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)],
+                   check=True, capture_output=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                  text=True).stdout.splitlines())
+    for py in MIRRORS:
+        cls = getattr(_lib, py)
+        assert int(got[py]) == ctypes.sizeof(cls), py
+        for f, _ in cls._fields_:
+            assert int(got[f"{py}.{f}"]) == getattr(cls, f).offset, f"{py}.{f}"
