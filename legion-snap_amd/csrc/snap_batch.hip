@@ -281,7 +281,6 @@ struct snap_batch {
 
   // one outer iteration, every step predicated on the device flags
   int outer_iteration() {
-    const int lma[4] = {1, 3, 5, 7};
     const int gb = (count + 255) / 256;
     const int64_t gc = (int64_t)ng * total;
     const int nm1 = cmom - 1;
@@ -293,12 +292,11 @@ struct snap_batch {
                                 cmom > 1 ? q2grpm.as<double>() : nullptr, total, ng, nmat, nmom,
                                 false, stream));
     SNAP_TRY(copy(flux0po, flux0, gc));
-    Acc3 none{};
     for (int inno = 0; inno < iitm; inno++) {
-      SNAP_TRY(launch_inner_source(acc(sxs, nmom, 8), acc(flux0, 1, 8), acc(q2grp0, 1, 8),
-                                   cmom > 1 ? acc(fluxm, nm1, 8) : none,
-                                   cmom > 1 ? acc(q2grpm, nm1, 8) : none, acc(qtot, cmom, 8),
-                                   Extent3{(int)total, 1, 1}, ng, cmom, lma, nmom, stream));
+      SNAP_TRY(launch_inner_source_dense(sxs.as<double>(), flux0.as<double>(), q2grp0.as<double>(),
+                                         cmom > 1 ? fluxm.as<double>() : nullptr,
+                                         cmom > 1 ? q2grpm.as<double>() : nullptr, qtot.as<double>(),
+                                         gc, nmom, stream));
       SNAP_TRY(copy(flux0pi, flux0, gc));
       SNAP_TRY(sweep());
       SNAP_TRY(launch_octant_reduce(part_flux.as<double>(), flux0.as<double>(), gc, stream));

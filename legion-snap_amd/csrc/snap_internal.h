@@ -123,6 +123,11 @@ int launch_outer_source(Acc3 qi, Acc3 flux0, Acc2 slgg, Acc3 qo0, Acc3 mat, Exte
                         hipStream_t st);
 int launch_outer_source_m(Acc3 fluxm, Acc2 slgg, Acc3 qom, Acc3 mat, Extent3 e, int ng,
                           int cmom, const int *lma, int nmom, hipStream_t st);
+// CalcInnerSource over dense [g][cell][...] arrays (cg = ng * cells
+// cell-groups, the Legendre moment counts 1, 3, 5, 7)
+int launch_inner_source_dense(const double *sxs, const double *flux0, const double *q2grp0,
+                              const double *fluxm, const double *q2grpm, double *qtot, int64_t cg,
+                              int nmom, hipStream_t st);
 int launch_inner_source(Acc3 sxs, Acc3 flux0, Acc3 q2grp0, Acc3 fluxm, Acc3 q2grpm, Acc3 qtot,
                         Extent3 e, int ng, int cmom, const int *lma, int nmom, hipStream_t st);
 int launch_convergence(Acc3 flux0, Acc3 flux0p, Extent3 e, int ng, double epsi, int32_t *result,

@@ -164,12 +164,10 @@ struct snap_solver {
   }
 
   int inner_source() {
-    const int lma[4] = {1, 3, 5, 7};
-    Acc3 none{};
-    return launch_inner_source(g_acc(sxs, nmom, 8), g_acc(flux0, 1, 8), g_acc(q2grp0, 1, 8),
-                               cmom > 1 ? g_acc(fluxm, cmom - 1, 8) : none,
-                               cmom > 1 ? g_acc(q2grpm, cmom - 1, 8) : none, g_acc(qtot, cmom, 8),
-                               e, ng, cmom, lma, nmom, stream);
+    return launch_inner_source_dense(sxs.as<double>(), flux0.as<double>(), q2grp0.as<double>(),
+                                     cmom > 1 ? fluxm.as<double>() : nullptr,
+                                     cmom > 1 ? q2grpm.as<double>() : nullptr, qtot.as<double>(),
+                                     (int64_t)ng * cells, nmom, stream);
   }
 
   int test(const DeviceBuffer &prev, double eps, bool *converged) {

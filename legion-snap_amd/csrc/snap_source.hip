@@ -134,6 +134,65 @@ int launch_inner_source(Acc3 sxs, Acc3 flux0, Acc3 q2grp0, Acc3 fluxm, Acc3 q2gr
   return SNAP_OK;
 }
 
+// The same sum over the solver's dense arrays (sxs [cg][nmom], flux0 and
+// q2grp0 [cg], fluxm and q2grpm [cg][cmom-1], qtot [cg][cmom] with
+// cg = g * cells + cell): one thread per qtot element in memory order, the
+// flat index split by the compile-time moment count instead of six 64-bit
+// divisions per element.
+template <int CMOM, typename Index>
+__global__ void k_inner_source_dense(const double *__restrict__ sxs, const double *__restrict__ flux0,
+                                     const double *__restrict__ q2grp0,
+                                     const double *__restrict__ fluxm,
+                                     const double *__restrict__ q2grpm, double *__restrict__ qtot,
+                                     Index n, int nmom, OrderTable ord) {
+  constexpr int NM = CMOM - 1;
+  for (Index t = (Index)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (Index)gridDim.x * blockDim.x) {
+    const Index cg = t / CMOM;
+    const int m = (int)(t - cg * CMOM);
+    const double *sx = sxs + cg * nmom;
+    double v;
+    if (m == 0)
+      v = q2grp0[cg] + flux0[cg] * sx[0];
+    else
+      v = q2grpm[cg * NM + (m - 1)] + fluxm[cg * NM + (m - 1)] * sx[ord.order[m]];
+    qtot[t] = v;
+  }
+}
+
+template <int CMOM>
+static void enqueue_inner_dense(const double *sxs, const double *flux0, const double *q2grp0,
+                                const double *fluxm, const double *q2grpm, double *qtot, int64_t n,
+                                int nmom, const OrderTable &ord, hipStream_t st) {
+  const dim3 grid(grid_for(n, 256)), block(256);
+  if (n <= (int64_t)UINT32_MAX - 65536 * 8 * 256)
+    hipLaunchKernelGGL((k_inner_source_dense<CMOM, uint32_t>), grid, block, 0, st, sxs, flux0,
+                       q2grp0, fluxm, q2grpm, qtot, (uint32_t)n, nmom, ord);
+  else
+    hipLaunchKernelGGL((k_inner_source_dense<CMOM, int64_t>), grid, block, 0, st, sxs, flux0,
+                       q2grp0, fluxm, q2grpm, qtot, n, nmom, ord);
+}
+
+int launch_inner_source_dense(const double *sxs, const double *flux0, const double *q2grp0,
+                              const double *fluxm, const double *q2grpm, double *qtot, int64_t cg,
+                              int nmom, hipStream_t st) {
+  const int cmom = nmom * nmom;
+  if (nmom < 1 || nmom > 4) return fail(SNAP_ERR_INVALID, "nmom %d outside 1..4", nmom);
+  if (cmom > 1 && (!fluxm || !q2grpm)) return fail(SNAP_ERR_INVALID, "moment arrays missing");
+  const int lma[4] = {1, 3, 5, 7};
+  const OrderTable ord = make_order_table(lma, nmom);
+  const int64_t n = cg * cmom;
+  if (n <= 0) return SNAP_OK;
+  switch (cmom) {
+    case 1: enqueue_inner_dense<1>(sxs, flux0, q2grp0, fluxm, q2grpm, qtot, n, nmom, ord, st); break;
+    case 4: enqueue_inner_dense<4>(sxs, flux0, q2grp0, fluxm, q2grpm, qtot, n, nmom, ord, st); break;
+    case 9: enqueue_inner_dense<9>(sxs, flux0, q2grp0, fluxm, q2grpm, qtot, n, nmom, ord, st); break;
+    default: enqueue_inner_dense<16>(sxs, flux0, q2grp0, fluxm, q2grpm, qtot, n, nmom, ord, st); break;
+  }
+  SNAP_HIP_TRY(hipGetLastError());
+  return SNAP_OK;
+}
+
 // TestInnerConvergence / TestOuterConvergence (inner.cc:283-324,
 // outer.cc:354-403; GPU gpu_inner.cu:119-204, gpu_outer.cu:443-531):
 // a cell fails when |flux0/flux0p - 1| > epsi (|flux0| when |flux0p| < 1e-12).

@@ -92,5 +92,5 @@ def test_torch_autograd_harness(gpu):
         s.run()
         g, R2, _ = s.gradient(r)
         f, _ = s.flux()
-    assert np.isclose(float(R), float(np.sum(r * f)), rtol=1e-12)
+    assert np.isclose(float(R.detach()), float(np.sum(r * f)), rtol=1e-12)
     assert np.allclose(siga.grad.numpy(), g.sum(axis=0), rtol=1e-10, atol=0)
