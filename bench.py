@@ -52,6 +52,39 @@ def bytes_per_unit(nang: int, cmom: int, real_bytes: int) -> float:
     return real_bytes + (24.0 * cmom + 8.0) / nang
 
 
+CPU_INFO = {}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def copy_bandwidth_gbs(torch, gib: int = 2, reps: int = 5) -> float:
+    """Device-to-device copy rate (bytes read + written per second, best of
+    reps) of a gib-GiB buffer: the HBM rate a streaming kernel reaches here."""
+    n = (gib << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = float("inf")
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(src)
+        b.record()
+        b.synchronize()
+        best = min(best, a.elapsed_time(b) * 1e-3)
+    del src, dst
+    return 2 * n * 8 / best / 1e9
+
+
 def cpu_baseline(problem, budget_s: float = 25.0):
     """Time the CPU oracle on one inner iteration's sweeps (all octants and
     groups) of `problem` using every host core, or of a reduced group count
@@ -72,6 +105,11 @@ def cpu_baseline(problem, budget_s: float = 25.0):
                                  qi=problem.qi[:ng])
     t = oracle.time_sweeps(sample, 1, cores)
     units = sample.ncells * 8 * sample.nang * ng
+    # the same sweeps on one core (the two-group probe), and the host CPU
+    t1 = oracle.time_sweeps(probe, 1, 1)
+    CPU_INFO["one_core_grind_ns"] = t1 * 1e9 / (probe.ncells * 8 * probe.nang * probe.ng)
+    CPU_INFO["threads"] = cores
+    CPU_INFO["model"] = cpu_model()
     return {
         "value": t * 1e9 / units,
         "unit": "ns/(cell·angle·group)",
@@ -275,6 +313,13 @@ def main() -> int:
     if not args.no_cpu:
         cpu = cpu_baseline(p)
         details["fp64_speedup_vs_cpu"] = cpu["value"] / grind
+        details["cpu"] = dict(CPU_INFO)
+    # the sweep's algorithmic rate against a measured copy as well as the
+    # published peak
+    if not args.no_extra:
+        copy_gbs = copy_bandwidth_gbs(torch)
+        details["hbm_copy_GBps"] = copy_gbs
+        details["roofline_frac_of_copy"] = achieved / copy_gbs
 
     line = {
         "metric": METRIC,

@@ -467,12 +467,7 @@ int adjoint_enqueue(const SweepPlan &p, const QuadDev &quad, int ng, int fixup, 
   if (!k) return fail(SNAP_ERR_INVALID, "no adjoint kernel for %d cells per wave", cpw);
   const size_t lds = adj_lds(p.TX, p.TY, p.TZ, p.nang, waves, sb, A.ncells_tile, p.ncellplanes,
                              fp32 ? 4 : 8).total;
-  {
-    static std::mutex mu;
-    std::lock_guard<std::mutex> lock(mu);
-    SNAP_HIP_TRY(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds));
-  }
+  SNAP_TRY(ensure_dynamic_lds((const void *)k, lds));
   for (int D = p.nplanes - 1; D >= 0; D--) {
     const int count = p.plane_off[D + 1] - p.plane_off[D];
     hipLaunchKernelGGL(k, dim3(count, 8 * ng), dim3(64 * waves), lds, st, A, p.plane_off[D]);

@@ -4,6 +4,9 @@
 #include <cstring>
 
 #include <algorithm>
+#include <mutex>
+#include <tuple>
+#include <vector>
 
 #include "snap_internal.h"
 
@@ -16,6 +19,27 @@ void set_error(const char *fmt, ...) {
   va_start(ap, fmt);
   vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
   va_end(ap);
+}
+
+// Dynamic LDS above 64 KiB must be requested per kernel and per device (the
+// attribute belongs to the kernel's code object on the current device); a
+// process may drive several devices, so requests are kept per (device, kernel).
+int ensure_dynamic_lds(const void *kernel, size_t bytes) {
+  int dev = 0;
+  SNAP_HIP_TRY(hipGetDevice(&dev));
+  static std::mutex mu;
+  static std::vector<std::tuple<int, const void *, size_t>> done;
+  std::lock_guard<std::mutex> lock(mu);
+  for (auto &d : done)
+    if (std::get<0>(d) == dev && std::get<1>(d) == kernel && std::get<2>(d) >= bytes) return SNAP_OK;
+  SNAP_HIP_TRY(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  for (auto &d : done)
+    if (std::get<0>(d) == dev && std::get<1>(d) == kernel) {
+      std::get<2>(d) = bytes;
+      return SNAP_OK;
+    }
+  done.emplace_back(dev, kernel, bytes);
+  return SNAP_OK;
 }
 
 int fail(int code, const char *fmt, ...) {

@@ -16,6 +16,9 @@ namespace snap {
 // ---------------------------------------------------------------------------
 void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+// raise a kernel's dynamic LDS limit to bytes on the current device (once
+// per device, kernel and size)
+int ensure_dynamic_lds(const void *kernel, size_t bytes);
 
 #define SNAP_HIP_TRY(expr)                                                     \
   do {                                                                         \

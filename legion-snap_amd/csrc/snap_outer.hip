@@ -231,12 +231,7 @@ int launch_outer_dense(const double *qi, const double *flux0, const double *flux
   const Orders ord = orders_for(nmom);
   if (!ordered && ng <= 4 * kMaxKSteps && outer_mfma_lds(ng, cmom) <= 160 * 1024) {
     const size_t lds = outer_mfma_lds(ng, cmom);
-    static size_t lds_set = 0;  // dynamic LDS above 64 KiB is requested per kernel
-    if (lds > lds_set) {
-      SNAP_HIP_TRY(hipFuncSetAttribute((const void *)k_outer_mfma,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      lds_set = lds;
-    }
+    SNAP_TRY(ensure_dynamic_lds((const void *)k_outer_mfma, lds));
     const int64_t chunks = (cells + kChunk - 1) / kChunk;
     if (chunks > INT32_MAX) return fail(SNAP_ERR_UNSUPPORTED, "too many cells");
     const unsigned mblocks = (unsigned)((cmom + kMomBlock - 1) / kMomBlock);

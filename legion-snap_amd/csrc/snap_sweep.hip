@@ -1524,16 +1524,7 @@ static SweepKernel kernel_for_wide(int apl, bool gen, bool td) {
 }
 
 static int ensure_lds_attr(SweepKernel k, size_t bytes) {
-  // dynamic LDS above 64 KiB must be requested per kernel
-  static std::mutex mu;
-  static std::vector<std::pair<SweepKernel, size_t>> done;
-  std::lock_guard<std::mutex> lock(mu);
-  for (auto &d : done)
-    if (d.first == k && d.second >= bytes) return SNAP_OK;
-  SNAP_HIP_TRY(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)bytes));
-  done.emplace_back(k, bytes);
-  return SNAP_OK;
+  return ensure_dynamic_lds((const void *)k, bytes);
 }
 
 int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
@@ -1687,6 +1678,9 @@ int current_device(int *dev) {
 }
 
 void context_release(DeviceContext &c) {
+  // sweeps already enqueued on any stream may still read the tables and
+  // face workspaces freed here
+  if (c.valid || !c.work.empty()) (void)hipDeviceSynchronize();
   quad_free(&c.quad);
   for (auto &pp : c.plans) {
     plan_free(pp.second);
