@@ -140,7 +140,7 @@ int launch_convergence(Acc3 flux0, Acc3 flux0p, Extent3 e, int ng, double epsi, 
 int launch_outer_dense(const double *qi, const double *flux0, const double *fluxm,
                        const double *slgg, const int32_t *mat, double *q0, double *qm,
                        int64_t cells, int ng, int nmat, int nmom, bool ordered, hipStream_t st);
-size_t outer_mfma_lds(int ng, int cmom);
+size_t outer_mfma_lds(int ng, int cmom, int sgt);
 // zeta_new = r + S^T u and norms[0] = max|zeta_new - zeta|, norms[1] = max|zeta_new|
 // (as the bit patterns of the non-negative doubles)
 int launch_adjoint_update(const double *r, const double *u, const double *slgg,

@@ -127,6 +127,20 @@ def test_outer_source_matrix_cores(gpu, oracle_mod, name):
         assert np.max(np.abs(fm2 - fm1)) <= 1e-13 * np.max(np.abs(fm1))
 
 
+@pytest.mark.parametrize("ng", [40, 56])
+def test_outer_source_matrix_cores_many_groups(gpu, ng):
+    # P3 with 40 groups stages its outputs in LDS over two passes of 16-group
+    # blocks; with 56 groups no block fits beside the fluxes and every lane
+    # stores directly.  Both against the CPU-ordered outer source.
+    p = problems.config1(n=6, ng=ng, nang=4, nmom=4)
+    f1, fm1, st1 = gpu_solve(p, "fp64", outer_ordered=True)
+    f2, fm2, st2 = gpu_solve(problems.config1(n=6, ng=ng, nang=4, nmom=4), "fp64")
+    assert (st1["inner_total"], st1["outer_total"]) == (st2["inner_total"], st2["outer_total"])
+    mx, _ = rel_errors(f2, f1)
+    assert mx <= 1e-13
+    assert np.max(np.abs(fm2 - fm1)) <= 1e-13 * np.max(np.abs(fm1))
+
+
 def test_dinv_modes_agree(gpu):
     # forming dinv in the kernel is the same arithmetic as the geometry kernel
     p = CASES["config1_small"]()
