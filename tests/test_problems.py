@@ -84,3 +84,25 @@ def test_validate_rejects_bad_tables():
     p.epsi = 0.5
     with pytest.raises(ValueError):
         p.validate()
+
+
+def test_bench_algorithmic_bytes_per_unit():
+    """The roofline's algorithmic bytes per cell x direction x group of the
+    reference data flow: dinv streamed once per octant (8 B, 4 B in FP32),
+    plus the source moments, t_xs and the flux-moment read-modify-write
+    (3*8*cmom + 8 B per cell, group and octant) shared by the octant's
+    angles -- 11.2, 18.89 and 9.33 B for the 16^3 P0 10-angle, the 64^3 P3
+    36-angle and the 48^3 P0 24-angle workloads."""
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location(
+        "bench", os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    assert math.isclose(bench.bytes_per_unit(10, 1, 8), 11.2)
+    assert math.isclose(bench.bytes_per_unit(36, 16, 8), 18.0 + 8.0 / 9.0)
+    assert math.isclose(bench.bytes_per_unit(24, 1, 8), 8.0 + 32.0 / 24.0)
+    assert math.isclose(bench.bytes_per_unit(36, 16, 4), 14.0 + 8.0 / 9.0)
+    # units of one sweep of every octant and group
+    p = problems.config1()
+    assert p.units_per_sweep == 64 ** 3 * 8 * 36 * 32
