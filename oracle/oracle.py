@@ -1,6 +1,6 @@
 """ctypes front end of the CPU oracle (libsnap_oracle.so).
 
-TEST INFRASTRUCTURE ONLY: the oracle is the checker for the HIP library.
+TEST INFRASTRUCTURE ONLY: the HIP library's results are tested against it.
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
 Each wrapper names the reference task body its C function restates.
 """

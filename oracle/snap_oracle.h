@@ -2,9 +2,10 @@
  * snap_oracle.h -- CPU oracle for the Legion-SNAP mini-KBA steady-state solve.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a Legion-free FP64 restatement of the
- * reference's CPU task bodies and of its transport_solve control flow.  It is
- * the checker for the HIP library: only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg may load it.  The product path never calls it.
+ * reference's CPU task bodies and of its transport_solve control flow, which
+ * the HIP library's results are tested against: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product path never calls it.
  *
  * Data layout (dense, x fastest, as the reference's SOA layout
  * snap.cc:1456-1475 with one field per energy group):
