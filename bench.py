@@ -180,7 +180,7 @@ def main() -> int:
                          "FETCH_SIZE / WRITE_SIZE traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for {args.gpus}).",
               file=sys.stderr)
         return 2
 
