@@ -367,6 +367,9 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
     return fail(SNAP_ERR_INVALID, "batch precision must be FP64 or MIXED");
   if (b->nmom < 1 || b->nmom > 4 || b->ng < 1 || b->nang < 1)
     return fail(SNAP_ERR_INVALID, "bad ng / nang / nmom");
+  if (b->nang > 128)
+    return fail(SNAP_ERR_UNSUPPORTED,
+                "batches use the tiled sweep: at most 128 angles per octant, got %d", b->nang);
   b->cell_off.assign(count + 1, 0);
   int nmat_all = 0;
   for (int p = 0; p < count; p++) {

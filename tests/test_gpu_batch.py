@@ -45,3 +45,26 @@ def test_batch_with_fixed_tile(gpu):
         s.run()
         f, _ = s.flux()
     assert np.array_equal(f0, f)
+
+
+def test_batch_two_angles_per_lane_p3(gpu):
+    # 80 angles need two angles per lane in FP64 as well; P3 moments
+    probs = problems.config4(count=3, seed=3, ng=3, nang=80, nmom=4, edges=(6, 8, 10))
+    with Batch(probs, precision="fp64") as b:
+        st, per = b.run()
+        assert st["angles_per_lane"] == 2
+        fluxes = [b.flux(i) for i in range(len(probs))]
+    for i, p in enumerate(probs):
+        with Solver(p, precision="fp64") as s:
+            ss = s.run()
+            f, fm = s.flux()
+        assert (per[i]["inner_total"], per[i]["outer_total"]) == (ss["inner_total"], ss["outer_total"])
+        assert np.array_equal(fluxes[i][0], f) and np.array_equal(fluxes[i][1], fm)
+
+
+def test_batch_rejects_more_than_128_angles(gpu):
+    from snap_amd import _lib
+    probs = problems.config4(count=2, seed=1, ng=2, nang=130, nmom=1, edges=(4, 6))
+    with pytest.raises(_lib.SnapError) as e:
+        Batch(probs, precision="fp64")
+    assert e.value.code == _lib.SNAP_ERR_UNSUPPORTED
