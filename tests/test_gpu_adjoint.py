@@ -94,3 +94,19 @@ def test_torch_autograd_harness(gpu):
         f, _ = s.flux()
     assert np.isclose(float(R.detach()), float(np.sum(r * f)), rtol=1e-12)
     assert np.allclose(siga.grad.numpy(), g.sum(axis=0), rtol=1e-10, atol=0)
+
+
+def test_gradient_refuses_what_it_does_not_cover(gpu):
+    # the adjoint is of the steady P0 solve with the tiled one-angle-per-lane
+    # sweep: P1 scattering, more than 64 angles and time dependence are refused
+    from snap_amd import _lib
+    cases = [problems.config1(n=6, ng=2, nang=8, nmom=2),
+             problems.config0(n=6, ng=2, nang=80),
+             dataclasses.replace(problems.config0(n=6, ng=2, nang=8), timedep=True, nsteps=1)]
+    for p in cases:
+        r = np.ones((p.ng, p.nz, p.ny, p.nx))
+        with Solver(p, precision="fp64") as s:
+            s.run()
+            with pytest.raises(_lib.SnapError) as e:
+                s.gradient(r)
+            assert e.value.code == _lib.SNAP_ERR_UNSUPPORTED
