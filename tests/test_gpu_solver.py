@@ -141,6 +141,20 @@ def test_outer_source_matrix_cores_many_groups(gpu, ng):
     assert np.max(np.abs(fm2 - fm1)) <= 1e-13 * np.max(np.abs(fm1))
 
 
+@pytest.mark.parametrize("kw", [{"first_sweep_fp64": True}, {"dinv": "recompute"}])
+def test_mixed_options(gpu, oracle_mod, kw):
+    # the first inner's sweep in FP64, or dinv formed in the FP32 sweep from
+    # t_xs: same iteration counts, errors within the mixed-precision bar
+    fo, _, sto = oracle_solution(oracle_mod, "config1_small")
+    f, _, st = gpu_solve(CASES["config1_small"](), "mixed", **kw)
+    mx, l2 = rel_errors(f, fo)
+    print(kw, f"maxrel={mx:.3e} l2rel={l2:.3e}", st)
+    assert (st["inner_total"], st["outer_total"]) == (sto["inner_total"], sto["outer_total"])
+    if kw.get("first_sweep_fp64"):
+        assert (st["sweeps_fp64"], st["sweeps_fp32"]) == (1, st["inner_total"] - 1)
+    assert mx <= 5e-6 and l2 <= 1e-6
+
+
 def test_dinv_modes_agree(gpu):
     # forming dinv in the kernel is the same arithmetic as the geometry kernel
     p = CASES["config1_small"]()
