@@ -47,6 +47,12 @@ def test_expand_and_geometry(setup, oracle_mod):
     assert np.array_equal(sxs.cpu().numpy()[..., :nmom], sx_o)
     mu, eta, xi, w, ec = S["quad"]
     assert np.array_equal(dinv.cpu().numpy(), oracle_mod.geometry_param(xs_o, nang, S["h"], mu, eta, xi))
+    # the time-dependent form adds vdelt_g to the denominator (expxs.cc:378-381)
+    vd = np.linspace(0.5, 2.5, ng)
+    tasks.calculate_geometry_param(xs, dinv, S["h"], vdelts=vd)
+    torch.cuda.synchronize()
+    assert np.array_equal(dinv.cpu().numpy(),
+                          oracle_mod.geometry_param(xs_o, nang, S["h"], mu, eta, xi, vdelt=vd))
 
 
 def test_outer_and_inner_source(setup, oracle_mod):
