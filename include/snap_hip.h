@@ -118,6 +118,7 @@ typedef struct {
  * SIMULTANEOUS read-write, sweep.cc:54-55, gpu_sweep.cu:1705). */
 int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream_t stream);
 
+
 /* Replaces run_flux0_outer_source (decl outer.cc:220-225, def
  * gpu_outer.cu:104-223): qo0[g] = qi0[g] + sum_{g'!=g} slgg[g][mat][g'].l0 * flux0[g'].
  * slgg[g] is a 2-D accessor at point (material, g') of nmom doubles. */
@@ -322,9 +323,11 @@ int snap_hip_mapper_tunables(int num_groups, int max_inner_iters, int processors
  * (may be NULL) to host memory. */
 int snap_hip_solver_get_flux(snap_solver_t *solver, double *flux0, double *fluxm);
 /* Time `count` inner sweeps (all octants and groups) in the given precision
- * on the source left by the last run, after one untimed sweep.  Writes the mean GPU time of one inner
- * sweep and the mean duration of one launch of the sweep kernel.  MIXED needs
- * a solver created with precision MIXED. */
+ * on the source left by the last run, after one untimed sweep.  Writes the
+ * mean GPU time of one inner sweep, including the sum of its octant partials
+ * into the flux moments (which the reference does inside its sweep kernel),
+ * and the mean duration of one launch of the sweep kernel.  MIXED needs a
+ * solver created with precision MIXED. */
 int snap_hip_solver_time_sweeps(snap_solver_t *solver, int count, int precision,
                                 double *ms_per_sweep, double *ms_per_launch,
                                 int *launches_per_sweep);

@@ -403,8 +403,10 @@ extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) 
         SNAP_TRY(s->sweep(fp32 ? SNAP_PRECISION_MIXED
                                : (mixed ? SNAP_PRECISION_FP64 : s->opt.precision),
                           nullptr, !(refine && fp32)));
-        SNAP_HIP_TRY(hipEventRecord(s->ev[1], stream));
+        // the octant sum is part of the sweep: the reference accumulates the
+        // flux moments inside its sweep kernel (gpu_sweep.cu:1680-1733)
         SNAP_TRY(s->reduce_octants(refine && fp32));
+        SNAP_HIP_TRY(hipEventRecord(s->ev[1], stream));
         if (refine && !fp32) {  // keep the FP64 base of this outer
           SNAP_TRY(s->copy(s->qbase, s->qtot, sizeof(double) * gc * s->cmom));
           SNAP_TRY(s->copy(s->flux0base, s->flux0, sizeof(double) * gc));
@@ -513,7 +515,10 @@ extern "C" int snap_hip_solver_time_sweeps(snap_solver_t *s, int count, int prec
   for (auto &x : evs) SNAP_HIP_TRY(hipEventCreate(&x));
   SNAP_TRY(s->sweep(precision, nullptr));  // first launches load the code object
   SNAP_HIP_TRY(hipEventRecord(s->ev[0], s->stream));
-  for (int i = 0; i < count; i++) SNAP_TRY(s->sweep(precision, evs.data() + (size_t)2 * np * i));
+  for (int i = 0; i < count; i++) {
+    SNAP_TRY(s->sweep(precision, evs.data() + (size_t)2 * np * i));
+    SNAP_TRY(s->reduce_octants());
+  }
   SNAP_HIP_TRY(hipEventRecord(s->ev[1], s->stream));
   SNAP_HIP_TRY(hipEventSynchronize(s->ev[1]));
   float total = 0.f;

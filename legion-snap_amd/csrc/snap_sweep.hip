@@ -31,6 +31,7 @@
 // and faces in float and the moment sums in double.
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <type_traits>
 
@@ -201,9 +202,13 @@ void plan_free(SweepPlan *p) {
   *p = SweepPlan();
 }
 
-int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int TZ, int waves,
-                int apl, bool td) {
-  plan_free(p);
+// Host part of plan_create: every field of the plan and its three tables
+// (tiles by tile diagonal, cell-diagonal offsets and cells inside one tile),
+// without touching device memory.
+int plan_layout(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int TZ, int waves,
+                int apl, bool td, std::vector<int> &tiles, std::vector<int> &coff,
+                std::vector<int16_t> &cells) {
+  *p = SweepPlan();
   if (e.nx < 1 || e.ny < 1 || e.nz < 1) return fail(SNAP_ERR_INVALID, "empty chunk");
   if (nang < 1 || nang > 128)
     return fail(SNAP_ERR_UNSUPPORTED, "the tiled sweep handles 1..128 angles per octant, got %d",
@@ -243,7 +248,7 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
   }
   // tile diagonals in sweep order
   p->nplanes = p->nTx + p->nTy + p->nTz - 2;
-  std::vector<int> tiles;
+  tiles.clear();
   p->plane_off.assign(p->nplanes + 1, 0);
   for (int D = 0; D < p->nplanes; D++) {
     p->plane_off[D] = (int)tiles.size();
@@ -257,8 +262,8 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
 
   // cell diagonals inside a full tile
   p->ncellplanes = TX + TY + TZ - 2;
-  std::vector<int16_t> cells;
-  std::vector<int> coff(p->ncellplanes + 1, 0);
+  cells.clear();
+  coff.assign(p->ncellplanes + 1, 0);
   p->maxdiag = 0;
   for (int d = 0; d < p->ncellplanes; d++) {
     coff[d] = (int)cells.size();
@@ -285,7 +290,15 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
   if (p->lds_bytes(8, false) > 160 * 1024)
     return fail(SNAP_ERR_INVALID, "tile %dx%dx%d with %d angles needs %zu B of LDS (> 160 KiB)",
                 TX, TY, TZ, nang, p->lds_bytes(8, false));
+  return SNAP_OK;
+}
 
+int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int TZ, int waves,
+                int apl, bool td) {
+  plan_free(p);
+  std::vector<int> tiles, coff;
+  std::vector<int16_t> cells;
+  SNAP_TRY(plan_layout(p, e, nang, cmom, TX, TY, TZ, waves, apl, td, tiles, coff, cells));
   SNAP_HIP_TRY(hipMalloc((void **)&p->plane_tiles, sizeof(int) * tiles.size()));
   SNAP_HIP_TRY(hipMalloc((void **)&p->cell_off, sizeof(int) * coff.size()));
   SNAP_HIP_TRY(hipMalloc((void **)&p->cells, sizeof(int16_t) * cells.size()));
@@ -301,8 +314,9 @@ int plan_create(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int
 // Most angles the one-cell workgroup holds: 1024 lanes with two angles each.
 constexpr int kWideMaxAngles = 2048;
 
-int plan_create_wide(SweepPlan *p, Extent3 e, int nang, int cmom, bool td) {
-  plan_free(p);
+int plan_layout_wide(SweepPlan *p, Extent3 e, int nang, int cmom, bool td,
+                     std::vector<int> &cells) {
+  *p = SweepPlan();
   if (e.nx < 1 || e.ny < 1 || e.nz < 1) return fail(SNAP_ERR_INVALID, "empty chunk");
   if (nang < 1 || nang > kWideMaxAngles)
     return fail(SNAP_ERR_UNSUPPORTED, "the sweep handles 1..%d angles per octant, got %d",
@@ -330,7 +344,7 @@ int plan_create_wide(SweepPlan *p, Extent3 e, int nang, int cmom, bool td) {
   p->maxdiag = 0;
   p->ncellplanes = 1;
   p->nplanes = e.nx + e.ny + e.nz - 2;
-  std::vector<int> cells;
+  cells.clear();
   p->plane_off.assign(p->nplanes + 1, 0);
   for (int D = 0; D < p->nplanes; D++) {
     p->plane_off[D] = (int)cells.size();
@@ -342,6 +356,13 @@ int plan_create_wide(SweepPlan *p, Extent3 e, int nang, int cmom, bool td) {
     p->maxdiag = std::max(p->maxdiag, (int)cells.size() - p->plane_off[D]);
   }
   p->plane_off[p->nplanes] = (int)cells.size();
+  return SNAP_OK;
+}
+
+int plan_create_wide(SweepPlan *p, Extent3 e, int nang, int cmom, bool td) {
+  plan_free(p);
+  std::vector<int> cells;
+  SNAP_TRY(plan_layout_wide(p, e, nang, cmom, td, cells));
   SNAP_HIP_TRY(hipMalloc((void **)&p->plane_tiles, sizeof(int) * cells.size()));
   SNAP_HIP_TRY(hipMemcpy(p->plane_tiles, cells.data(), sizeof(int) * cells.size(),
                          hipMemcpyHostToDevice));
@@ -1655,17 +1676,23 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
 // the drop-in context (initialize_gpu_context) and run_gpu_sweep
 // ---------------------------------------------------------------------------
 namespace {
+// Face workspace of one stream.  Its own lock orders a growth (which waits
+// for the stream's earlier sweeps before freeing) against enqueues from other
+// threads on the same stream; sweeps on different streams never share it.
 struct StreamWork {
-  hipStream_t stream;
-  void *faces;
-  size_t bytes;
+  hipStream_t stream = nullptr;
+  void *faces = nullptr;
+  size_t bytes = 0;
+  std::mutex lock;
 };
 struct DeviceContext {
   bool valid = false;
   QuadDev quad;
   Extent3 chunk{};
+  // entries are never moved or freed until the context is released, so
+  // pointers to them stay valid outside the context lock
   std::vector<std::pair<Extent3, SweepPlan *>> plans;
-  std::vector<StreamWork> work;
+  std::vector<std::unique_ptr<StreamWork>> work;
 };
 constexpr int kMaxDevices = 64;
 DeviceContext g_ctx[kMaxDevices];
@@ -1688,7 +1715,7 @@ void context_release(DeviceContext &c) {
   }
   c.plans.clear();
   for (auto &w : c.work)
-    if (w.faces) (void)hipFree(w.faces);
+    if (w->faces) (void)hipFree(w->faces);
   c.work.clear();
   c.valid = false;
 }
@@ -1725,67 +1752,27 @@ extern "C" int snap_hip_initialize_gpu_context(const double *ec, const double *m
   return SNAP_OK;
 }
 
-extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream_t stream) {
-  if (!args) return fail(SNAP_ERR_INVALID, "NULL sweep arguments");
-  const snap_sweep_args_t &a = *args;
+namespace snap {
+namespace {
+
+int check_sweep_args(const snap_sweep_args_t &a) {
   if (a.mms_source && !a.qim.base) return fail(SNAP_ERR_INVALID, "mms_source set without qim");
   if (a.vdelt != 0.0 && (!a.time_flux_in.base || !a.time_flux_out.base))
     return fail(SNAP_ERR_INVALID, "vdelt != 0 needs time_flux_in and time_flux_out");
   if (a.stride_x_positive != ((a.corner & 1) != 0) || a.stride_y_positive != ((a.corner & 2) != 0) ||
       a.stride_z_positive != ((a.corner & 4) != 0) || a.corner < 0 || a.corner > 7)
     return fail(SNAP_ERR_INVALID, "corner %d does not match the stride directions", a.corner);
-  int dev;
-  SNAP_TRY(current_device(&dev));
-  std::lock_guard<std::mutex> lock(g_ctx_mutex);
-  DeviceContext &c = g_ctx[dev];
-  if (!c.valid) return fail(SNAP_ERR_NO_CONTEXT, "initialize_gpu_context was not called on device %d", dev);
-  if (a.num_angles != c.quad.nang || a.num_moments != c.quad.nmom)
-    return fail(SNAP_ERR_INVALID, "angles/moments differ from the context");
+  if (a.x_range < 1 || a.y_range < 1 || a.z_range < 1) return fail(SNAP_ERR_INVALID, "empty chunk");
+  return SNAP_OK;
+}
+
+// The launch of one drop-in sweep on `plan` with the given face workspace,
+// from the reference's run_gpu_sweep arguments (accessors relative to the
+// chunk origin, ghost points absolute).
+int dropin_launch(const snap_sweep_args_t &a, const SweepPlan *plan, const QuadDev *quad,
+                  void *faces, hipStream_t st) {
   const Extent3 e{a.x_range, a.y_range, a.z_range};
-  if (e.nx < 1 || e.ny < 1 || e.nz < 1) return fail(SNAP_ERR_INVALID, "empty chunk");
-  // plan per chunk shape, one for steady and one for time-dependent sweeps
   const bool td = a.vdelt != 0.0 || a.mms_source;
-  SweepPlan *plan = nullptr;
-  for (auto &pp : c.plans)
-    if (pp.first.nx == e.nx && pp.first.ny == e.ny && pp.first.nz == e.nz && pp.second->td == td)
-      plan = pp.second;
-  if (!plan) {
-    int TX = 0, TY = 0, TZ = 0, waves = 0, apl = 0;
-    plan = new SweepPlan();
-    int rc;
-    if (c.quad.nang > 128) {  // past two angles per lane of a wavefront
-      rc = plan_create_wide(plan, e, c.quad.nang, c.quad.cmom, td);
-    } else {
-      plan_choose(e, c.quad.nang, c.quad.cmom, 8, &TX, &TY, &TZ, &waves, &apl, td);
-      rc = plan_create(plan, e, c.quad.nang, c.quad.cmom, TX, TY, TZ, waves, apl, td);
-    }
-    if (rc != SNAP_OK) {
-      delete plan;
-      return rc;
-    }
-    c.plans.emplace_back(e, plan);
-  }
-  // per-stream face workspace (gpu_sweep.cu keeps 12 per-stream scratch slots)
-  const size_t need = plan->face_doubles_per_sweep() * sizeof(double);
-  hipStream_t st = (hipStream_t)stream;
-  StreamWork *wk = nullptr;
-  for (auto &w : c.work)
-    if (w.stream == st) wk = &w;
-  if (!wk) {
-    c.work.push_back(StreamWork{st, nullptr, 0});
-    wk = &c.work.back();
-  }
-  if (wk->bytes < need) {
-    if (wk->faces) {
-      SNAP_HIP_TRY(hipStreamSynchronize(st));
-      SNAP_HIP_TRY(hipFree(wk->faces));
-      wk->faces = nullptr;
-      wk->bytes = 0;
-    }
-    if (hipMalloc(&wk->faces, need) != hipSuccess)
-      return fail(SNAP_ERR_NO_MEMORY, "face workspace of %zu bytes", need);
-    wk->bytes = need;
-  }
   snap_rect3_t b;
   for (int i = 0; i < 3; i++) b.lo[i] = a.origin[i];
   b.hi[0] = a.origin[0] + e.nx - 1;
@@ -1794,7 +1781,7 @@ extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream
   SweepLaunch L;
   memset(&L, 0, sizeof(L));
   L.plan = plan;
-  L.quad = &c.quad;
+  L.quad = quad;
   L.nsweeps = 1;
   L.ng_job = 1;
   L.corner0 = a.corner;
@@ -1806,8 +1793,8 @@ extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream
   SNAP_TRY(single_acc3(a.dinv, b, &L.dinv, "dinv"));
   SNAP_TRY(single_acc3(a.t_xs, b, &L.txs, "t_xs"));
   SNAP_TRY(single_acc3(a.flux, b, &L.flux, "flux"));
-  SNAP_TRY(single_acc3(a.fluxm, b, &L.fluxm, "fluxm", c.quad.cmom == 1));
-  if (c.quad.cmom > 1 && !L.fluxm.base) return fail(SNAP_ERR_INVALID, "fluxm is required when cmom > 1");
+  SNAP_TRY(single_acc3(a.fluxm, b, &L.fluxm, "fluxm", quad->cmom == 1));
+  if (quad->cmom > 1 && !L.fluxm.base) return fail(SNAP_ERR_INVALID, "fluxm is required when cmom > 1");
   L.general = true;
   // ghost points are absolute (y,z) / (x,z) / (x,y): fold the origin in
   auto ghost = [](const snap_accessor2_t &gacc, int64_t o0, int64_t o1) {
@@ -1821,7 +1808,7 @@ extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream
   L.gx = ghost(a.ghostx, a.origin[1], a.origin[2]);
   L.gy = ghost(a.ghosty, a.origin[0], a.origin[2]);
   L.gz = ghost(a.ghostz, a.origin[0], a.origin[1]);
-  L.faces = wk->faces;
+  L.faces = faces;
   L.fp32 = false;
   L.exact = true;
   // time-dependent terms and the MMS source (gpu_sweep.cu:2102-2140 picks the
@@ -1836,4 +1823,76 @@ extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream
     if (a.mms_source) SNAP_TRY(single_acc3(a.qim, b, &L.qim_acc, "qim"));
   }
   return sweep_enqueue(L, st);
+}
+
+
+}  // namespace
+}  // namespace snap
+
+extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream_t stream) {
+  if (!args) return fail(SNAP_ERR_INVALID, "NULL sweep arguments");
+  const snap_sweep_args_t &a = *args;
+  SNAP_TRY(check_sweep_args(a));
+  int dev;
+  SNAP_TRY(current_device(&dev));
+  const Extent3 e{a.x_range, a.y_range, a.z_range};
+  // plan per chunk shape, one for steady and one for time-dependent sweeps
+  const bool td = a.vdelt != 0.0 || a.mms_source;
+  hipStream_t st = (hipStream_t)stream;
+  SweepPlan *plan = nullptr;
+  StreamWork *wk = nullptr;
+  const QuadDev *quad = nullptr;
+  {
+    // the context lock covers the table lookups only; workspace growth and
+    // the enqueue run under the stream's own lock, so concurrent corner
+    // tasks on different streams do not wait on each other
+    std::lock_guard<std::mutex> lock(g_ctx_mutex);
+    DeviceContext &c = g_ctx[dev];
+    if (!c.valid)
+      return fail(SNAP_ERR_NO_CONTEXT, "initialize_gpu_context was not called on device %d", dev);
+    if (a.num_angles != c.quad.nang || a.num_moments != c.quad.nmom)
+      return fail(SNAP_ERR_INVALID, "angles/moments differ from the context");
+    quad = &c.quad;
+    for (auto &pp : c.plans)
+      if (pp.first.nx == e.nx && pp.first.ny == e.ny && pp.first.nz == e.nz && pp.second->td == td)
+        plan = pp.second;
+    if (!plan) {
+      int TX = 0, TY = 0, TZ = 0, waves = 0, apl = 0;
+      plan = new SweepPlan();
+      int rc;
+      if (c.quad.nang > 128) {  // past two angles per lane of a wavefront
+        rc = plan_create_wide(plan, e, c.quad.nang, c.quad.cmom, td);
+      } else {
+        plan_choose(e, c.quad.nang, c.quad.cmom, 8, &TX, &TY, &TZ, &waves, &apl, td);
+        rc = plan_create(plan, e, c.quad.nang, c.quad.cmom, TX, TY, TZ, waves, apl, td);
+      }
+      if (rc != SNAP_OK) {
+        delete plan;
+        return rc;
+      }
+      c.plans.emplace_back(e, plan);
+    }
+    // per-stream face workspace (gpu_sweep.cu keeps 12 per-stream scratch slots)
+    for (auto &w : c.work)
+      if (w->stream == st) wk = w.get();
+    if (!wk) {
+      c.work.emplace_back(new StreamWork());
+      wk = c.work.back().get();
+      wk->stream = st;
+    }
+  }
+  const size_t need = plan->face_doubles_per_sweep() * sizeof(double);
+  std::lock_guard<std::mutex> wlock(wk->lock);
+  if (wk->bytes < need) {
+    if (wk->faces) {
+      SNAP_HIP_TRY(hipStreamSynchronize(st));
+      SNAP_HIP_TRY(hipFree(wk->faces));
+      wk->faces = nullptr;
+      wk->bytes = 0;
+    }
+    if (hipMalloc(&wk->faces, need) != hipSuccess)
+      return fail(SNAP_ERR_NO_MEMORY, "face workspace of %zu bytes", need);
+    wk->bytes = need;
+  }
+  return dropin_launch(a, plan, quad, wk->faces, st);
 }

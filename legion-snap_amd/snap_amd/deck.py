@@ -101,15 +101,20 @@ def reference_cross_sections(ng: int, nmom: int, two_materials: bool):
         for m in range(nmat):
             fin, fup, fdown, ffirst, flast = split[m]
             for g in range(ng):
+                # the split fractions multiply by a precomputed reciprocal of
+                # the group count, (frac * sigs) * (1/n), as snap.cc:637-650
+                # and 663-676 do (dividing by n can differ in the last bit)
                 slgg[g, m, g, 0] = fin * sigs[m, g]
                 if g > 0:
+                    t = 1.0 / float(g)
                     for g2 in range(g):
-                        slgg[g2, m, g, 0] = fup * sigs[m, g] / g
+                        slgg[g2, m, g, 0] = fup * sigs[m, g] * t
                 else:
                     slgg[g, m, g, 0] = ffirst * sigs[m, g]
                 if g < ng - 1:
+                    t = 1.0 / float(ng - (g + 1))
                     for g2 in range(g + 1, ng):
-                        slgg[g2, m, g, 0] = fdown * sigs[m, g] / (ng - (g + 1))
+                        slgg[g2, m, g, 0] = fdown * sigs[m, g] * t
                 else:
                     slgg[g, m, g, 0] = flast * sigs[m, g]
     ratios = [(0.1, 0.5), (0.8, 0.6)]

@@ -57,6 +57,29 @@ def test_scattering_conserves_sigs(ng, nmom):
         assert np.isclose(slgg[0, 0, ng - 1, 0], 0.1 * sigs[0, ng - 1] / (ng - 1), rtol=1e-14)
 
 
+@pytest.mark.parametrize("ng", [4, 7, 32])
+def test_scattering_split_rounds_like_the_reference(ng):
+    """snap.cc:637-650 and 663-676 form the up- and downscatter entries as
+    (fraction * sigs) * t with t = 1.0/double(n) computed once; dividing by n
+    rounds differently in the last bit for some entries, so the table must
+    be bit-equal to the reciprocal form and (for these group counts) differ
+    from the quotient form somewhere."""
+    _, _, sigs, slgg = deck.reference_cross_sections(ng, 1, True)
+    differs = 0
+    for m, (fup, fdown) in enumerate([(0.1, 0.7), (0.1, 0.4)]):
+        for g in range(1, ng):
+            t = np.float64(1.0) / np.float64(g)
+            want = (np.float64(fup) * sigs[m, g]) * t
+            assert slgg[0, m, g, 0] == want
+            differs += int(np.float64(fup) * sigs[m, g] / np.float64(g) != want)
+        for g in range(ng - 1):
+            t = np.float64(1.0) / np.float64(ng - (g + 1))
+            want = (np.float64(fdown) * sigs[m, g]) * t
+            assert slgg[ng - 1, m, g, 0] == want
+            differs += int(np.float64(fdown) * sigs[m, g] / np.float64(ng - (g + 1)) != want)
+    assert differs > 0
+
+
 def test_layouts():
     mat, qi = deck.reference_layouts(8, 8, 4, 2, 1, 1)
     # CENTER: 1-based [n/4+1, 3n/4] per axis

@@ -149,3 +149,37 @@ def test_uninitialized_device_context_is_an_error(setup):
     rc = _lib.load().snap_hip_run_gpu_sweep(ctypes.byref(a), None)
     # the context above has 12 angles: a mismatching call is rejected
     assert rc == _lib.SNAP_ERR_INVALID
+
+
+@pytest.mark.parametrize("w", [(1.0, 0.0), (0.0, 1.0)])
+def test_fixup_count_is_cell_global(gpu, oracle_mod, w):
+    """The oracle's hand-derived one-cell, two-angle fixup case (see
+    tests/test_oracle_sweep.py): angle A's negative x outflow makes the cell
+    enter the fixup loop, which recomputes angle B's pc with the fixup
+    formula too (0x1.3333333333333p-1, not its diamond-difference
+    0x1.3333333333334p-1).  The drop-in sweep, run with that quadrature and
+    ghost inflows, must give the oracle's bits for each angle (one weight
+    zeroed at a time) and the same ghost outflows."""
+    import torch
+    mu, eta, xi = np.array([1.0, 0.1]), np.array([1.0, 0.1]), np.array([1.0, 1.3])
+    ec = np.ones((8, 1, 2))
+    quad = (mu, eta, xi, np.asarray(w), ec)
+    h = (1.0, 1.0, 1.0)
+    txs = np.ones((1, 1, 1, 1))
+    dinv = oracle_mod.geometry_param(txs, 2, h, mu, eta, xi)
+    gin = (np.array([[[[10.0, 1.0]]]]), np.array([[[[0.0, 1.0]]]]), np.array([[[[0.0, 1.0]]]]))
+    tasks.initialize_gpu_context(quad, 2, 1, (1, 1, 1))
+    dev = torch.device("cuda")
+    T = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)  # noqa: E731
+    flux = torch.zeros((1, 1, 1, 1), dtype=torch.float64, device=dev)
+    gd = tuple(T(x) for x in gin)
+    tasks.mini_kba(7, [0], T(np.zeros((1, 1, 1, 1, 1))), flux, None, T(dinv), T(txs), gd, h, 2, 1)
+    torch.cuda.synchronize()
+    pf, _, nfix, outs = oracle_mod.sweep(1, 1, 1, 2, 1, 7, h, quad, np.zeros((1, 1, 1, 1)),
+                                         np.ascontiguousarray(dinv[0]), np.ascontiguousarray(txs[0]),
+                                         fixup=True, ghost_in=tuple(x[0] for x in gin),
+                                         want_ghost_out=True)
+    assert nfix == 1
+    assert flux.cpu().numpy()[0, 0, 0, 0] == pf[0, 0, 0]
+    for k in range(3):
+        assert np.array_equal(gd[k][0].cpu().numpy(), outs[k]), k

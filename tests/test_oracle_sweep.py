@@ -153,3 +153,56 @@ def test_moments_are_weighted_angular_sums(oracle_mod):
         pf, pm, _, _ = oracle_mod.sweep(nx, ny, nz, nang, 4, corner, h, quad, qt, dinv, txs)
         for l in range(1, 4):
             assert np.allclose(pm[..., l - 1], ec[corner, l, 0] * pf, rtol=1e-14, atol=1e-17)
+
+
+def _two_angle_cell(oracle_mod, w):
+    """One cell (t_xs = 1, no source, hi = hj = hk = 1) with two hand-picked
+    directions: angle A (mu = eta = xi = 1) gets inflows (10, 0, 0), angle B
+    (0.1, 0.1, 1.3) gets (1, 1, 1).  Returns (part_flux, ghost outflows)."""
+    mu, eta, xi = np.array([1.0, 0.1]), np.array([1.0, 0.1]), np.array([1.0, 1.3])
+    ec = np.ones((8, 1, 2))
+    h = (1.0, 1.0, 1.0)
+    txs = np.ones((1, 1, 1))
+    q = np.zeros((1, 1, 1, 1))
+    dinv = oracle_mod.geometry_param(txs[None], 2, h, mu, eta, xi)[0]
+    gin = (np.array([[[10.0, 1.0]]]), np.array([[[0.0, 1.0]]]), np.array([[[0.0, 1.0]]]))
+    pf, _, nfix, outs = oracle_mod.sweep(1, 1, 1, 2, 1, 7, h, (mu, eta, xi, np.asarray(w), ec), q,
+                                         dinv, txs, fixup=True, ghost_in=gin, want_ghost_out=True)
+    return pf[0, 0, 0], outs, nfix
+
+
+def test_fixup_count_is_cell_global(oracle_mod):
+    """sweep.cc:437-505 counts negative outflows over all angles of the cell
+    and, once any is negative, recomputes every angle's pc with the fixup
+    formula.  Derived by hand for the cell of _two_angle_cell:
+
+    * angle A: DD pc = 10/4 = 2.5, x outflow 2*2.5 - 10 < 0 (one negative);
+      fixed pc = (0 + 0.5*(10*1*1*(1+0))) / (1 + 0 + 1 + 1) = 5/3, whose x
+      outflow is still negative and y, z outflows are not: count 1 again, so
+      the loop stops with pc_A = 5/3 and outflows (0, 10/3, 10/3);
+    * angle B has no negative outflow (DD pc = 1.5/2.5 = 0.6), yet the cell
+      entered the loop, so its pc is recomputed as
+      0.5*(0.1*2 + 0.1*2 + 1.3*2) / (1 + 0.1 + 0.1 + 1.3), which rounds to
+      0x1.3333333333333p-1, while the DD form (1.5 * dinv) rounds to
+      0x1.3333333333334p-1.  A per-angle rule (each angle iterating on its own
+      count) would keep B's DD value.
+
+    (A face once negative stays negative after its flux is fixed -- 2 pc >=
+    psi_in after the fix contradicts 2 pc < psi_in before it, for positive
+    direction terms -- so per-angle counts only grow and the two rules give
+    the same flags; they part only in this recomputation.)"""
+    pa, outs, nfix = _two_angle_cell(oracle_mod, [1.0, 0.0])
+    assert nfix == 1
+    assert pa == 5.0 / 3.0
+    assert outs[0][0, 0, 0] == 0.0 and outs[1][0, 0, 0] == 2 * (5.0 / 3.0) - 0.0
+    assert outs[2][0, 0, 0] == 2 * (5.0 / 3.0) - 0.0
+    pb, outs, _ = _two_angle_cell(oracle_mod, [0.0, 1.0])
+    fixup_form = float.fromhex("0x1.3333333333333p-1")
+    dd_form = float.fromhex("0x1.3333333333334p-1")
+    num = 0.0 + 0.5 * (1.0 * 0.1 * 1.0 * (1.0 + 1.0) + 1.0 * 0.1 * 1.0 * (1.0 + 1.0)
+                       + 1.0 * 1.3 * 1.0 * (1.0 + 1.0))
+    den = 1.0 + 0.1 * 1.0 * 1.0 + 0.1 * 1.0 * 1.0 + 1.3 * 1.0 * 1.0
+    assert num / den == fixup_form
+    assert pb == fixup_form and pb != dd_form
+    # B's outflows keep their diamond-difference closure of the recomputed pc
+    assert outs[0][0, 0, 1] == 2 * fixup_form - 1.0
