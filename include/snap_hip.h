@@ -118,6 +118,17 @@ typedef struct {
  * SIMULTANEOUS read-write, sweep.cc:54-55, gpu_sweep.cu:1705). */
 int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream_t stream);
 
+/* snap_hip_run_gpu_sweep without any device allocation: the same kernels and
+ * results, with the plan tables and tile-face workspace in a fixed scratch
+ * array of 64 MiB that each device holds until the process ends (chunks whose
+ * faces and tables exceed it get SNAP_ERR_UNSUPPORTED).  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  initialize_gpu_context must have
+ * run on the device.  Every call on one device shares its scratch array:
+ * calls that overlap on one device (other streams, other threads) race;
+ * calls on one stream are ordered.  The array needs no clearing: each call
+ * writes its tables and every face slot before reading it. */
+int snap_hip_run_gpu_sweep_noalloc(const snap_sweep_args_t *args, snap_stream_t stream);
 
 /* Replaces run_flux0_outer_source (decl outer.cc:220-225, def
  * gpu_outer.cu:104-223): qo0[g] = qi0[g] + sum_{g'!=g} slgg[g][mat][g'].l0 * flux0[g'].

@@ -1698,6 +1698,16 @@ constexpr int kMaxDevices = 64;
 DeviceContext g_ctx[kMaxDevices];
 std::mutex g_ctx_mutex;
 
+// Scratch memory of the allocation-free sweep: the plan tables and the tile
+// face workspace of one call, on every device the library runs on, held
+// until the process ends.  Host code never uses this name as an address; it
+// uses the per-device pointer that hipGetSymbolAddress returns.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+void *g_scratch[kMaxDevices];
+
+inline size_t a256(size_t b) { return (b + 255) & ~size_t(255); }
+
 int current_device(int *dev) {
   SNAP_HIP_TRY(hipGetDevice(dev));
   if (*dev < 0 || *dev >= kMaxDevices) return fail(SNAP_ERR_INVALID, "device %d out of range", *dev);
@@ -1747,6 +1757,13 @@ extern "C" int snap_hip_initialize_gpu_context(const double *ec, const double *m
   DeviceContext &c = g_ctx[dev];
   context_release(c);
   SNAP_TRY(quad_upload(&c.quad, ec, mu, eta, xi, w, num_angles, num_moments));
+  // the device address of this device's copy of the allocation-free sweep's
+  // scratch array (each device holds its own)
+  if (!g_scratch[dev]) {
+    void *ptr = nullptr;
+    SNAP_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = ptr;
+  }
   c.chunk = Extent3{nx_per_chunk, ny_per_chunk, nz_per_chunk};
   c.valid = true;
   return SNAP_OK;
@@ -1825,6 +1842,35 @@ int dropin_launch(const snap_sweep_args_t &a, const SweepPlan *plan, const QuadD
   return sweep_enqueue(L, st);
 }
 
+// Tables of a plan built in device memory that the caller provides (the
+// allocation-free entry point): tiles by tile diagonal, in plan_layout's
+// order (diagonal, then tz, then ty), and for tiled plans the cell-diagonal
+// offsets and cells of one tile.  One thread writes them: a few thousand
+// entries at most for a chunk that fits the scratch array.
+__global__ void k_plan_tables(int nTx, int nTy, int nTz, int TX, int TY, int TZ, int tiled,
+                              int *plane_tiles, int *cell_off, int16_t *cells) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int n = 0;
+  for (int D = 0; D < nTx + nTy + nTz - 2; D++)
+    for (int tz = 0; tz < nTz; tz++)
+      for (int ty = 0; ty < nTy; ty++) {
+        const int tx = D - ty - tz;
+        if (tx >= 0 && tx < nTx) plane_tiles[n++] = tx | (ty << 10) | (tz << 20);
+      }
+  if (!tiled) return;
+  int m = 0;
+  const int ncp = TX + TY + TZ - 2;
+  for (int d = 0; d < ncp; d++) {
+    cell_off[d] = m;
+    for (int k = 0; k < TZ; k++)
+      for (int j = 0; j < TY; j++) {
+        const int i = d - j - k;
+        if (i >= 0 && i < TX) cells[m++] = (int16_t)(i | (j << 5) | (k << 10));
+      }
+  }
+  cell_off[ncp] = m;
+}
+
 
 }  // namespace
 }  // namespace snap
@@ -1895,4 +1941,66 @@ extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream
     wk->bytes = need;
   }
   return dropin_launch(a, plan, quad, wk->faces, st);
+}
+
+extern "C" int snap_hip_run_gpu_sweep_noalloc(const snap_sweep_args_t *args,
+                                              snap_stream_t stream) {
+  if (!args) return fail(SNAP_ERR_INVALID, "NULL sweep arguments");
+  const snap_sweep_args_t &a = *args;
+  SNAP_TRY(check_sweep_args(a));
+  int dev;
+  SNAP_TRY(current_device(&dev));
+  const Extent3 e{a.x_range, a.y_range, a.z_range};
+  const bool td = a.vdelt != 0.0 || a.mms_source;
+  const QuadDev *quad = nullptr;
+  char *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_ctx_mutex);
+    DeviceContext &c = g_ctx[dev];
+    if (!c.valid)
+      return fail(SNAP_ERR_NO_CONTEXT, "initialize_gpu_context was not called on device %d", dev);
+    if (a.num_angles != c.quad.nang || a.num_moments != c.quad.nmom)
+      return fail(SNAP_ERR_INVALID, "angles/moments differ from the context");
+    quad = &c.quad;
+    base = (char *)g_scratch[dev];
+  }
+  if (!base) return fail(SNAP_ERR_NO_CONTEXT, "no scratch address for device %d", dev);
+  // the plan of this chunk shape, laid out on the host (no device memory)
+  SweepPlan plan;
+  std::vector<int> tiles, coff;
+  std::vector<int16_t> cells;
+  const bool wide = quad->nang > 128;
+  if (wide) {
+    SNAP_TRY(plan_layout_wide(&plan, e, quad->nang, quad->cmom, td, tiles));
+  } else {
+    int TX = 0, TY = 0, TZ = 0, waves = 0, apl = 0;
+    plan_choose(e, quad->nang, quad->cmom, 8, &TX, &TY, &TZ, &waves, &apl, td);
+    SNAP_TRY(plan_layout(&plan, e, quad->nang, quad->cmom, TX, TY, TZ, waves, apl, td, tiles, coff,
+                         cells));
+  }
+  // scratch: plan tables, then the face workspace
+  const size_t o_tiles = 0;
+  const size_t o_coff = a256(o_tiles + sizeof(int) * tiles.size());
+  const size_t o_cells = a256(o_coff + sizeof(int) * coff.size());
+  const size_t o_faces = a256(o_cells + sizeof(int16_t) * cells.size());
+  const size_t total = o_faces + plan.face_doubles_per_sweep() * sizeof(double);
+  if (total > kScratchBytes)
+    return fail(SNAP_ERR_UNSUPPORTED,
+                "a %dx%dx%d chunk with %d angles needs %zu bytes of sweep scratch; the "
+                "allocation-free sweep holds %zu (use snap_hip_run_gpu_sweep)",
+                e.nx, e.ny, e.nz, quad->nang, total, kScratchBytes);
+  hipStream_t st = (hipStream_t)stream;
+  plan.plane_tiles = (int *)(base + o_tiles);
+  plan.cell_off = wide ? nullptr : (int *)(base + o_coff);
+  plan.cells = wide ? nullptr : (int16_t *)(base + o_cells);
+  hipLaunchKernelGGL(k_plan_tables, dim3(1), dim3(64), 0, st, plan.nTx, plan.nTy, plan.nTz,
+                     plan.TX, plan.TY, plan.TZ, wide ? 0 : 1, plan.plane_tiles, plan.cell_off,
+                     plan.cells);
+  SNAP_HIP_TRY(hipGetLastError());
+  const int rc = dropin_launch(a, &plan, quad, base + o_faces, st);
+  // the tables belong to the scratch array: nothing for plan_free to release
+  plan.plane_tiles = nullptr;
+  plan.cell_off = nullptr;
+  plan.cells = nullptr;
+  return rc;
 }

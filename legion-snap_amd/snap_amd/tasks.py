@@ -98,14 +98,17 @@ def quadrature(nang: int, nmom: int):
 
 def mini_kba(corner: int, groups: Sequence[int], qtot, flux, fluxm, dinv, t_xs,
              ghosts, hijk, nang: int, nmom: int, fixup: bool = True, origin=(0, 0, 0),
-             vdelt=None, time_flux_in=None, time_flux_out=None, qim=None, extent=None) -> None:
+             vdelt=None, time_flux_in=None, time_flux_out=None, qim=None, extent=None,
+             noalloc: bool = False) -> None:
     """One corner over a group range: run_gpu_sweep once per group.
     ghosts: (gx [ng, nz, ny, nang], gy [ng, nz, nx, nang], gz [ng, ny, nx, nang]) or None,
     indexed by absolute points.  extent: (nx, ny, nz) of the chunk at `origin`
     (default: the whole of flux), whose arrays are indexed by absolute points.
     Time dependence (MiniKBATask regions 5, 6, 11, sweep.cc:45-84): vdelt
     [ng] host values, time_flux_in / time_flux_out [ng, nz, ny, nx, nang] of
-    this corner; qim [ng, nz, ny, nx, nang] is the corner's MMS source."""
+    this corner; qim [ng, nz, ny, nx, nang] is the corner's MMS source.
+    noalloc: call the allocation-free entry point (scratch array, no device
+    allocation) instead of run_gpu_sweep."""
     lib = _lib.load()
     nx, ny, nz = extent if extent is not None else flux.shape[1:4][::-1]
     for g in groups:
@@ -142,7 +145,12 @@ def mini_kba(corner: int, groups: Sequence[int], qtot, flux, fluxm, dinv, t_xs,
             a.mms_source = 1
         a.num_angles = nang
         a.fixup = int(bool(fixup))
-        _lib.check(lib.snap_hip_run_gpu_sweep(ctypes.byref(a), _stream()), "snap_hip_run_gpu_sweep")
+        if noalloc:
+            _lib.check(lib.snap_hip_run_gpu_sweep_noalloc(ctypes.byref(a), _stream()),
+                       "snap_hip_run_gpu_sweep_noalloc")
+        else:
+            _lib.check(lib.snap_hip_run_gpu_sweep(ctypes.byref(a), _stream()),
+                       "snap_hip_run_gpu_sweep")
 
 
 def calc_outer_source(qi, flux0, slgg, qo0, mat, fluxm=None, qom=None, nmom: int = 1) -> None:
