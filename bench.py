@@ -175,6 +175,13 @@ def main() -> int:
     ap.add_argument("--decks", nargs="*", default=None,
                     help="run the reference input decks (all when no names are given) and "
                          "print their suite line instead of the default workload")
+    ap.add_argument("--sweeps", nargs=2, metavar=("KERNEL", "PRECISION"), default=None,
+                    help="profiling harness: one solve of the workload, then --count timed inner "
+                         "sweeps with the given sweep kernel (auto, tiled, wave_tile) and precision "
+                         "(fp64, fp64_exact, mixed); prints one JSON line")
+    ap.add_argument("--count", type=int, default=3, help="timed inner sweeps for --sweeps")
+    ap.add_argument("--dinv", default="default", help="dinv mode for --sweeps (stream, recompute)")
+    ap.add_argument("--tz", type=int, default=0, help="tile depth for --sweeps (0 = default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy and exit (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE traffic counters)")
@@ -201,6 +208,17 @@ def main() -> int:
                           "value": tot_sweep * 1e9 / tot_units, "unit": "ns/(cell·angle·group)",
                           "n_gpus": 1, "higher_is_better": False, "dtype": "f64",
                           "data": "the reference's input decks (input/*.in)", "decks": rows}))
+        return 0
+    if args.sweeps is not None:
+        kern, prec = args.sweeps
+        p = problems.config1(n=args.size)
+        with Solver(p, precision=prec, sweep_kernel=kern, dinv=args.dinv,
+                    tile=(0, 0, args.tz)) as s:
+            st = s.run()
+            ts = s.time_sweeps(args.count, prec)
+            plan = s.plan()
+        print(json.dumps({"kernel": kern, "precision": prec, "dinv": args.dinv, "plan": plan,
+                          "inner": st["inner_total"], "grind_ns": st["grind_ns"], **ts}))
         return 0
     if args.copy_probe:
         n = 1 << 27  # 1 GiB of doubles

@@ -248,8 +248,9 @@ typedef struct {
   int mms;
 } snap_problem_t;
 
-#define SNAP_DINV_DEFAULT 0     /* stream; MIXED reads the FP64-formed dinv rounded
-                                   once to FP32 (half the error of forming it in FP32) */
+#define SNAP_DINV_DEFAULT 0     /* the one-wavefront tile sweep forms dinv from t_xs;
+                                   the multi-wave tile sweeps stream it (MIXED reads the
+                                   FP64-formed dinv rounded once to FP32) */
 #define SNAP_DINV_STREAM 1      /* sweep reads the dinv array (reference data flow) */
 #define SNAP_DINV_RECOMPUTE 2   /* sweep forms dinv from t_xs per cell and angle */
 
@@ -266,7 +267,17 @@ typedef struct {
                                when not FP64_EXACT (default: FP64 matrix cores) */
   int wide_sweep;           /* 1 = one workgroup per cell and sweep (the only form past
                                128 angles per octant; up to 2048); 0 = tiled when it fits */
+  int sweep_kernel;         /* SNAP_SWEEP_AUTO, _TILED (multi-wave workgroups per tile)
+                               or _WAVE_TILE (one wavefront per 4x4xTZ tile, steady
+                               problems of up to 64 angles; tile[2] sets TZ).  AUTO takes
+                               the wave tile for steady problems of 33..64 angles unless
+                               a multi-wave option (tile x/y, waves, angles per lane) is set */
+  double fp64_polish;       /* MIXED: run an inner's sweep in FP64 when the previous inner
+                               of the same outer changed the scalar flux by at most
+                               fp64_polish * epsi (max relative change); 0 = never */
 } snap_solve_options_t;
+
+enum { SNAP_SWEEP_AUTO = 0, SNAP_SWEEP_TILED = 1, SNAP_SWEEP_WAVE_TILE = 2 };
 
 /* MIXED with refine = 1 (iterative refinement of each outer iteration): the
  * first inner sweep of an outer iteration runs in FP64 with the fixup and is

@@ -24,7 +24,7 @@ OUT_DIR = os.path.join(HERE, "snap_amd")
 LIB = os.path.join(OUT_DIR, "libsnap_hip.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 SOURCES = ["snap_common.hip", "snap_xs.hip", "snap_source.hip", "snap_outer.hip",
-           "snap_sweep.hip", "snap_adjoint.hip", "snap_dropin.hip", "snap_solver.hip",
+           "snap_sweep.hip", "snap_sweep_seq.hip", "snap_adjoint.hip", "snap_dropin.hip", "snap_solver.hip",
            "snap_batch.hip", "snap_mms.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

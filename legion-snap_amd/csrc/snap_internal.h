@@ -151,6 +151,9 @@ int launch_to_float(const double *a, float *b, int64_t n, hipStream_t st);
 // out = (base ? base : 0) + sum over the 8 octant partials, octants in order
 int launch_octant_reduce(const double *part, double *out, int64_t n, hipStream_t st,
                          const double *base = nullptr);
+// *out = bit pattern of max |f/fp - 1| over n values (|f| where |fp| < 1e-12)
+int launch_max_change(const double *f, const double *fp, int64_t n, unsigned long long *out,
+                      hipStream_t st);
 // y -= x
 int launch_subtract(double *y, const double *x, int64_t n, hipStream_t st);
 int launch_fill_int(int32_t *p, int32_t v, hipStream_t st);
@@ -172,6 +175,9 @@ struct SweepPlan {
   // one workgroup per cell (any angle count): tiles are single cells, the
   // faces are in-place planes per sweep and `waves` spans the cell's angles
   bool wide = false;
+  // one wavefront per 4 x 4 x TZ tile walking its cells in sequence
+  // (snap_sweep_seq.hip): no cell tables, waves = 1
+  bool seq = false;
   int nplanes = 0;             // tile diagonals
   int ncellplanes = 0;         // cell diagonals inside one tile
   std::vector<int> plane_off;  // host: offset of each tile diagonal in plane_tiles
@@ -190,6 +196,13 @@ void plan_free(SweepPlan *p);
 // The plan of the one-cell-per-workgroup sweep (angle counts past 128, or on
 // request): cell diagonals of the chunk, angles across up to 1024 lanes.
 int plan_create_wide(SweepPlan *p, Extent3 e, int nang, int cmom, bool td = false);
+// Host layout of plan_create / plan_create_wide without device memory: the
+// plan's fields and the tables plan_create would upload.
+int plan_layout(SweepPlan *p, Extent3 e, int nang, int cmom, int TX, int TY, int TZ, int waves,
+                int apl, bool td, std::vector<int> &tiles, std::vector<int> &coff,
+                std::vector<int16_t> &cells);
+int plan_layout_wide(SweepPlan *p, Extent3 e, int nang, int cmom, bool td,
+                     std::vector<int> &cells);
 // Fill in the tile edges, waves per workgroup and angles per lane left at 0
 // for a sweep whose faces are real_bytes wide (8 FP64, 4 FP32).
 void plan_choose(Extent3 e, int nang, int cmom, int real_bytes, int *TX, int *TY, int *TZ,
@@ -253,6 +266,11 @@ struct SweepLaunch {
 // Enqueue every tile diagonal of the sweep (plan->nplanes launches).  When
 // `events` is given (2*nplanes events) each launch is bracketed by a pair.
 int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events = nullptr);
+// The plan of the one-wavefront tile sweep (snap_sweep_seq.hip): 4 x 4 x TZ
+// tiles, up to 64 angles, steady dense sweeps.
+int plan_create_seq(SweepPlan *p, Extent3 e, int nang, int cmom, int TZ);
+// Enqueue every tile diagonal of a one-wavefront tile sweep (dense, steady).
+int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events = nullptr);
 // Quadrature of the drop-in context on device `dev` (NULL before init).
 const QuadDev *context_quad(int dev);
 

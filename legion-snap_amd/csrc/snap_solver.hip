@@ -23,6 +23,7 @@
 // mid-time before its outer iterations.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <memory>
 
@@ -64,9 +65,12 @@ struct snap_solver {
   QuadDev quad;
   SweepPlan plan;    // the sweep plan of the solver's precision
   SweepPlan plan64;  // MIXED: the plan of its FP64 sweeps
+  // one-wavefront tile sweep (steady problems up to 64 angles): used for
+  // every inner sweep when present; the adjoint keeps the multi-wave plan
+  SweepPlan plan_seq;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  int32_t *h_flag = nullptr;  // pinned
+  int32_t *h_flag = nullptr;  // pinned: [0] the test's flag, [2..3] max change bits
   DeviceBuffer mat, sigt, slgg, qi, txs, txs32, sxs, dinv, dinv32;
   DeviceBuffer flux0, flux0pi, flux0po, fluxm, q2grp0, q2grpm, qtot;
   DeviceBuffer part_flux, part_fluxm, faces, flag;
@@ -86,6 +90,7 @@ struct snap_solver {
     mms_free(&mms);
     plan_free(&plan);
     plan_free(&plan64);
+    plan_free(&plan_seq);
     quad_free(&quad);
     for (auto &x : ev)
       if (x) (void)hipEventDestroy(x);
@@ -104,8 +109,15 @@ struct snap_solver {
   const SweepPlan &plan_for(int precision) const {
     return (precision != SNAP_PRECISION_MIXED && plan64.nplanes) ? plan64 : plan;
   }
+  // the plan the inner sweeps run on
+  const SweepPlan &sweep_plan(int precision) const {
+    return plan_seq.nplanes ? plan_seq : plan_for(precision);
+  }
 
-  int sweep(int precision, hipEvent_t *events, bool fixup = true, bool recompute = false) {
+  // tiled_only: the multi-wave tile plan even when the one-wavefront plan
+  // exists (the adjoint's checkpoint faces are in that plan's layout)
+  int sweep(int precision, hipEvent_t *events, bool fixup = true, bool recompute = false,
+            bool tiled_only = false) {
     const bool fp32 = precision == SNAP_PRECISION_MIXED;
     SweepLaunch L;
     memset(&L, 0, sizeof(L));
@@ -127,7 +139,10 @@ struct snap_solver {
     L.faces = faces.p;
     L.fp32 = fp32;
     L.exact = precision == SNAP_PRECISION_FP64_EXACT;
-    L.recompute_dinv = recompute || opt.dinv_mode == SNAP_DINV_RECOMPUTE;
+    // the one-wavefront tile sweep forms dinv from t_xs unless told to
+    // stream it (measured faster: it saves 8 bytes per unit of HBM traffic)
+    L.recompute_dinv = recompute || opt.dinv_mode == SNAP_DINV_RECOMPUTE ||
+                       (opt.dinv_mode == SNAP_DINV_DEFAULT && plan_seq.nplanes && !tiled_only);
     if (fp32 && !L.recompute_dinv && !dinv32.p) return fail(SNAP_ERR_INVALID, "no FP32 dinv");
     L.td = td;
     if (td) {
@@ -137,6 +152,10 @@ struct snap_solver {
         L.tout = tflux[(step & 1) ^ 1].as<double>();
       }
       L.qim = prob.mms ? qim.as<double>() : nullptr;
+    }
+    if (plan_seq.nplanes && !tiled_only) {
+      L.plan = &plan_seq;
+      return sweep_seq_enqueue(L, stream, events);
     }
     return sweep_enqueue(L, stream, events);
   }
@@ -170,12 +189,19 @@ struct snap_solver {
                                      (int64_t)ng * cells, nmom, stream);
   }
 
-  int test(const DeviceBuffer &prev, double eps, bool *converged) {
+  // change: when not NULL, also the largest relative change of the test's
+  // quantity (the mixed-precision FP64 polish reads it)
+  int test(const DeviceBuffer &prev, double eps, bool *converged, double *change = nullptr) {
     SNAP_TRY(launch_convergence(g_acc(flux0, 1, 8), g_acc(prev, 1, 8), e, ng, eps, flag.as<int32_t>(),
                                 stream));
-    SNAP_HIP_TRY(hipMemcpyAsync(h_flag, flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    if (change)
+      SNAP_TRY(launch_max_change(flux0.as<double>(), prev.as<double>(), (int64_t)ng * cells,
+                                 (unsigned long long *)(flag.as<char>() + 8), stream));
+    SNAP_HIP_TRY(hipMemcpyAsync(h_flag, flag.p, change ? 16 : sizeof(int32_t),
+                                hipMemcpyDeviceToHost, stream));
     SNAP_HIP_TRY(hipStreamSynchronize(stream));
     *converged = (*h_flag != 0);
+    if (change) memcpy(change, (const char *)h_flag + 8, sizeof(double));
     return SNAP_OK;
   }
 
@@ -238,6 +264,23 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
     if (s->opt.precision == SNAP_PRECISION_MIXED)
       SNAP_TRY(plan_create_wide(&s->plan64, s->e, s->nang, s->cmom, s->td));
   } else {
+    // the one-wavefront tile sweep for steady problems of 33..64 angles
+    // (asked for: any steady problem of up to 64 angles)
+    const bool seq_ok = !s->td && s->nang <= 64;
+    if (s->opt.sweep_kernel == SNAP_SWEEP_WAVE_TILE && !seq_ok)
+      return fail(SNAP_ERR_UNSUPPORTED,
+                  "the one-wavefront tile sweep takes steady problems of up to 64 angles");
+    if (s->opt.sweep_kernel < SNAP_SWEEP_AUTO || s->opt.sweep_kernel > SNAP_SWEEP_WAVE_TILE)
+      return fail(SNAP_ERR_INVALID, "unknown sweep kernel %d", s->opt.sweep_kernel);
+    // options that only the multi-wave tile sweep has keep AUTO on it
+    const bool tiled_opts = s->opt.angles_per_lane != 0 || s->opt.waves_per_tile != 0 ||
+                            s->opt.tile[0] != 0 || s->opt.tile[1] != 0;
+    if (s->opt.sweep_kernel == SNAP_SWEEP_WAVE_TILE ||
+        (s->opt.sweep_kernel == SNAP_SWEEP_AUTO && seq_ok && s->nang > 32 && !tiled_opts)) {
+      const int tz = (s->opt.sweep_kernel == SNAP_SWEEP_WAVE_TILE && s->opt.tile[2] > 0)
+                         ? s->opt.tile[2] : 8;
+      SNAP_TRY(plan_create_seq(&s->plan_seq, s->e, s->nang, s->cmom, tz));
+    }
     plan_choose(s->e, s->nang, s->cmom, s->opt.precision == SNAP_PRECISION_MIXED ? 4 : 8, &TX,
                 &TY, &TZ, &waves, &apl, s->td);
     SNAP_TRY(plan_create(&s->plan, s->e, s->nang, s->cmom, TX, TY, TZ, waves, apl, s->td));
@@ -253,7 +296,7 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
 
   SNAP_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   for (auto &x : s->ev) SNAP_HIP_TRY(hipEventCreate(&x));
-  SNAP_HIP_TRY(hipHostMalloc((void **)&s->h_flag, sizeof(int32_t), hipHostMallocDefault));
+  SNAP_HIP_TRY(hipHostMalloc((void **)&s->h_flag, 16, hipHostMallocDefault));
 
   const size_t gc = (size_t)s->ng * s->cells;
   const size_t nm1 = s->cmom > 1 ? (size_t)(s->cmom - 1) : 0;
@@ -291,9 +334,11 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
   SNAP_TRY(s->part_flux.alloc(sizeof(double) * 8 * gc));
   SNAP_TRY(s->part_fluxm.alloc(sizeof(double) * 8 * gc * nm1));
   SNAP_TRY(s->faces.alloc(sizeof(double) * 8 * s->ng *
-                          std::max(s->plan.face_doubles_per_sweep(),
-                                   s->plan64.nplanes ? s->plan64.face_doubles_per_sweep() : 0)));
-  SNAP_TRY(s->flag.alloc(sizeof(int32_t)));
+                          std::max({s->plan.face_doubles_per_sweep(),
+                                    s->plan64.nplanes ? s->plan64.face_doubles_per_sweep() : 0,
+                                    s->plan_seq.nplanes ? s->plan_seq.face_doubles_per_sweep()
+                                                        : 0})));
+  SNAP_TRY(s->flag.alloc(16));
 
   // ExpandCrossSection(sigt -> t_xs), ExpandScatteringCrossSection,
   // CalculateGeometryParam (snap.cc:419-450)
@@ -385,17 +430,25 @@ extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) 
       SNAP_TRY(launch_scale(s->qim.as<double>(), (int64_t)ta,
                             cy == 0 ? dt * 0.5 : double(2 * cy + 1) / double(2 * cy - 1), stream));
     bool inner_converged = false, converged = false;
+    const bool polish = mixed && s->opt.fp64_polish > 0.0;
     for (int otno = 0; otno < s->prob.oitm; otno++) {
       SNAP_TRY(s->outer_source());
       SNAP_TRY(s->copy(s->flux0po, s->flux0, sizeof(double) * gc));
       st.outer_total++;
       inner_converged = false;
+      // the previous inner's largest relative change, within this outer
+      double change = HUGE_VAL;
       for (int inno = 0; inno < s->prob.iitm; inno++) {
         SNAP_TRY(s->inner_source());
         SNAP_TRY(s->copy(s->flux0pi, s->flux0, sizeof(double) * gc));
         const bool refine = mixed && s->opt.refine;
+        // FP64 polish: an inner whose predecessor moved the flux by at most
+        // fp64_polish * epsi is likely one of the outer's last (the change
+        // contracts by the in-group scattering ratio each inner), so it runs
+        // in FP64 and the outer ends on FP64 sweeps
+        const bool polish_now = polish && change <= s->opt.fp64_polish * s->prob.epsi;
         const bool fp32 = mixed && !(s->opt.first_sweep_fp64 && st.inner_total == 0) &&
-                          !(refine && inno == 0);
+                          !(refine && inno == 0) && !polish_now;
         if (refine && fp32)  // FP64 defect of the source against the outer's base
           SNAP_TRY(launch_subtract(s->qtot.as<double>(), s->qbase.as<double>(),
                                    (int64_t)gc * s->cmom, stream));
@@ -417,7 +470,7 @@ extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) 
         else
           st.sweeps_fp64++;
         st.inner_total++;
-        SNAP_TRY(s->test(s->flux0pi, s->prob.epsi, &inner_converged));
+        SNAP_TRY(s->test(s->flux0pi, s->prob.epsi, &inner_converged, polish ? &change : nullptr));
         record(0, cy, otno, inno, inner_converged);
         float ms = 0.f;
         SNAP_HIP_TRY(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
@@ -510,7 +563,7 @@ extern "C" int snap_hip_solver_time_sweeps(snap_solver_t *s, int count, int prec
   const bool fp32 = precision == SNAP_PRECISION_MIXED;
   if (fp32 && !s->have_fp32) return fail(SNAP_ERR_INVALID, "solver was created without mixed precision");
   if (precision < 0 || precision > 2) return fail(SNAP_ERR_INVALID, "unknown precision %d", precision);
-  const int np = s->plan_for(precision).nplanes;
+  const int np = s->sweep_plan(precision).nplanes;
   std::vector<hipEvent_t> evs((size_t)2 * np * count);
   for (auto &x : evs) SNAP_HIP_TRY(hipEventCreate(&x));
   SNAP_TRY(s->sweep(precision, nullptr));  // first launches load the code object
@@ -572,7 +625,7 @@ extern "C" int snap_hip_solver_gradient(snap_solver_t *s, const double *resp, in
   // sweep of it: its tile faces are the checkpoint the reverse sweeps start from
   SNAP_TRY(s->outer_source());
   SNAP_TRY(s->inner_source());
-  SNAP_TRY(s->sweep(precision, nullptr, true, true));
+  SNAP_TRY(s->sweep(precision, nullptr, true, true, true));
   SNAP_HIP_TRY(hipMemcpyAsync(zeta.p, r.p, sizeof(double) * gc, hipMemcpyDeviceToDevice, st));
   auto reverse = [&](const DeviceBuffer &z) -> int {
     SNAP_TRY(adjoint_enqueue(pl, s->quad, s->ng, s->prob.fixup ? 1 : 0, s->hi, s->hj, s->hk,
@@ -619,12 +672,13 @@ extern "C" int snap_hip_solver_gradient(snap_solver_t *s, const double *resp, in
 
 extern "C" int snap_hip_solver_plan(snap_solver_t *s, int32_t plan[6]) {
   if (!s || !plan) return fail(SNAP_ERR_INVALID, "null argument");
-  plan[0] = s->plan.TX;
-  plan[1] = s->plan.TY;
-  plan[2] = s->plan.TZ;
-  plan[3] = s->plan.waves;
-  plan[4] = s->plan.apl;
-  plan[5] = s->plan.cpw;
+  const SweepPlan &p = s->sweep_plan(s->opt.precision);
+  plan[0] = p.TX;
+  plan[1] = p.TY;
+  plan[2] = p.TZ;
+  plan[3] = p.waves;
+  plan[4] = p.apl;
+  plan[5] = p.cpw;
   return SNAP_OK;
 }
 

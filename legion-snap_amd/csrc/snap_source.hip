@@ -240,6 +240,43 @@ int launch_convergence(Acc3 flux0, Acc3 flux0p, Extent3 e, int ng, double epsi, 
   return SNAP_OK;
 }
 
+// Largest relative change max |flux0/flux0p - 1| of the convergence test's
+// quantity (|flux0| where |flux0p| < 1e-12), as the bit pattern of the
+// non-negative double (bit patterns of non-negative doubles order like the
+// values).  *out must hold 0 on entry.
+__global__ void k_max_change(const double *__restrict__ f, const double *__restrict__ fp,
+                             int64_t n, unsigned long long *out) {
+  double m = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double p = fp[i], d = 1.0;
+    if (fabs(p) < 1.0e-12) {
+      p = 1.0;
+      d = 0.0;
+    }
+    const double v = fabs(f[i] / p - d);
+    m = (v > m || v != v) ? v : m;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double v = __shfl_xor(m, o, 64);
+    m = (v > m || v != v) ? v : m;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(fabs(m)));
+}
+
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+int launch_max_change(const double *f, const double *fp, int64_t n, unsigned long long *out,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, out, 0ull);
+  hipLaunchKernelGGL(k_max_change, dim3(grid_for(n, 256, 2048)), dim3(256), 0, st, f, fp, n, out);
+  SNAP_HIP_TRY(hipGetLastError());
+  return SNAP_OK;
+}
+
 // Sum of the eight octant partials in octant order 0..7.
 __global__ void k_octant_reduce(const double *__restrict__ part, double *__restrict__ out,
                                 const double *__restrict__ base, int64_t n) {

@@ -1,0 +1,762 @@
+// snap_sweep_seq.hip -- the mini-KBA sweep with one wavefront per tile
+// (reference: MiniKBATask, sweep.cc:242-616 CPU, gpu_sweep.cu:1395-1742 GPU).
+//
+// Decomposition.  As in snap_sweep.hip, a chunk is cut into tiles and every
+// launch covers one tile diagonal of every (octant, group) sweep of the job,
+// so launch order is dependency order.  Here a tile is 4 x 4 x TZ cells and
+// belongs to ONE wavefront, which walks its z planes in sweep order: every
+// cell's upstream neighbours inside the tile were done by the same wavefront
+// before it, so no barrier of any kind is needed.  The tile's internal faces
+// never leave registers: the x inflows are one carry per row, the y inflows
+// a 4-entry pencil and the z inflows a 16-entry plane, all indexed at compile
+// time (the loops over a plane are unrolled).  Faces reach HBM only at tile
+// entry and exit.  Lanes carry angles (up to 64 per octant, one per lane), so
+// the fixup's negative count is a wavefront popcount.
+//
+// Full tiles of the fused and FP32 sweeps take the fast path, per z plane:
+//  1. the angular sources of the plane's 16 cells, psi[cell][a] =
+//     sum_l q[cell][l] ec[l][a], as FP64 matrix products
+//     (v_mfma_f64_16x16x4f64) into LDS;
+//  2. the diamond-difference update of all 16 cells with no branch at all,
+//     the negative-outflow tests of every cell OR-ed into one flag, so the
+//     scheduler overlaps independent cells (those on one i+j diagonal);
+//  3. only when some outflow of the plane came out negative, the plane is
+//     swept again cell by cell with the reference's fixup loop (from the
+//     plane's saved z inflows), which is rare;
+//  4. the flux moments phi[l][cell] = sum_a ec[l][a] (w[a] pc[cell][a]) as
+//     FP64 matrix products.
+// The EXACT form (reference operation order, separate roundings, bit-equal
+// to the CPU arithmetic) and ragged tiles take the general path: per cell,
+// the source expansion on the vector units in the reference's order and the
+// fixup test right after the cell (sweep.cc:331-591).
+#include <algorithm>
+#include <cstring>
+#include <type_traits>
+
+#include "snap_internal.h"
+
+namespace snap {
+
+namespace {
+
+constexpr int kSX = 4, kSY = 4, kNP = kSX * kSY;  // tile cross-section, cells per z plane
+#ifndef SEQ_WAVES
+#define SEQ_WAVES 2  // waves per SIMD the register allocation aims for
+#endif
+// Wavefronts per workgroup: each takes its own tile of the same sweep, so
+// they share the sweep's matrix-core operand tables in LDS.
+constexpr int kWGW = 4;
+
+// a*b + c with one rounding in the sweep's type
+__device__ inline double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ inline float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// lane n's value for the whole wavefront (n is a constant once the plane's
+// loops are unrolled, so this is one or two v_readlane)
+__device__ inline float bcast(float v, int n) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), n));
+}
+__device__ inline double bcast(double v, int n) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, n);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), n);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+struct SeqArgs {
+  Extent3 ext;
+  int TZ, nTx, nTy, nTz, ntiles;
+  int nang, cmom, ng_job, corner0, fixup;
+  int rs;               // LDS row stride (doubles) of the per-plane [cell][angle] tables
+  double hi, hj, hk;
+  const double *mu, *eta, *xi, *w, *ec;
+  const double *q;      // [ng_job][cells][cmom]
+  const void *dinv;     // [ng_job][cells][nang] of Real
+  const void *txs;      // [ng_job][cells] of Real
+  double *part_flux;    // [8][ng_job][cells]
+  double *part_fluxm;   // [8][ng_job][cells][cmom-1]
+  void *faces;          // [nsweeps][ntiles][x|y|z face slots][nang] of Real
+  const int *plane_tiles;
+};
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// chunk cell of tile-local (i, j, k) in sweep order, or -1 outside a
+// ragged chunk edge: the tile origin's cell plus signed strides (a sweep in
+// the -x direction steps through the cells backwards)
+struct TileGeom {
+  int x0, y0, z0, ex, ey, ez, nx, ny, nz;
+  int c0, sx, sy, sz;
+  __device__ inline void init(int corner) {
+    const bool sxp = (corner & 1) != 0, syp = (corner & 2) != 0, szp = (corner & 4) != 0;
+    const int px = sxp ? x0 : nx - 1 - x0, py = syp ? y0 : ny - 1 - y0, pz = szp ? z0 : nz - 1 - z0;
+    c0 = (pz * ny + py) * nx + px;
+    sx = sxp ? 1 : -1;
+    sy = syp ? nx : -nx;
+    sz = szp ? nx * ny : -nx * ny;
+  }
+  __device__ inline int cell(int i, int j, int k) const {
+    if (i >= ex || j >= ey || k >= ez) return -1;
+    return c0 + i * sx + j * sy + k * sz;
+  }
+};
+
+}  // namespace
+
+// CMOM: flux moments (1, 4, 9 or 16), a compile-time count so that every loop
+// over moments unrolls with register operands at fixed places.
+template <typename Real, bool EXACT, bool REC, int CMOM>
+__global__ void __launch_bounds__(64 * kWGW) __attribute__((amdgpu_waves_per_eu(SEQ_WAVES, 8)))
+k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
+  constexpr bool kFP64 = std::is_same<Real, double>::value;
+  constexpr int KS = (CMOM + 3) / 4;  // four-moment steps of the source product
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int RS = A.rs;
+  const int nang = A.nang;
+  const int NB = (nang + 15) >> 4;  // 16-angle blocks of the source product
+  const int NST = (nang + 3) >> 2;  // four-angle steps of the moment product
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int s = blockIdx.y;
+  const int corner = A.corner0 + s / A.ng_job, g = s % A.ng_job;
+  // the sweep's operand tables, shared by the workgroup's wavefronts:
+  // EB[nb][kk][lane] = ec[4kk + lane/16][16nb + lane%16] (the source
+  // product's B, ec[0] = 1) and EA[st][lane] = ec[lane%16][4st + lane/16]
+  // (the moment product's A), zero outside the moments and angles
+  double *EB = (double *)smem_raw;
+  double *EA = EB + (size_t)NB * KS * 64;
+  {
+    const double *ecc0 = A.ec + (size_t)(A.corner0 + s / A.ng_job) * CMOM * nang;
+    for (int idx = threadIdx.x; idx < NB * KS * 64; idx += 64 * kWGW) {
+      const int L = idx & 63, blk = idx >> 6, kk = blk % KS, nb = blk / KS;
+      const int l = 4 * kk + (L >> 4), ang = 16 * nb + (L & 15);
+      EB[idx] = (l < CMOM && ang < nang) ? ((l == 0) ? 1.0 : ecc0[(size_t)l * nang + ang]) : 0.0;
+    }
+    for (int idx = threadIdx.x; idx < NST * 64; idx += 64 * kWGW) {
+      const int L = idx & 63, st = idx >> 6;
+      const int l = L & 15, ang = 4 * st + (L >> 4);
+      EA[idx] = (l < CMOM && ang < nang) ? ((l == 0) ? 1.0 : ecc0[(size_t)l * nang + ang]) : 0.0;
+    }
+  }
+  __syncthreads();  // the only workgroup barrier: before any wavefront can leave
+  const int tix = blockIdx.x * kWGW + wave;
+  if (tix >= plane_count) return;
+  // per wavefront: W: w*pc of the plane's cells [slot][angle]; P: their
+  // angular sources (fast path) or their source moments [slot][16] (general)
+  double *W = EA + (size_t)NST * 64 +
+              (size_t)wave * (kNP * RS + 64 + (kNP * RS > kNP * 16 ? kNP * RS : kNP * 16));
+  double *P = W + kNP * RS + 64;
+  Real *qs = (Real *)P;
+  const int tp = A.plane_tiles[plane_off + tix];
+  const int tx = tp & 1023, ty = (tp >> 10) & 1023, tz = (tp >> 20) & 1023;
+  constexpr int cmom = CMOM;
+  const int TZ = A.TZ;
+  TileGeom T;
+  T.nx = A.ext.nx;
+  T.ny = A.ext.ny;
+  T.nz = A.ext.nz;
+  T.x0 = tx * kSX;
+  T.y0 = ty * kSY;
+  T.z0 = tz * TZ;
+  T.ex = min(kSX, T.nx - T.x0);
+  T.ey = min(kSY, T.ny - T.y0);
+  T.ez = min(TZ, T.nz - T.z0);
+  T.init(corner);
+  const size_t gstride = (size_t)T.nx * T.ny * T.nz;
+
+  // tile faces: this tile's outflow regions and its upstream tiles' ones
+  const int fxn = kSY * TZ * nang, fyn = kSX * TZ * nang, fzn = kSX * kSY * nang;
+  const int per_tile = fxn + fyn + fzn;
+  const int tile_id = (tz * A.nTy + ty) * A.nTx + tx;
+  Real *faces_s = (Real *)A.faces + (size_t)s * A.ntiles * per_tile;
+  Real *out_x = faces_s + (size_t)tile_id * per_tile;
+  Real *out_y = out_x + fxn;
+  Real *out_z = out_y + fyn;
+  const Real *in_x = tx > 0 ? faces_s + (size_t)(tile_id - 1) * per_tile : nullptr;
+  const Real *in_y = ty > 0 ? faces_s + (size_t)(tile_id - A.nTx) * per_tile + fxn : nullptr;
+  const Real *in_z =
+      tz > 0 ? faces_s + (size_t)(tile_id - A.nTx * A.nTy) * per_tile + fxn + fyn : nullptr;
+
+  const double *qg = A.q + (size_t)g * gstride * cmom;
+  const Real *dg = (const Real *)A.dinv + (size_t)g * gstride * nang;
+  const Real *tg = (const Real *)A.txs + (size_t)g * gstride;
+  double *pfg = A.part_flux + ((size_t)corner * A.ng_job + g) * gstride;
+  double *pmg = A.part_fluxm + ((size_t)corner * A.ng_job + g) * gstride * (size_t)(cmom - 1);
+
+  // ---- per-lane constants (lane = angle; lanes past nang carry angle 0
+  // with zero weight and never store)
+  const bool aok = lane < nang;
+  const int a = aok ? lane : 0;
+  const double mu_d = A.mu[a], eta_d = A.eta[a], xi_d = A.xi[a];
+  const double w_d = aok ? A.w[a] : 0.0;
+  // EXACT keeps (psi*mu)*hi as the reference writes it; the others fold the
+  // spacing into the direction cosine
+  const Real cx = EXACT ? (Real)mu_d : (Real)(mu_d * A.hi);
+  const Real cy = EXACT ? (Real)eta_d : (Real)(eta_d * A.hj);
+  const Real cz = EXACT ? (Real)xi_d : (Real)(xi_d * A.hk);
+  const Real hi = (Real)A.hi, hj = (Real)A.hj, hk = (Real)A.hk;
+  const double *ecc = A.ec + (size_t)corner * cmom * nang;
+  // dinv formed in the kernel: CalculateGeometryParam's sum (expxs.cc:378-381)
+  const double dsum_x = A.hi * mu_d, dsum_y = A.hj * eta_d, dsum_z = A.hk * xi_d;
+  const float gsum32 = (float)((A.hi * mu_d + A.hj * eta_d) + A.hk * xi_d);
+  const Real tolr = (Real)1.0e-12;
+  auto dinv_of = [&](Real t) -> Real {
+    if constexpr (kFP64 && EXACT)
+      return 1.0 / ((((double)t + dsum_x) + dsum_y) + dsum_z);
+    else if constexpr (kFP64) {
+      // the fused form: a hardware reciprocal refined by two Newton steps
+      // (within an ulp or two of the division)
+      const double den = (((double)t + (double)cx) + (double)cy) + (double)cz;
+      double r = __builtin_amdgcn_rcp(den);
+      double e = __builtin_fma(-den, r, 1.0);
+      r = __builtin_fma(r, e, r);
+      e = __builtin_fma(-den, r, 1.0);
+      return __builtin_fma(r, e, r);
+    }
+    else {
+      // a hardware reciprocal refined by one Newton step
+      const float den = (float)t + gsum32;
+      const float r = __builtin_amdgcn_rcpf(den);
+      return __builtin_fmaf(r, __builtin_fmaf(-den, r, 1.0f), r);
+    }
+  };
+  const int r16 = lane & 15, q4 = lane >> 4;  // matrix-core operand coordinates
+  // W slot of the lane in row n: its angle's column, or (lanes past the
+  // row) its own slot of a junk row after the 16 rows, which nothing reads
+  auto wslot = [&](int n) { return lane < RS ? n * RS + lane : kNP * RS + lane; };
+
+  // ---- one cell of the general path: DD update, the reference's fixup loop
+  // (sweep.cc:402-520) and the outflows; returns pc
+  auto cell_step = [&](Real psi, Real d, Real pi_, Real pj_, Real pk_, int c, Real &ox, Real &oy,
+                       Real &oz) -> Real {
+    Real pc;
+    if constexpr (EXACT) {
+      Real p = psi;
+      p += pi_ * cx * hi;
+      p += pj_ * cy * hj;
+      p += pk_ * cz * hk;
+      pc = p * d;
+    } else {
+      Real p = fmar(pi_, cx, psi);
+      p = fmar(pj_, cy, p);
+      p = fmar(pk_, cz, p);
+      pc = p * d;
+    }
+    bool hx = true, hy = true, hz = true;
+    if (A.fixup) {
+      // lanes past nang hold angle 0's values and flags (they mirror lane 0
+      // exactly) but are not counted
+      auto count_negative = [&]() -> unsigned {
+        const Real p2 = Real(2) * pc;
+        const bool nxn = p2 - pi_ < Real(0);
+        const bool nyn = p2 - pj_ < Real(0);
+        const bool nzn = p2 - pk_ < Real(0);
+        hx = hx & !nxn;
+        hy = hy & !nyn;
+        hz = hz & !nzn;
+        return (unsigned)__popcll(__builtin_amdgcn_ballot_w64(aok & nxn)) +
+               (unsigned)__popcll(__builtin_amdgcn_ballot_w64(aok & nyn)) +
+               (unsigned)__popcll(__builtin_amdgcn_ballot_w64(aok & nzn));
+      };
+      unsigned old_neg = count_negative();
+      if (old_neg != 0) {
+        const Real txs = tg[c];
+        // a pass that clears no new flag leaves pc unchanged, so the pass
+        // after it repeats the count: 3*nang+2 passes always reach the break
+        for (int it = 1; it < 3 * nang + 2; ++it) {
+          const Real vx = hx ? Real(1) : Real(0), vy = hy ? Real(1) : Real(0),
+                     vz = hz ? Real(1) : Real(0);
+          Real den, p;
+          if constexpr (EXACT) {
+            p = psi + 0.5 * (pi_ * cx * hi * (1.0 + vx) + pj_ * cy * hj * (1.0 + vy) +
+                             pk_ * cz * hk * (1.0 + vz));
+            den = (p <= 0.0) ? 0.0 : (txs + cx * hi * vx + cy * hj * vy + cz * hk * vz);
+          } else {
+            const Real t = pi_ * cx * (Real(1) + vx) + pj_ * cy * (Real(1) + vy) +
+                           pk_ * cz * (Real(1) + vz);
+            p = psi + Real(0.5) * t;
+            den = (p <= Real(0)) ? Real(0) : (txs + cx * vx + cy * vy + cz * vz);
+          }
+          pc = (den < tolr) ? Real(0) : p / den;
+          const unsigned neg = count_negative();
+          if (neg == old_neg) break;
+          old_neg = neg;
+        }
+      }
+    }
+    const Real p2 = Real(2) * pc;
+    ox = (p2 - pi_) * (hx ? Real(1) : Real(0));
+    oy = (p2 - pj_) * (hy ? Real(1) : Real(0));
+    oz = (p2 - pk_) * (hz ? Real(1) : Real(0));
+    return pc;
+  };
+
+  // ---- flux moments of plane k from W (sweep.cc:568-591)
+  auto moments = [&](int k) {
+    if constexpr (EXACT) {
+      // lane -> cell slot lane % 16, moments lane / 16 + 4 t, each summed in
+      // ascending angle order with separate roundings
+      const int n = r16;
+      const int c = T.cell(n & 3, n >> 2, k);
+      if (c >= 0) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const int l = q4 + 4 * t;
+          if (l >= cmom) break;
+          double sum = 0.0;
+          if (l == 0) {
+            for (int b = 0; b < nang; b++) sum = sum + W[n * RS + b];
+            pfg[c] = sum;
+          } else {
+            const double *e = ecc + (size_t)l * nang;
+            for (int b = 0; b < nang; b++) sum = sum + e[b] * W[n * RS + b];
+            pmg[(size_t)c * (cmom - 1) + (l - 1)] = sum;
+          }
+        }
+      }
+    } else {
+      // phi[l][cell] over four angles per product: A[l][kk] = ec[l][4s+kk]
+      // (ec[0] = 1) in lane l + 16 kk, B[kk][cell] = w*pc in lane cell + 16 kk,
+      // D[l][cell] in lane cell + 16 (l % 4), register l / 4
+      // two accumulators halve the dependent-product chain
+      f64x4 acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
+      int st = 0;
+      for (; st + 1 < NST; st += 2) {
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(EA[st * 64 + lane], W[r16 * RS + 4 * st + q4],
+                                                   acc, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(EA[(st + 1) * 64 + lane],
+                                                    W[r16 * RS + 4 * st + 4 + q4], acc2, 0, 0, 0);
+      }
+      if (st < NST)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(EA[st * 64 + lane], W[r16 * RS + 4 * st + q4],
+                                                   acc, 0, 0, 0);
+      acc += acc2;
+      const int n = r16;
+      const int c = T.cell(n & 3, n >> 2, k);
+      if (c >= 0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int l = 4 * r + q4;
+          if (l >= cmom) break;
+          if (l == 0)
+            pfg[c] = acc[r];
+          else
+            pmg[(size_t)c * (cmom - 1) + (l - 1)] = acc[r];
+        }
+      }
+    }
+  };
+
+  // ---- faces inside the tile
+  Real zf[kNP], yf[kSX], xf[kSY];
+#pragma unroll
+  for (int n = 0; n < kNP; n++) {
+    const int i = n & 3, j = n >> 2;
+    zf[n] = (in_z && i < T.ex && j < T.ey) ? in_z[(size_t)(j * kSX + i) * nang + a] : Real(0);
+  }
+#pragma unroll
+  for (int i = 0; i < kSX; i++) yf[i] = Real(0);
+#pragma unroll
+  for (int j = 0; j < kSY; j++) xf[j] = Real(0);
+  // upstream faces of a vacuum boundary read this tile's own (valid) face
+  // memory and are replaced by zeros: loads without branches
+  const bool has_x = in_x != nullptr, has_y = in_y != nullptr;
+  const Real *in_x_ld = has_x ? in_x : out_x;
+  const Real *in_y_ld = has_y ? in_y : out_y;
+  auto load_xy_inflows = [&](int k, Real(&xin)[kSY], Real(&yin)[kSX]) {
+#pragma unroll
+    for (int j = 0; j < kSY; j++) {
+      const Real v = in_x_ld[(size_t)(k * kSY + min(j, T.ey - 1)) * nang + a];
+      xin[j] = (has_x && j < T.ey) ? v : Real(0);
+    }
+#pragma unroll
+    for (int i = 0; i < kSX; i++) {
+      const Real v = in_y_ld[(size_t)(k * kSX + min(i, T.ex - 1)) * nang + a];
+      yin[i] = (has_y && i < T.ex) ? v : Real(0);
+    }
+  };
+
+  const bool fast = !EXACT && T.ex == kSX && T.ey == kSY;
+  if (fast) {
+    // ================= fast path: full tile, fused or FP32 =================
+    // source operands of plane k: lane L holds q[slot L%16][4 kk + L/16]
+    double qa[KS];
+    auto qa_load = [&](int k) {
+      const int c = T.c0 + (r16 & 3) * T.sx + (r16 >> 2) * T.sy + k * T.sz;
+#pragma unroll
+      for (int kk = 0; kk < KS; kk++) {
+        const int l = 4 * kk + q4;
+        qa[kk] = (l < CMOM) ? qg[(size_t)c * cmom + l] : 0.0;
+      }
+    };
+    // t_xs of slot L%16 in lane L (read by the cells with a lane broadcast)
+    Real tv = Real(0);
+    auto tv_load = [&](int k) {
+      const int c = T.c0 + (r16 & 3) * T.sx + (r16 >> 2) * T.sy + k * T.sz;
+      tv = tg[c];
+    };
+    Real dv[kNP];
+    auto dv_load = [&](int k) {
+#pragma unroll
+      for (int n = 0; n < kNP; n++) {
+        if constexpr (!REC)
+          dv[n] = dg[(size_t)(T.c0 + (n & 3) * T.sx + (n >> 2) * T.sy + k * T.sz) * nang + a];
+        else
+          dv[n] = Real(0);
+      }
+    };
+    Real xin[kSY], yin[kSX];
+    qa_load(0);
+    tv_load(0);
+    dv_load(0);
+    load_xy_inflows(0, xin, yin);
+    for (int k = 0; k < T.ez; k++) {
+      // (1) psi[slot][angle] = sum_l q[slot][l] ec[l][angle] for each block of
+      // 16 angles: A = q (cells x moments), B = ec (moments x angles), D[cell]
+      // [angle] in lane angle + 16 (cell % 4), register cell / 4
+      for (int nb = 0; nb < NB; nb++) {
+        f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+        const int ang = 16 * nb + r16;
+#pragma unroll
+        for (int kk = 0; kk < KS; kk++)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[kk], EB[(nb * KS + kk) * 64 + lane], acc,
+                                                     0, 0, 0);
+        if (ang < RS) {
+#pragma unroll
+          for (int r = 0; r < 4; r++) P[(4 * r + q4) * RS + ang] = acc[r];
+        }
+      }
+      // the next plane's operands, in flight while this plane is swept
+      Real xin_n[kSY], yin_n[kSX], dv_c[kNP];
+      const Real tv_c = tv;
+#pragma unroll
+      for (int n = 0; n < kNP; n++) dv_c[n] = dv[n];
+#pragma unroll
+      for (int j = 0; j < kSY; j++) xin_n[j] = Real(0);
+#pragma unroll
+      for (int i = 0; i < kSX; i++) yin_n[i] = Real(0);
+      if (k + 1 < T.ez) {
+        qa_load(k + 1);
+        tv_load(k + 1);
+        dv_load(k + 1);
+        load_xy_inflows(k + 1, xin_n, yin_n);
+      }
+      wave_sync();
+
+      // (2) every cell of the plane without a branch, in i+j diagonal order
+      // with a scheduling barrier between diagonals, so that at most four
+      // independent cells are in flight (the z inflows are kept in LDS for
+      // step 3)
+      if (A.fixup) {
+        // the tile's own z outflow face holds the plane's z inflows until the
+        // tile's last plane writes its real outflows there
+#pragma unroll
+        for (int n = 0; n < kNP; n++) out_z[(size_t)n * nang + a] = zf[n];
+      }
+      bool neg = false;
+#pragma unroll
+      for (int dg2 = 0; dg2 < kSX + kSY - 1; dg2++) {
+#pragma unroll
+        for (int i = 0; i < kSX; i++) {
+          const int j = dg2 - i;
+          if (j < 0 || j >= kSY) continue;  // resolved at compile time
+          const int n = j * kSX + i;
+          const Real psi = (Real)P[n * RS + a];
+          Real d;
+          if constexpr (REC)
+            d = dinv_of(bcast(tv_c, n));
+          else
+            d = dv_c[n];
+          const Real pi_ = (i == 0) ? xin[j] : xf[j];
+          const Real pj_ = (j == 0) ? yin[i] : yf[i];
+          const Real pk_ = zf[n];
+          Real p = fmar(pi_, cx, psi);
+          p = fmar(pj_, cy, p);
+          p = fmar(pk_, cz, p);
+          const Real pc = p * d;
+          const Real p2 = Real(2) * pc;
+          const Real ox = p2 - pi_, oy = p2 - pj_, oz = p2 - pk_;
+          neg = neg | (ox < Real(0)) | (oy < Real(0)) | (oz < Real(0));
+          xf[j] = ox;
+          yf[i] = oy;
+          zf[n] = oz;
+          // lanes past nang mirror lane 0 and store its values again
+          if (i == kSX - 1) out_x[(size_t)(k * kSY + j) * nang + a] = ox;
+          if (j == kSY - 1) out_y[(size_t)(k * kSX + i) * nang + a] = oy;
+          W[wslot(n)] = aok ? w_d * (double)pc : 0.0;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      // (3) a negative outflow anywhere in the plane: sweep it again cell by
+      // cell with the fixup, from the plane's z inflows (zf) and tile inflows
+      if (A.fixup && __builtin_amdgcn_ballot_w64(aok & neg) != 0) {
+        // A rolled loop, so that this rarer path does not set the register
+        // budget: the y inflows of a row are the outflows of four cells
+        // earlier (a four-entry queue), the z inflows come from the backup
+        // in the tile's z face and each z outflow overwrites its own, now
+        // consumed, backup slot; tile inflows and dinv are read again.
+        Real xc = Real(0), yq0 = Real(0), yq1 = Real(0), yq2 = Real(0), yq3 = Real(0);
+#pragma unroll 1
+        for (int n = 0; n < kNP; n++) {
+          const int i = n & 3, j = n >> 2;
+          const int c = T.c0 + i * T.sx + j * T.sy + k * T.sz;
+          const Real psi = (Real)P[n * RS + a];
+          Real d;
+          if constexpr (REC)
+            d = dinv_of(bcast(tv_c, n));
+          else
+            d = dg[(size_t)c * nang + a];
+          Real pi_ = xc, pj_ = yq0;
+          if (i == 0) {
+            const Real v = in_x_ld[(size_t)(k * kSY + j) * nang + a];
+            pi_ = has_x ? v : Real(0);
+          }
+          if (j == 0) {
+            const Real v = in_y_ld[(size_t)(k * kSX + i) * nang + a];
+            pj_ = has_y ? v : Real(0);
+          }
+          Real ox, oy, oz;
+          const Real pc = cell_step(psi, d, pi_, pj_, out_z[(size_t)n * nang + a], c, ox, oy, oz);
+          xc = ox;
+          yq0 = yq1;
+          yq1 = yq2;
+          yq2 = yq3;
+          yq3 = oy;
+          out_z[(size_t)n * nang + a] = oz;
+          if (i == kSX - 1) out_x[(size_t)(k * kSY + j) * nang + a] = ox;
+          if (j == kSY - 1) out_y[(size_t)(k * kSX + i) * nang + a] = oy;
+          W[wslot(n)] = aok ? w_d * (double)pc : 0.0;
+        }
+#pragma unroll
+        for (int n = 0; n < kNP; n++) zf[n] = out_z[(size_t)n * nang + a];
+      }
+      wave_sync();
+      // (4) flux moments
+      moments(k);
+      wave_sync();
+#pragma unroll
+      for (int j = 0; j < kSY; j++) xin[j] = xin_n[j];
+#pragma unroll
+      for (int i = 0; i < kSX; i++) yin[i] = yin_n[i];
+    }
+  } else {
+    // ================= general path: exact form, ragged tiles ==============
+    Real ecl[CMOM];  // ecl[l] = ec[l][a], l >= 1
+#pragma unroll
+    for (int l = 1; l < CMOM; l++) ecl[l] = (Real)ecc[(size_t)l * nang + a];
+    // source moments of plane k in LDS: 16 cells x cmom values, four per
+    // lane (the loads of the next plane are issued while the current one is
+    // swept, and stored once its moments are done)
+    double qn[4];
+    auto q_load = [&](int k) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int idx = lane + 64 * r;  // = slot * 16 + l
+        const int slot = idx >> 4, l = idx & 15;
+        const int c = (k < T.ez && l < cmom) ? T.cell(slot & 3, slot >> 2, k) : -1;
+        qn[r] = c >= 0 ? qg[(size_t)c * cmom + l] : 0.0;
+      }
+    };
+    auto q_store = [&]() {
+#pragma unroll
+      for (int r = 0; r < 4; r++) qs[lane + 64 * r] = (Real)qn[r];
+    };
+    q_load(0);
+    q_store();
+    for (int k = 0; k < T.ez; k++) {
+      Real xin[kSY], yin[kSX];
+      load_xy_inflows(k, xin, yin);
+      if (k + 1 < T.ez) q_load(k + 1);
+      wave_sync();
+#pragma unroll
+      for (int n = 0; n < kNP; n++) {
+        const int i = n & 3, j = n >> 2;
+        if (i >= T.ex || j >= T.ey) continue;  // uniform: outside a ragged edge
+        const int c = T.cell(i, j, k);
+        const Real *q = qs + n * 16;
+        // angular source (sweep.cc:331-345)
+        Real psi;
+        if constexpr (EXACT) {
+          double p = q[0];
+#pragma unroll
+          for (int l = 1; l < CMOM; l++) p += (double)ecl[l] * (double)q[l];
+          psi = (Real)p;
+        } else {
+          // two independent chains halve the dependent-FMA latency
+          Real p0 = q[0], p1 = Real(0);
+#pragma unroll
+          for (int l = 1; l < CMOM; l++) {
+            if (l & 1)
+              p0 = fmar(ecl[l], q[l], p0);
+            else
+              p1 = fmar(ecl[l], q[l], p1);
+          }
+          psi = CMOM > 2 ? p0 + p1 : p0;
+        }
+        const Real d = REC ? dinv_of(tg[c]) : dg[(size_t)c * nang + a];
+        const Real pi_ = (i == 0) ? xin[j] : xf[j];
+        const Real pj_ = (j == 0) ? yin[i] : yf[i];
+        Real ox, oy, oz;
+        const Real pc = cell_step(psi, d, pi_, pj_, zf[n], c, ox, oy, oz);
+        xf[j] = ox;
+        yf[i] = oy;
+        zf[n] = oz;
+        if (aok) {
+          if (i == T.ex - 1) out_x[(size_t)(k * kSY + j) * nang + a] = ox;
+          if (j == T.ey - 1) out_y[(size_t)(k * kSX + i) * nang + a] = oy;
+        }
+        W[wslot(n)] = aok ? w_d * (double)pc : 0.0;
+      }
+      wave_sync();
+      moments(k);
+      wave_sync();
+      if (k + 1 < T.ez) {
+        q_store();
+        wave_sync();
+      }
+    }
+  }
+  // ---- the tile's z outflow face
+  if (aok) {
+#pragma unroll
+    for (int n = 0; n < kNP; n++) {
+      const int i = n & 3, j = n >> 2;
+      if (i < T.ex && j < T.ey) out_z[(size_t)(j * kSX + i) * nang + a] = zf[n];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// plan and launch
+// ---------------------------------------------------------------------------
+namespace {
+// LDS row stride of the [cell][angle] tables: the angles rounded up to the
+// four-angle matrix steps, plus one so the 16 rows start on different banks
+int seq_row_stride(int nang) { return ((nang + 3) & ~3) + 1; }
+size_t seq_lds_bytes(int nang, int cmom) {
+  const int rs = seq_row_stride(nang);
+  const size_t tables = (size_t)((nang + 15) / 16) * ((cmom + 3) / 4) * 64 + (size_t)((nang + 3) / 4) * 64;
+  const size_t per_wave = (size_t)kNP * rs + 64 + std::max(kNP * rs, kNP * 16);
+  return sizeof(double) * (tables + kWGW * per_wave);
+}
+}  // namespace
+
+int plan_create_seq(SweepPlan *p, Extent3 e, int nang, int cmom, int TZ) {
+  plan_free(p);
+  if (e.nx < 1 || e.ny < 1 || e.nz < 1) return fail(SNAP_ERR_INVALID, "empty chunk");
+  if (nang < 1 || nang > 64)
+    return fail(SNAP_ERR_UNSUPPORTED, "the one-wavefront tile sweep handles 1..64 angles, got %d",
+                nang);
+  if (cmom != 1 && cmom != 4 && cmom != 9 && cmom != 16)
+    return fail(SNAP_ERR_INVALID, "cmom must be 1, 4, 9 or 16, got %d", cmom);
+  if (TZ < 1 || TZ > 1023) return fail(SNAP_ERR_INVALID, "tile depth must be 1..1023");
+  p->ext = e;
+  p->seq = true;
+  p->TX = kSX;
+  p->TY = kSY;
+  p->TZ = std::min(TZ, e.nz);
+  p->nTx = (e.nx + kSX - 1) / kSX;
+  p->nTy = (e.ny + kSY - 1) / kSY;
+  p->nTz = (e.nz + p->TZ - 1) / p->TZ;
+  if (p->nTx > 1023 || p->nTy > 1023 || p->nTz > 1023)
+    return fail(SNAP_ERR_INVALID, "too many tiles along one axis");
+  p->ntiles = p->nTx * p->nTy * p->nTz;
+  p->nang = nang;
+  p->cmom = cmom;
+  p->waves = 1;
+  p->apl = 1;
+  p->lpc = nang;
+  p->cpw = 1;
+  p->nplanes = p->nTx + p->nTy + p->nTz - 2;
+  std::vector<int> tiles;
+  p->plane_off.assign(p->nplanes + 1, 0);
+  for (int D = 0; D < p->nplanes; D++) {
+    p->plane_off[D] = (int)tiles.size();
+    for (int tz = 0; tz < p->nTz; tz++)
+      for (int ty = 0; ty < p->nTy; ty++) {
+        const int tx = D - ty - tz;
+        if (tx >= 0 && tx < p->nTx) tiles.push_back(tx | (ty << 10) | (tz << 20));
+      }
+  }
+  p->plane_off[p->nplanes] = (int)tiles.size();
+  SNAP_HIP_TRY(hipMalloc((void **)&p->plane_tiles, sizeof(int) * tiles.size()));
+  SNAP_HIP_TRY(hipMemcpy(p->plane_tiles, tiles.data(), sizeof(int) * tiles.size(),
+                         hipMemcpyHostToDevice));
+  return SNAP_OK;
+}
+
+typedef void (*SeqKernel)(SeqArgs, int, int);
+
+template <typename Real, bool EXACT>
+static SeqKernel seq_kernel(int cmom, bool rec) {
+  switch (cmom) {
+    case 1: return rec ? k_sweep_seq<Real, EXACT, true, 1> : k_sweep_seq<Real, EXACT, false, 1>;
+    case 4: return rec ? k_sweep_seq<Real, EXACT, true, 4> : k_sweep_seq<Real, EXACT, false, 4>;
+    case 9: return rec ? k_sweep_seq<Real, EXACT, true, 9> : k_sweep_seq<Real, EXACT, false, 9>;
+    case 16: return rec ? k_sweep_seq<Real, EXACT, true, 16> : k_sweep_seq<Real, EXACT, false, 16>;
+    default: return nullptr;
+  }
+}
+
+int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
+  const SweepPlan &p = *L.plan;
+  if (!p.seq) return fail(SNAP_ERR_INVALID, "not a one-wavefront tile plan");
+  if (L.general || L.batch || L.td)
+    return fail(SNAP_ERR_INVALID, "the one-wavefront tile sweep is dense and steady");
+  if (!L.q || !L.txs_dense || !L.part_flux || (p.cmom > 1 && !L.part_fluxm) ||
+      (!L.recompute_dinv && !L.dinv_dense))
+    return fail(SNAP_ERR_INVALID, "dense sweep inputs missing");
+  if (L.nsweeps < 1 || L.nsweeps > 65535) return fail(SNAP_ERR_INVALID, "bad sweep count");
+  SeqArgs A;
+  memset(&A, 0, sizeof(A));
+  A.ext = p.ext;
+  A.TZ = p.TZ;
+  A.nTx = p.nTx;
+  A.nTy = p.nTy;
+  A.nTz = p.nTz;
+  A.ntiles = p.ntiles;
+  A.nang = p.nang;
+  A.cmom = p.cmom;
+  A.rs = seq_row_stride(p.nang);
+  A.ng_job = L.ng_job;
+  A.corner0 = L.corner0;
+  A.fixup = L.fixup;
+  A.hi = L.hi;
+  A.hj = L.hj;
+  A.hk = L.hk;
+  A.mu = L.quad->mu;
+  A.eta = L.quad->eta;
+  A.xi = L.quad->xi;
+  A.w = L.quad->w;
+  A.ec = L.quad->ec;
+  A.q = L.q;
+  A.dinv = L.dinv_dense;
+  A.txs = L.txs_dense;
+  A.part_flux = L.part_flux;
+  A.part_fluxm = L.part_fluxm;
+  A.faces = L.faces;
+  A.plane_tiles = p.plane_tiles;
+  const SeqKernel k = L.fp32 ? seq_kernel<float, false>(p.cmom, L.recompute_dinv)
+                   : L.exact ? seq_kernel<double, true>(p.cmom, L.recompute_dinv)
+                             : seq_kernel<double, false>(p.cmom, L.recompute_dinv);
+  if (!k) return fail(SNAP_ERR_UNSUPPORTED, "no one-wavefront tile sweep for %d moments", p.cmom);
+  const size_t lds = seq_lds_bytes(p.nang, p.cmom);
+  SNAP_TRY(ensure_dynamic_lds((const void *)k, lds));
+  for (int D = 0; D < p.nplanes; D++) {
+    const int count = p.plane_off[D + 1] - p.plane_off[D];
+    if (count == 0) continue;
+    if (events) SNAP_HIP_TRY(hipEventRecord(events[2 * D], st));
+    hipLaunchKernelGGL(k, dim3((count + kWGW - 1) / kWGW, L.nsweeps), dim3(64 * kWGW), lds, st, A,
+                       p.plane_off[D], count);
+    if (events) SNAP_HIP_TRY(hipEventRecord(events[2 * D + 1], st));
+  }
+  SNAP_HIP_TRY(hipGetLastError());
+  return SNAP_OK;
+}
+
+}  // namespace snap

@@ -82,6 +82,7 @@ class SolveOptions(ctypes.Structure):
         ("waves_per_tile", ctypes.c_int), ("first_sweep_fp64", ctypes.c_int),
         ("dinv_mode", ctypes.c_int), ("angles_per_lane", ctypes.c_int), ("refine", ctypes.c_int),
         ("outer_ordered", ctypes.c_int), ("wide_sweep", ctypes.c_int),
+        ("sweep_kernel", ctypes.c_int), ("fp64_polish", ctypes.c_double),
     ]
 
 

@@ -60,7 +60,8 @@ class Solver:
     def __init__(self, problem: Problem, precision: str = FP64, tile=(0, 0, 0),
                  waves_per_tile: int = 0, first_sweep_fp64: bool = False,
                  dinv: str = "default", angles_per_lane: int = 0, refine: bool = False,
-                 outer_ordered: bool = False, wide_sweep: bool = False):
+                 outer_ordered: bool = False, wide_sweep: bool = False,
+                 sweep_kernel: str = "auto", fp64_polish: float = 0.0):
         problem.validate()
         self.problem = p = problem.contiguous()
         self.lib = _lib.load()
@@ -76,6 +77,8 @@ class Solver:
         o.refine = int(bool(refine))
         o.outer_ordered = int(bool(outer_ordered))
         o.wide_sweep = int(bool(wide_sweep))
+        o.sweep_kernel = {"auto": 0, "tiled": 1, "wave_tile": 2}[sweep_kernel]
+        o.fp64_polish = float(fp64_polish)
         self.precision = precision
         h = ctypes.c_void_p()
         _lib.check(self.lib.snap_hip_solver_create(ctypes.byref(s), ctypes.byref(o), ctypes.byref(h)),
