@@ -9,12 +9,15 @@ unit source in the central cube, vacuum boundaries, epsi 1e-4.  Inputs are
 resident in HBM before the timed region.
 
 Prints one JSON line.  `value` is the FP64 sweep grind time
-t_sweep / (cells * 8*nang * ng * inner_iterations) in ns (lower is better).
-`roofline` prices the sweep kernel's algorithmic bytes per unit of the
-reference data flow, B64 = 8 + (24*cmom + 8)/nang, against 8 TB/s, with the
-kernel's mean launch duration from HIP events around each launch.
-`cpu_baseline` times the CPU oracle (a restatement of the reference's CPU
-sweep) over one inner iteration's sweeps of the same problem.
+t_sweep / (cells * 8*nang * ng * inner_iterations) in ns (lower is better),
+where t_sweep covers the sweep launches and the sum of the octant partials
+into the flux moments (which the reference does inside its sweep kernel).
+`roofline` prices the algorithmic bytes per unit of the reference data flow,
+B64 = 8 + (24*cmom + 8)/nang, against 8 TB/s over the same t_sweep, timed
+with HIP events on the solver's stream.  `cpu_baseline` times the CPU oracle
+in the operation order of the reference's AVX sweep variant (the one it
+registers on x86 with AVX) over one inner iteration's sweeps of the same
+problem on every host core.
 """
 from __future__ import annotations
 
@@ -85,28 +88,32 @@ def copy_bandwidth_gbs(torch, gib: int = 2, reps: int = 5) -> float:
     return 2 * n * 8 / best / 1e9
 
 
-def cpu_baseline(problem, budget_s: float = 25.0):
+def cpu_baseline(problem, budget_s: float = 25.0, reps: int = 3):
     """Time the CPU oracle on one inner iteration's sweeps (all octants and
-    groups) of `problem` using every host core, or of a reduced group count
-    when the full sweep would exceed the budget (the grind time is per unit)."""
+    groups) of `problem` using every host core (best of `reps` after a
+    warm-up), or of a reduced group count when that would exceed the budget
+    (the grind time is per unit).  The sweeps run in the reference's AVX
+    variant's operation order when nang % 4 == 0, its generic order else."""
     from oracle import oracle
     import dataclasses
+    order = "avx" if problem.nang % 4 == 0 else "generic"
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores) or cores)))
     # probe on 2 groups, then size the sample to the budget
     probe = dataclasses.replace(problem, ng=2, sigt=problem.sigt[:, :2], siga=problem.siga[:, :2],
                                 sigs=problem.sigs[:, :2], slgg=problem.slgg[:2, :, :2, :],
                                 qi=problem.qi[:2])
-    t = oracle.time_sweeps(probe, 1, cores)
+    t = oracle.time_sweeps(probe, 1, cores, order=order)
     per_group = t / 2.0
-    ng = int(max(2, min(problem.ng, budget_s / max(per_group, 1e-9))))
+    ng = int(max(2, min(problem.ng, budget_s / (reps + 1) / max(per_group, 1e-9))))
     sample = dataclasses.replace(problem, ng=ng, sigt=problem.sigt[:, :ng], siga=problem.siga[:, :ng],
                                  sigs=problem.sigs[:, :ng], slgg=problem.slgg[:ng, :, :ng, :],
                                  qi=problem.qi[:ng])
-    t = oracle.time_sweeps(sample, 1, cores)
+    oracle.time_sweeps(sample, 1, cores, order=order)  # warm-up
+    t = min(oracle.time_sweeps(sample, 1, cores, order=order) for _ in range(reps))
     units = sample.ncells * 8 * sample.nang * ng
     # the same sweeps on one core (the two-group probe), and the host CPU
-    t1 = oracle.time_sweeps(probe, 1, 1)
+    t1 = oracle.time_sweeps(probe, 1, 1, order=order)
     CPU_INFO["one_core_grind_ns"] = t1 * 1e9 / (probe.ncells * 8 * probe.nang * probe.ng)
     CPU_INFO["threads"] = cores
     CPU_INFO["model"] = cpu_model()
@@ -116,9 +123,66 @@ def cpu_baseline(problem, budget_s: float = 25.0):
         "cores": cores,
         "kind": "port",
         "sample": (f"one inner iteration's sweeps (8 octants x {ng} of {problem.ng} groups) of the "
-                   f"{problem.nx}^3-cell, {problem.nang}-angle, P{problem.nmom - 1} workload, "
-                   f"{units:.3g} cell-angle-groups in {t:.1f} s"),
+                   f"{problem.nx}^3-cell, {problem.nang}-angle, P{problem.nmom - 1} workload in the "
+                   f"reference's {order.upper() if order == 'avx' else order} variant's operation "
+                   f"order, {units:.3g} cell-angle-groups in {t:.2f} s (best of {reps} after a "
+                   f"warm-up, {cores} threads)"),
     }
+
+
+def rocprof_summary(dirs, kernel_filter: str = "", sweeps_of: str = "",
+                    launches_per_sweep: int = 0) -> str:
+    """Markdown tables from rocprofv3 CSV output under `dirs`: per-kernel
+    call counts and durations (kernel_stats.csv of --stats), and per-kernel
+    counter values (counter_collection.csv of --pmc) as the mean per
+    dispatch.  With sweeps_of (a kernel-name prefix) and launches_per_sweep,
+    also each counter summed over one inner sweep of that kernel (dispatches
+    / launches_per_sweep sweeps)."""
+    import csv
+    import glob
+    out = []
+    for d in dirs:
+        for f in sorted(glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True)):
+            rows = list(csv.DictReader(open(f)))
+            out.append(f"kernel statistics ({os.path.relpath(f, d)})\n")
+            out.append("| kernel | calls | mean (ms) | min (ms) | max (ms) | share |")
+            out.append("|---|---|---|---|---|---|")
+            for r in rows:
+                if kernel_filter and kernel_filter not in r["Name"]:
+                    continue
+                out.append(f"| `{r['Name']}` | {r['Calls']} | {float(r['AverageNs']) / 1e6:.4g} | "
+                           f"{float(r['MinNs']) / 1e6:.4g} | {float(r['MaxNs']) / 1e6:.4g} | "
+                           f"{float(r['Percentage']):.1f}% |")
+            out.append("")
+        per = {}
+        for f in sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)):
+            for r in csv.DictReader(open(f)):
+                name = r["Kernel_Name"]
+                if kernel_filter and kernel_filter not in name:
+                    continue
+                k = per.setdefault(name, {"res": (r["VGPR_Count"], r["Accum_VGPR_Count"],
+                                                  r["SGPR_Count"], r["LDS_Block_Size"],
+                                                  r["Scratch_Size"]), "c": {}})
+                c = k["c"].setdefault(r["Counter_Name"], [0.0, set()])
+                c[0] += float(r["Counter_Value"])
+                c[1].add(r["Dispatch_Id"])
+        if per:
+            out.append(f"counters ({d})\n")
+            out.append("| kernel | VGPR/AGPR/SGPR/LDS/scratch | counter | dispatches | mean per dispatch"
+                       + (" | per inner sweep |" if sweeps_of else " |"))
+            out.append("|---|---|---|---|---|" + ("---|" if sweeps_of else ""))
+            for name, k in sorted(per.items()):
+                for cn, (tot, ids) in sorted(k["c"].items()):
+                    line = (f"| `{name}` | {'/'.join(k['res'])} | {cn} | {len(ids)} | "
+                            f"{tot / max(1, len(ids)):.5g} |")
+                    if sweeps_of:
+                        if name.startswith(sweeps_of) and launches_per_sweep:
+                            line += f" {tot / (len(ids) / launches_per_sweep):.5g} |"
+                        else:
+                            line += " |"
+                    out.append(line)
+            out.append("")
+    return "\n".join(out)
 
 
 def deck_suite(names, cpu: bool) -> dict:
@@ -182,10 +246,21 @@ def main() -> int:
     ap.add_argument("--count", type=int, default=3, help="timed inner sweeps for --sweeps")
     ap.add_argument("--dinv", default="default", help="dinv mode for --sweeps (stream, recompute)")
     ap.add_argument("--tz", type=int, default=0, help="tile depth for --sweeps (0 = default)")
+    ap.add_argument("--summarize", nargs="+", metavar="DIR", default=None,
+                    help="print markdown tables of the rocprofv3 CSV output under DIR(s) "
+                         "(no GPU needed) and exit")
+    ap.add_argument("--kernel", default="", help="with --summarize: only kernels containing this")
+    ap.add_argument("--sweeps-of", default="", help="with --summarize: kernel-name prefix whose "
+                    "counters are also summed per inner sweep")
+    ap.add_argument("--launches-per-sweep", type=int, default=0,
+                    help="with --sweeps-of: launches of that kernel per inner sweep")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy and exit (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE traffic counters)")
     args = ap.parse_args()
+    if args.summarize:
+        print(rocprof_summary(args.summarize, args.kernel, args.sweeps_of, args.launches_per_sweep))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for {args.gpus}).",
               file=sys.stderr)
@@ -233,29 +308,44 @@ def main() -> int:
     units = p.units_per_sweep
     details = {}
 
-    # ---- FP64 (fused multiply-adds, dinv streamed: the plain drop-in data flow)
-    with Solver(p, precision="fp64") as s:
-        for _ in range(args.warmup):
-            s.run()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        stats = [s.run() for _ in range(args.steps)]
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        f64, fm64 = s.flux()
-        ts = s.time_sweeps(3)
-        plan64 = s.plan()
+    # ---- FP64, fused multiply-adds (the default plan: the one-wavefront tile
+    # sweep for 33..64 angles)
+    def fp64_run(kernel):
+        with Solver(p, precision="fp64", sweep_kernel=kernel) as s:
+            for _ in range(args.warmup):
+                s.run()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            stats = [s.run() for _ in range(args.steps)]
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            f, _ = s.flux()
+            ts = s.time_sweeps(3)
+            plan = s.plan()
+        return stats, wall, f, ts, plan
+
+    stats, wall, f64, ts, plan64 = fp64_run("auto")
     inner = stats[-1]["inner_total"]
     sweep_ms = float(np.mean([st["sweep_ms"] for st in stats]))
     solve_ms = float(np.mean([st["solve_ms"] for st in stats]))
     grind = sweep_ms * 1e6 / (units * inner)
     b64 = bytes_per_unit(p.nang, p.cmom, 8)
-    launches = ts["launches_per_sweep"]
-    achieved = (units / launches) * b64 / (ts["ms_per_launch"] * 1e-3) / 1e9
+    # algorithmic bytes of one inner sweep over its measured time (sweep
+    # launches plus the octant sum, HIP events on the solver's stream)
+    achieved = units * b64 / (ts["ms_per_sweep"] * 1e-3) / 1e9
     details["fp64"] = {"grind_ns": grind, "sweep_ms_per_inner": sweep_ms / inner,
                        "solve_s": solve_ms / 1e3, "inner": inner, "outer": stats[-1]["outer_total"],
-                       "wall_s_per_step": wall / args.steps, "launch_ms": ts["ms_per_launch"],
-                       "launches_per_sweep": launches, "bytes_per_unit": b64, "plan": plan64}
+                       "wall_s_per_step": wall / args.steps,
+                       "timed_sweep_ms": ts["ms_per_sweep"], "launch_ms": ts["ms_per_launch"],
+                       "launches_per_sweep": ts["launches_per_sweep"], "bytes_per_unit": b64,
+                       "plan": plan64}
+    if not args.no_extra:
+        st_t, _, f_t, ts_t, plan_t = fp64_run("tiled")
+        g_t = float(np.mean([st["sweep_ms"] for st in st_t])) * 1e6 / (units * st_t[-1]["inner_total"])
+        details["fp64_multiwave_tiles"] = {
+            "grind_ns": g_t, "timed_sweep_ms": ts_t["ms_per_sweep"], "plan": plan_t,
+            "inner": st_t[-1]["inner_total"],
+            "maxrel_vs_default": float(np.max(np.abs(f_t - f64) / np.abs(f64)))}
 
     # ---- FP64 in the reference's operation order (bit-equal to the oracle)
     with Solver(p, precision="fp64_exact") as s:
@@ -265,9 +355,11 @@ def main() -> int:
                              "solve_s": st["solve_ms"] / 1e3, "inner": st["inner_total"],
                              "maxrel_vs_fp64_fused": float(np.max(np.abs(fex - f64) / np.abs(fex)))}
 
-    # ---- mixed precision (FP32 sweep, FP64 moments / sources / tests), plain
-    # and with per-outer defect correction
-    for key, kw in (("mixed", {}), ("mixed_refine", {"refine": True})):
+    # ---- mixed precision (FP32 sweeps, FP64 moments / sources / tests): the
+    # default with FP64 polish sweeps at the end of each outer, FP32 sweeps
+    # throughout, and per-outer defect correction
+    for key, kw in (("mixed", {}), ("mixed_fp32_only", {"fp64_polish": 0.0}),
+                    ("mixed_refine", {"refine": True, "fp64_polish": 0.0})):
         with Solver(p, precision="mixed", **kw) as s:
             for _ in range(args.warmup):
                 s.run()
@@ -281,36 +373,40 @@ def main() -> int:
                         "sweeps_fp32": st["sweeps_fp32"],
                         "maxrel_vs_fp64": float(np.max(d / np.abs(fex))),
                         "l2rel_vs_fp64": float(np.linalg.norm(d) / np.linalg.norm(fex)),
-                        "speedup_vs_fp64_grind": grind / g, "plan": plan}
+                        "speedup_vs_fp64_grind": grind / g,
+                        "speedup_vs_fp64_solve": solve_ms / st["solve_ms"], "plan": plan}
 
     # ---- adjoint gradient (config 3: 48^3, 16 groups, 24 angles, P0): the
-    # forward solve, then dR/dsiga for all cells by reverse sweeps
+    # FP64 forward solve, then dR/dsiga for all cells by reverse sweeps in
+    # FP64 and in FP32 (on the FP64 solve's fixup decisions)
     if not args.no_extra:
         c3 = problems.config3()
         resp = np.zeros((c3.ng, c3.nz, c3.ny, c3.nx))
         resp[:, -8:, -8:, -8:] = 1.0
-        for prec in ("fp64", "mixed"):
-            with Solver(c3, precision=prec) as s:
-                st = s.run()
+        with Solver(c3, precision="fp64") as s:
+            st = s.run()
+            for prec in ("fp64", "mixed"):
                 s.gradient(resp, precision=prec)  # warm-up
                 t0 = time.perf_counter()
                 g, R, it = s.gradient(resp, precision=prec)
                 tg = time.perf_counter() - t0
-            details[f"adjoint_config3_{prec}"] = {
-                "forward_solve_s": st["solve_ms"] / 1e3, "gradient_s": tg, "adjoint_iterations": it,
-                "response": R, "cells": c3.ncells,
-                "grad_siga_max_abs": float(np.max(np.abs(g.sum(axis=0))))}
+                details[f"adjoint_config3_{prec}"] = {
+                    "forward_solve_s": st["solve_ms"] / 1e3, "gradient_s": tg,
+                    "adjoint_iterations": it, "response": R, "cells": c3.ncells,
+                    "grad_siga_max_abs": float(np.max(np.abs(g.sum(axis=0))))}
 
     # ---- batch of uneven problems (config 4: 48 problems, edges 24..64,
     # 16 groups, 24 angles, P3), one launch per sweep stage, graph-captured
-    # outer iterations, autotuned tile shape
+    # outer iterations, autotuned tile shape (cached for the second batch)
     if not args.no_extra:
         from snap_amd.batch import Batch
         c4 = problems.config4()
         for prec in ("fp64", "mixed"):
             with Batch(c4, precision=prec, use_graph=True) as b:
-                b.run()  # warm-up (captures the graph)
+                st0, _ = b.run()  # warm-up (captures the graph)
                 st, per = b.run()
+            with Batch(c4, precision=prec, use_graph=True) as b:
+                st2, _ = b.run()
             times = np.array([x["time_ms"] for x in per])
             inners = [x["inner_total"] for x in per]
             outers = [x["outer_total"] for x in per]
@@ -320,12 +416,18 @@ def main() -> int:
                 "converged": st["converged"], "outer_iterations": st["outer_iterations"],
                 "per_problem_inner_min_max": [min(inners), max(inners)],
                 "per_problem_outer_min_max": [min(outers), max(outers)],
-                # a problem's time is read at the end of the (lockstep) outer
-                # iteration in which its outer test passed
+                "sweeps_fp64": int(sum(x["sweeps_fp64"] for x in per)),
+                "sweeps_total": int(sum(inners)),
+                # device clock at the end of the (lockstep) outer iteration in
+                # which the problem's outer test passed
                 "time_to_convergence_p50_ms": float(np.percentile(times, 50)),
                 "time_to_convergence_p99_ms": float(np.percentile(times, 99)),
                 "tile": list(st["tile"]), "angles_per_lane": st["angles_per_lane"],
-                "autotune_s": st["autotune_ms"] / 1e3}
+                "autotune_s": st0["autotune_ms"] / 1e3,
+                "second_batch_autotune_s": st2["autotune_ms"] / 1e3,
+                "second_batch_plan_cached": st2["plan_cached"]}
+        details["batch_config4_mixed_speedup"] = (details["batch_config4_fp64"]["solve_s"] /
+                                                  details["batch_config4_mixed"]["solve_s"])
 
     cpu = None
     if not args.no_cpu:

@@ -274,7 +274,9 @@ typedef struct {
                                a multi-wave option (tile x/y, waves, angles per lane) is set */
   double fp64_polish;       /* MIXED: run an inner's sweep in FP64 when the previous inner
                                of the same outer changed the scalar flux by at most
-                               fp64_polish * epsi (max relative change); 0 = never */
+                               fp64_polish * epsi (max relative change), or by at most
+                               1e-5 (FP32 sweeps' own rounding level) whatever epsi
+                               asks; 0 = never */
 } snap_solve_options_t;
 
 enum { SNAP_SWEEP_AUTO = 0, SNAP_SWEEP_TILED = 1, SNAP_SWEEP_WAVE_TILE = 2 };
@@ -357,10 +359,15 @@ int snap_hip_solver_time_sweeps(snap_solver_t *solver, int count, int precision,
  * at the solver's current (converged) flux with respect to the total cross
  * section t_xs[g][c], for nmom = 1.  No counterpart in the reference (the
  * adjoint-sweep pattern; callers that differentiate a solve).
- * precision: SNAP_PRECISION_FP64, or SNAP_PRECISION_MIXED for FP32 forward
- * and reverse sweeps.  Runs one forward sweep of the current flux's source
- * (its tile faces are the checkpoint), then reverse sweeps of the adjoint
- * source iteration until max|d zeta| <= tol * max|zeta|.  resp and grad are
+ * precision: SNAP_PRECISION_FP64, or SNAP_PRECISION_MIXED for FP32 reverse
+ * sweeps.  Runs one FP64 forward sweep of the current flux's source (its
+ * tile faces are the checkpoint), then reverse sweeps of the adjoint source
+ * iteration until max|d zeta| <= tol * max|zeta| (with FP32 reverse sweeps
+ * also once the change, below 1e-5 of max|zeta|, stops shrinking: their
+ * rounding floor); each reverse sweep
+ * recomputes its tile's forward fluxes and fixup decisions in FP64 from the
+ * checkpoint (an FP32 recompute flips decisions near zero), and runs the
+ * reverse recurrence in FP64 or FP32.  resp and grad are
  * host arrays [ng][nz][ny][nx]; response (optional) receives R. */
 int snap_hip_solver_gradient(snap_solver_t *solver, const double *resp, int precision,
                              double tol, int maxit, double *grad, double *response,
@@ -387,14 +394,21 @@ typedef struct {
   int waves_per_tile;       /* 0 = choose */
   int angles_per_lane;      /* 0 = choose (autotuned with the tile) */
   int use_graph;            /* 1 = capture one outer iteration in a hipGraph and replay it */
+  double fp64_polish;       /* MIXED: a problem sweeps its next inner iteration in FP64
+                               once its previous inner iteration of the same outer
+                               changed its flux by at most fp64_polish * epsi or
+                               1e-5, whichever is larger (max |flux0/flux0p - 1|);
+                               0 = FP32 sweeps throughout */
 } snap_batch_options_t;
 
 typedef struct {
   int inner_total;          /* inner iterations of this problem */
   int outer_total;          /* outer iterations of this problem */
   int converged;
-  double time_ms;           /* from the start of the run to the end of the outer
-                               iteration in which it converged */
+  int sweeps_fp64;          /* inner sweeps run in FP64 (all of them at FP64) */
+  double time_ms;           /* device clock: from the start of the run to the end of
+                               the outer iteration in which it converged (to the end
+                               of the run when it did not) */
 } snap_problem_stats_t;
 
 typedef struct {
@@ -406,7 +420,10 @@ typedef struct {
   int tile[3];              /* tile shape used */
   int waves;
   int angles_per_lane;
-  double autotune_ms;       /* time the tile autotuner took at creation */
+  double autotune_ms;       /* time the tile autotuner took at creation (0 when the
+                               tile came from the process's cache of earlier batches
+                               of the same shapes) */
+  int plan_cached;          /* 1 = tile taken from that cache */
 } snap_batch_stats_t;
 
 int snap_hip_batch_create(const snap_problem_t *problems, int count,

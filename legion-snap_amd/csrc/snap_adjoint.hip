@@ -19,6 +19,14 @@
 // adjoint source iteration zeta = r + S^T u(zeta) (u = sum of bN) converges
 // like the forward one.
 //
+// Precision: the forward recompute (pc and the fixup's decisions) is FP64
+// from an FP64 checkpoint; the reverse recurrence runs in AReal (FP64, or
+// FP32 for the FP32 reverse sweeps).  The decisions fix which linear map is
+// differentiated, and an FP32 forward flips some of them (a face near zero),
+// which moves the gradient along that face's characteristic by far more
+// than FP32 rounding; with FP64 decisions the FP32 reverse sweep differs
+// from the FP64 one by rounding only.
+//
 // Checkpointing: the forward sweep leaves every tile's outflow faces in its
 // face buffer (the tile-edge planes).  The reverse kernel takes one tile per
 // workgroup, recomputes the tile's forward pc and fixup decisions from those
@@ -29,7 +37,6 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
-#include <type_traits>
 
 #include "snap_internal.h"
 
@@ -65,12 +72,14 @@ struct AdjLds {
 
 __host__ __device__ inline size_t a16(size_t b) { return (b + 15) & ~size_t(15); }
 
+// LDS: the tile's faces (forward FP64, then adjoint AReal of arb bytes in
+// the same space), forward pc in AReal, decisions, tables, sums.
 __host__ __device__ inline AdjLds adj_lds(int TX, int TY, int TZ, int nang, int W, int SB,
-                                          int ncells_tile, int ncp, int rb) {
+                                          int ncells_tile, int ncp, int arb) {
   AdjLds L;
-  size_t o = a16((size_t)(TY * TZ + TX * TZ + TX * TY) * nang * rb);
+  size_t o = a16((size_t)(TY * TZ + TX * TZ + TX * TY) * nang * sizeof(double));
   L.pct = o;
-  o = a16(o + (size_t)ncells_tile * nang * rb);
+  o = a16(o + (size_t)ncells_tile * nang * arb);
   L.flt = o;
   o = a16(o + (size_t)ncells_tile * nang);
   L.tcr = o;
@@ -111,18 +120,19 @@ __device__ inline unsigned seg_count3(unsigned long long bx, unsigned long long 
   }
 }
 
-template <typename Real, int CPW>
+template <typename AReal, int CPW>
 __global__ void __launch_bounds__(1024) k_adjoint_tiles(AdjArgs A, int plane_off) {
+  using Real = double;  // the forward recompute
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int TX = A.TX, TY = A.TY, TZ = A.TZ, nang = A.nang, SB = A.sb;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
   const int ncp = A.ncellplanes;
-  const AdjLds lay = adj_lds(TX, TY, TZ, nang, W, SB, A.ncells_tile, ncp, (int)sizeof(Real));
+  const AdjLds lay = adj_lds(TX, TY, TZ, nang, W, SB, A.ncells_tile, ncp, (int)sizeof(AReal));
   const int fxn = TY * TZ * nang, fyn = TX * TZ * nang, fzn = TX * TY * nang;
   Real *fx = (Real *)smem_raw;
   Real *fy = fx + fxn;
   Real *fz = fy + fyn;
-  Real *pct = (Real *)(smem_raw + lay.pct);                   // [tile cell][angle] forward pc
+  AReal *pct = (AReal *)(smem_raw + lay.pct);                 // [tile cell][angle] forward pc
   unsigned char *flt = smem_raw + lay.flt;                    // [tile cell][angle] decisions
   int4 *tcr = (int4 *)(smem_raw + lay.tcr);                   // cell, face offsets
   int *toff = (int *)(smem_raw + lay.coff);
@@ -140,7 +150,7 @@ __global__ void __launch_bounds__(1024) k_adjoint_tiles(AdjArgs A, int plane_off
   const int tile_id = (tz * A.nTy + ty) * A.nTx + tx;
   const int per_tile = fxn + fyn + fzn;
   const Real *faces_s = (const Real *)A.faces + (size_t)s * A.ntiles * per_tile;
-  Real *adj_s = (Real *)A.adj_faces + (size_t)s * A.ntiles * per_tile;
+  AReal *adj_s = (AReal *)A.adj_faces + (size_t)s * A.ntiles * per_tile;
   const int ncells = nx * ny * nz;
   const double *qg = A.q + (size_t)g * ncells;
   const double *tg = A.txs + (size_t)g * ncells;
@@ -177,7 +187,7 @@ __global__ void __launch_bounds__(1024) k_adjoint_tiles(AdjArgs A, int plane_off
     for (int idx = threadIdx.x; idx < fzn; idx += blockDim.x) fz[idx] = tz > 0 ? uz[idx] : Real(0);
   }
   const double mu_d = A.mu[aa], eta_d = A.eta[aa], xi_d = A.xi[aa];
-  const Real cx = (Real)(mu_d * A.hi), cy = (Real)(eta_d * A.hj), cz = (Real)(xi_d * A.hk);
+  const Real cx = mu_d * A.hi, cy = eta_d * A.hj, cz = xi_d * A.hk;
   const double w_d = A.w[aa];
   const Real tolr = (Real)1.0e-12;
   __syncthreads();
@@ -191,22 +201,16 @@ __global__ void __launch_bounds__(1024) k_adjoint_tiles(AdjArgs A, int plane_off
       const int4 rec = (lane_ok && my < c1) ? tcr[my] : make_int4(-1, 0, 0, 0);
       const int cell = rec.x;
       if (cell >= 0) {
-        const Real q = (Real)qg[cell], txs = (Real)tg[cell];
+        const Real q = qg[cell], txs = tg[cell];
         Real *sx = fx + rec.y + a;
         Real *sy = fy + rec.z + a;
         Real *sz = fz + rec.w + a;
         const Real pi = *sx, pj = *sy, pk = *sz;
         const Real dv = Real(1) / (((txs + cx) + cy) + cz);
         Real pc = q;
-        if constexpr (std::is_same<Real, double>::value) {
-          pc = __builtin_fma(pi, cx, pc);
-          pc = __builtin_fma(pj, cy, pc);
-          pc = __builtin_fma(pk, cz, pc);
-        } else {
-          pc = __builtin_fmaf(pi, cx, pc);
-          pc = __builtin_fmaf(pj, cy, pc);
-          pc = __builtin_fmaf(pk, cz, pc);
-        }
+        pc = __builtin_fma(pi, cx, pc);
+        pc = __builtin_fma(pj, cy, pc);
+        pc = __builtin_fma(pk, cz, pc);
         pc *= dv;
         bool hx = true, hy = true, hz = true;
         unsigned f = 0;
@@ -244,7 +248,7 @@ __global__ void __launch_bounds__(1024) k_adjoint_tiles(AdjArgs A, int plane_off
         *sx = (p2 - pi) * (hx ? Real(1) : Real(0));
         *sy = (p2 - pj) * (hy ? Real(1) : Real(0));
         *sz = (p2 - pk) * (hz ? Real(1) : Real(0));
-        pct[(size_t)my * nang + a] = pc;
+        pct[(size_t)my * nang + a] = (AReal)pc;
         flt[(size_t)my * nang + a] = (unsigned char)f;
       }
     }
@@ -252,17 +256,22 @@ __global__ void __launch_bounds__(1024) k_adjoint_tiles(AdjArgs A, int plane_off
   }
 
   // ---- adjoint inflow: the downstream tiles' low-side adjoint faces (zero
-  // past the chunk's far side, where outflows leave)
+  // past the chunk's far side, where outflows leave), in the face space the
+  // forward recompute has finished with
+  AReal *bfx = (AReal *)smem_raw;
+  AReal *bfy = bfx + fxn;
+  AReal *bfz = bfy + fyn;
   {
-    const Real *dx = adj_s + (size_t)(tile_id + 1) * per_tile;
-    const Real *dy = adj_s + (size_t)(tile_id + A.nTx) * per_tile + fxn;
-    const Real *dz = adj_s + (size_t)(tile_id + A.nTx * A.nTy) * per_tile + fxn + fyn;
+    const AReal *dx = adj_s + (size_t)(tile_id + 1) * per_tile;
+    const AReal *dy = adj_s + (size_t)(tile_id + A.nTx) * per_tile + fxn;
+    const AReal *dz = adj_s + (size_t)(tile_id + A.nTx * A.nTy) * per_tile + fxn + fyn;
     const bool hx1 = tx + 1 < A.nTx, hy1 = ty + 1 < A.nTy, hz1 = tz + 1 < A.nTz;
-    for (int idx = threadIdx.x; idx < fxn; idx += blockDim.x) fx[idx] = hx1 ? dx[idx] : Real(0);
-    for (int idx = threadIdx.x; idx < fyn; idx += blockDim.x) fy[idx] = hy1 ? dy[idx] : Real(0);
-    for (int idx = threadIdx.x; idx < fzn; idx += blockDim.x) fz[idx] = hz1 ? dz[idx] : Real(0);
+    for (int idx = threadIdx.x; idx < fxn; idx += blockDim.x) bfx[idx] = hx1 ? dx[idx] : AReal(0);
+    for (int idx = threadIdx.x; idx < fyn; idx += blockDim.x) bfy[idx] = hy1 ? dy[idx] : AReal(0);
+    for (int idx = threadIdx.x; idx < fzn; idx += blockDim.x) bfz[idx] = hz1 ? dz[idx] : AReal(0);
   }
   __syncthreads();
+  const AReal acx = (AReal)cx, acy = (AReal)cy, acz = (AReal)cz;
 
   // ---- reverse pass over the tile's cell diagonals
   int nbuf = 0;
@@ -293,35 +302,36 @@ __global__ void __launch_bounds__(1024) k_adjoint_tiles(AdjArgs A, int plane_off
       const int cell = rec.x;
       double bn_out = 0.0, bd_out = 0.0;
       if (cell >= 0) {
+        using R = AReal;
         const unsigned f = flt[(size_t)my * nang + a];
-        const Real pc = pct[(size_t)my * nang + a];
-        const Real txs = (Real)tg[cell];
-        Real *sx = fx + rec.y + a;
-        Real *sy = fy + rec.z + a;
-        Real *sz = fz + rec.w + a;
-        const Real bx = *sx, by = *sy, bz = *sz;
-        const Real ox = (f & kAfOutX) ? Real(1) : Real(0), oy = (f & kAfOutY) ? Real(1) : Real(0),
-                   oz = (f & kAfOutZ) ? Real(1) : Real(0);
-        const Real bpc = (Real)(w_d * zg[cell]) + Real(2) * (ox * bx + oy * by + oz * bz);
-        Real kx = Real(1), ky = Real(1), kz = Real(1), den;
+        const R pc = pct[(size_t)my * nang + a];
+        const R txs = (R)tg[cell];
+        R *sx = bfx + rec.y + a;
+        R *sy = bfy + rec.z + a;
+        R *sz = bfz + rec.w + a;
+        const R bx = *sx, by = *sy, bz = *sz;
+        const R ox = (f & kAfOutX) ? R(1) : R(0), oy = (f & kAfOutY) ? R(1) : R(0),
+                oz = (f & kAfOutZ) ? R(1) : R(0);
+        const R bpc = (R)(w_d * zg[cell]) + R(2) * (ox * bx + oy * by + oz * bz);
+        R kx = R(1), ky = R(1), kz = R(1), den;
         if (f & kAfFixed) {
-          const Real vx = (f & kAfPcX) ? Real(1) : Real(0), vy = (f & kAfPcY) ? Real(1) : Real(0),
-                     vz = (f & kAfPcZ) ? Real(1) : Real(0);
-          kx = Real(0.5) * (Real(1) + vx);
-          ky = Real(0.5) * (Real(1) + vy);
-          kz = Real(0.5) * (Real(1) + vz);
-          den = txs + cx * vx + cy * vy + cz * vz;
+          const R vx = (f & kAfPcX) ? R(1) : R(0), vy = (f & kAfPcY) ? R(1) : R(0),
+                  vz = (f & kAfPcZ) ? R(1) : R(0);
+          kx = R(0.5) * (R(1) + vx);
+          ky = R(0.5) * (R(1) + vy);
+          kz = R(0.5) * (R(1) + vz);
+          den = txs + acx * vx + acy * vy + acz * vz;
         } else {
-          den = ((txs + cx) + cy) + cz;
+          den = ((txs + acx) + acy) + acz;
         }
-        Real bn = Real(0), bd = Real(0);
+        R bn = R(0), bd = R(0);
         if (!(f & kAfZero)) {
           bn = bpc / den;
           bd = -bn * pc;
         }
-        *sx = -ox * bx + bn * cx * kx;
-        *sy = -oy * by + bn * cy * ky;
-        *sz = -oz * bz + bn * cz * kz;
+        *sx = -ox * bx + bn * acx * kx;
+        *sy = -oy * by + bn * acy * ky;
+        *sz = -oz * bz + bn * acz * kz;
         bn_out = (double)bn;
         bd_out = (double)bd;
       }
@@ -338,8 +348,8 @@ __global__ void __launch_bounds__(1024) k_adjoint_tiles(AdjArgs A, int plane_off
   if (nbuf > 0) flush();
 
   // ---- low-side adjoint faces to the upstream tiles
-  Real *out = adj_s + (size_t)tile_id * per_tile;
-  for (int idx = threadIdx.x; idx < per_tile; idx += blockDim.x) out[idx] = fx[idx];
+  AReal *out = adj_s + (size_t)tile_id * per_tile;
+  for (int idx = threadIdx.x; idx < per_tile; idx += blockDim.x) out[idx] = bfx[idx];
 }
 
 typedef void (*AdjKernel)(AdjArgs, int);

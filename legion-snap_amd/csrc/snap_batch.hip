@@ -19,10 +19,21 @@
 // iteration -- outer source, iitm predicated inner iterations, outer test --
 // is captured once in a hipGraph and replayed; the host only reads the flags
 // between outer iterations.
+//
+// Mixed precision follows the single-problem solve's FP64 polish: prec[p]
+// says whether problem p's next inner sweep runs in FP64 (its previous inner
+// of the same outer changed its flux by at most fp64_polish * epsi).  Each
+// sweep is then two passes over the diagonals, an FP32 kernel whose
+// workgroups skip problems with prec[p] = 1 and an FP64 kernel that skips
+// the others.  Completion times are read from the device's constant-rate
+// clock when a problem's outer test passes.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <string>
 
 #include "snap_internal.h"
 
@@ -72,13 +83,17 @@ __global__ void k_fill_ints(int *__restrict__ dst, int v, int n) {
 // conv[p] stays 1 unless some group and cell of problem p fails
 // |flux0/flux0p - 1| <= eps (|flux0| when |flux0p| < 1e-12).  Block
 // (x, p) covers a slice of problem p's cells x groups.
+// With `change`, also change[p] = max |flux0/flux0p - 1| over the problem
+// as the bit pattern of the non-negative double (NaN above every number).
 __global__ void k_problem_test(const double *__restrict__ f, const double *__restrict__ fp,
                                const int64_t *__restrict__ cell_off, int64_t total, int ng,
-                               double eps, int *__restrict__ conv) {
+                               double eps, int *__restrict__ conv,
+                               unsigned long long *__restrict__ change) {
   const int p = blockIdx.y;
   const int64_t c0 = cell_off[p], nc = cell_off[p + 1] - c0;
   const int64_t n = nc * ng;
   bool ok = true;
+  double m = 0.0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int g = (int)(t / nc);
@@ -90,26 +105,61 @@ __global__ void k_problem_test(const double *__restrict__ f, const double *__res
     }
     df = fabs(f[i] / a - df);
     if (df > eps) ok = false;  // NaN compares false: counts as converged, as in the reference
+    m = (df > m || df != df) ? df : m;
   }
   const int all_ok = __syncthreads_and(ok ? 1 : 0);
   if (threadIdx.x == 0 && !all_ok) atomicAnd(&conv[p], 0);
+  if (change) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      const double v = __shfl_xor(m, o, 64);
+      m = (v > m || v != v) ? v : m;
+    }
+    if ((threadIdx.x & 63) == 0)
+      atomicMax(&change[p], (unsigned long long)__double_as_longlong(fabs(m)));
+  }
 }
 
-// Flag updates between the steps of the predicated outer iteration.
+__global__ void k_clear_test(int *__restrict__ conv, unsigned long long *__restrict__ change,
+                             int np) {
+  for (int p = threadIdx.x; p < np; p += blockDim.x) {
+    conv[p] = 1;
+    if (change) change[p] = 0ull;
+  }
+}
+
+// the device's constant-rate clock into t[i]
+__global__ void k_stamp(unsigned long long *t, int i) {
+  if (threadIdx.x == 0) t[i] = wall_clock64();
+}
+
+// Flag updates between the steps of the predicated outer iteration.  An
+// outer's first inner sweep is FP32 (prec[p] = 0).
 __global__ void k_begin_outer(const int *__restrict__ active, int *__restrict__ done,
-                              int *__restrict__ outers, int np) {
+                              int *__restrict__ outers, int *__restrict__ prec, int np) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < np && active[p]) {
     done[p] = 0;
     outers[p] += 1;
+    if (prec) prec[p] = 0;
   }
 }
 
+// After an inner test: count the inner (and its FP64 sweep), and pick the
+// precision of the problem's next inner sweep from the change just measured.
 __global__ void k_after_inner(const int *__restrict__ active, int *__restrict__ done,
-                              const int *__restrict__ conv, int *__restrict__ inners, int np) {
+                              const int *__restrict__ conv, int *__restrict__ inners,
+                              int *__restrict__ prec, const unsigned long long *__restrict__ change,
+                              double polish_at, int *__restrict__ n64, int np) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < np && active[p] && !done[p]) {
     inners[p] += 1;
+    if (prec) {
+      n64[p] += prec[p];
+      prec[p] = __longlong_as_double((long long)change[p]) <= polish_at ? 1 : 0;
+    } else if (n64) {
+      n64[p] += 1;  // an FP64 batch
+    }
     done[p] = conv[p];
   }
 }
@@ -118,15 +168,25 @@ __global__ void k_after_inner(const int *__restrict__ active, int *__restrict__ 
 // unless the last inner iteration converged
 __global__ void k_end_outer(int *__restrict__ active, const int *__restrict__ done,
                             const int *__restrict__ conv, const int *__restrict__ otno,
-                            int *__restrict__ converged, int np) {
+                            int *__restrict__ converged, unsigned long long *__restrict__ t_end,
+                            int np) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < np && active[p] && *otno > 0 && done[p] && conv[p]) {
     active[p] = 0;
     converged[p] = 1;
+    t_end[p] = wall_clock64();
   }
 }
 
 int64_t blocks_for(int64_t n) { return std::min<int64_t>((n + 255) / 256, 65536); }
+
+// Tile shapes the autotuner picked, per batch signature (precision, ng,
+// nang, nmom, every problem's extent), for the life of the process.
+struct TunedShape {
+  int TX, TY, TZ, waves, apl;
+};
+std::mutex g_tuned_lock;
+std::map<std::string, TunedShape> g_tuned;
 
 }  // namespace
 
@@ -138,6 +198,7 @@ struct snap_batch {
   int count = 0, ng = 0, nang = 0, nmom = 0, cmom = 0, nmat = 0;  // nmat: all problems' materials
   double epsi = 0;
   int iitm = 0, oitm = 0, fixup = 0, precision = 0;
+  bool polish = false;  // MIXED with FP64 polish sweeps
   snap_batch_options_t opt{};
   int64_t total = 0;
   std::vector<int64_t> cell_off;  // host [count + 1]
@@ -152,7 +213,9 @@ struct snap_batch {
   Buf mat, slgg, qi, txs, txs32, sxs, dinv, dinv32, flux0, flux0pi, flux0po, fluxm, q2grp0,
       q2grpm, qtot, part_flux, part_fluxm, faces, descs, tiles, d_cell_off;
   Buf active, done, conv, inners, outers, converged, otno;
+  Buf prec, change, n64, stamps;  // stamps: [count] ends, then the run's start and end
   double autotune_ms = 0;
+  bool plan_cached = false;
 
   ~snap_batch() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -238,8 +301,12 @@ struct snap_batch {
     return SNAP_OK;
   }
 
-  int sweep(hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr) {
-    const bool fp32 = precision == SNAP_PRECISION_MIXED;
+  // one sweep pass over every diagonal in one precision; with prec, only the
+  // problems whose prec[p] equals want
+  int sweep_pass(bool fp32, const int *prec, int want) {
+    BatchSweep b = bs;
+    b.prec = prec;
+    b.prec_want = want;
     SweepLaunch L;
     memset(&L, 0, sizeof(L));
     L.plan = &plan;
@@ -254,11 +321,14 @@ struct snap_batch {
     L.part_fluxm = cmom > 1 ? part_fluxm.as<double>() : nullptr;
     L.faces = faces.p;
     L.fp32 = fp32;
-    L.batch = &bs;
-    if (t0) SNAP_HIP_TRY(hipEventRecord(t0, stream));
-    SNAP_TRY(sweep_enqueue(L, stream));
-    if (t1) SNAP_HIP_TRY(hipEventRecord(t1, stream));
-    return SNAP_OK;
+    L.batch = &b;
+    return sweep_enqueue(L, stream);
+  }
+
+  int sweep() {
+    if (!polish) return sweep_pass(precision == SNAP_PRECISION_MIXED, nullptr, 0);
+    SNAP_TRY(sweep_pass(true, prec.as<int>(), 0));
+    return sweep_pass(false, prec.as<int>(), 1);
   }
 
   int copy(Buf &dst, const Buf &src, int64_t n) {
@@ -268,13 +338,16 @@ struct snap_batch {
     return SNAP_OK;
   }
 
-  int test(const Buf &prev, double eps) {
-    hipLaunchKernelGGL(k_fill_ints, dim3(1), dim3(256), 0, stream, conv.as<int>(), 1, count);
+  // conv[p] (and change[p] when asked) of the convergence test against prev
+  int test(const Buf &prev, double eps, bool with_change) {
+    unsigned long long *ch = with_change ? change.as<unsigned long long>() : nullptr;
+    hipLaunchKernelGGL(k_clear_test, dim3(1), dim3(256), 0, stream, conv.as<int>(), ch, count);
     int64_t maxn = 0;
     for (int p = 0; p < count; p++) maxn = std::max(maxn, (cell_off[p + 1] - cell_off[p]) * ng);
     const int bx = (int)std::min<int64_t>((maxn + 255) / 256, 1024);
     hipLaunchKernelGGL(k_problem_test, dim3(bx, count), dim3(256), 0, stream, flux0.as<double>(),
-                       prev.as<double>(), d_cell_off.as<int64_t>(), total, ng, eps, conv.as<int>());
+                       prev.as<double>(), d_cell_off.as<int64_t>(), total, ng, eps, conv.as<int>(),
+                       ch);
     SNAP_HIP_TRY(hipGetLastError());
     return SNAP_OK;
   }
@@ -284,8 +357,10 @@ struct snap_batch {
     const int gb = (count + 255) / 256;
     const int64_t gc = (int64_t)ng * total;
     const int nm1 = cmom - 1;
+    int *pr = polish ? prec.as<int>() : nullptr;
+    int *n64p = (polish || precision == SNAP_PRECISION_FP64) ? n64.as<int>() : nullptr;
     hipLaunchKernelGGL(k_begin_outer, dim3(gb), dim3(256), 0, stream, active.as<int>(),
-                       done.as<int>(), outers.as<int>(), count);
+                       done.as<int>(), outers.as<int>(), pr, count);
     SNAP_TRY(launch_outer_dense(qi.as<double>(), flux0.as<double>(),
                                 cmom > 1 ? fluxm.as<double>() : nullptr, slgg.as<double>(),
                                 mat.as<int32_t>(), q2grp0.as<double>(),
@@ -302,13 +377,16 @@ struct snap_batch {
       SNAP_TRY(launch_octant_reduce(part_flux.as<double>(), flux0.as<double>(), gc, stream));
       if (cmom > 1)
         SNAP_TRY(launch_octant_reduce(part_fluxm.as<double>(), fluxm.as<double>(), gc * nm1, stream));
-      SNAP_TRY(test(flux0pi, epsi));
+      SNAP_TRY(test(flux0pi, epsi, polish));
       hipLaunchKernelGGL(k_after_inner, dim3(gb), dim3(256), 0, stream, active.as<int>(),
-                         done.as<int>(), conv.as<int>(), inners.as<int>(), count);
+                         done.as<int>(), conv.as<int>(), inners.as<int>(), pr,
+                         change.as<unsigned long long>(), polish_threshold(opt.fp64_polish, epsi),
+                         n64p, count);
     }
-    SNAP_TRY(test(flux0po, 100.0 * epsi));
+    SNAP_TRY(test(flux0po, 100.0 * epsi, false));
     hipLaunchKernelGGL(k_end_outer, dim3(gb), dim3(256), 0, stream, active.as<int>(),
-                       done.as<int>(), conv.as<int>(), otno.as<int>(), converged.as<int>(), count);
+                       done.as<int>(), conv.as<int>(), otno.as<int>(), converged.as<int>(),
+                       stamps.as<unsigned long long>(), count);
     SNAP_HIP_TRY(hipGetLastError());
     return SNAP_OK;
   }
@@ -321,19 +399,21 @@ struct snap_batch {
       hipLaunchKernelGGL(k_fill, dim3(blocks_for(gc * (cmom - 1))), dim3(256), 0, stream,
                          fluxm.as<double>(), 0.0, gc * (cmom - 1));
     hipLaunchKernelGGL(k_fill_ints, dim3(1), dim3(256), 0, stream, active.as<int>(), 1, count);
-    for (Buf *b : {&done, &inners, &outers, &converged})
+    for (Buf *b : {&done, &inners, &outers, &converged, &prec, &n64})
       hipLaunchKernelGGL(k_fill_ints, dim3(1), dim3(256), 0, stream, b->as<int>(), 0, count);
     SNAP_HIP_TRY(hipGetLastError());
     return SNAP_OK;
   }
 
   // time `reps` sweeps of the current plan with every problem active
-  int time_plan(int reps, float *ms) {
+  // (warm: one untimed sweep first, for a kernel's first launches, which load
+  // its code object)
+  int time_plan(int reps, float *ms, bool warm) {
     SNAP_TRY(reset());
     hipEvent_t e0, e1;
     SNAP_HIP_TRY(hipEventCreate(&e0));
     SNAP_HIP_TRY(hipEventCreate(&e1));
-    SNAP_TRY(sweep());  // untimed: first launches load the code object
+    if (warm) SNAP_TRY(sweep());
     SNAP_HIP_TRY(hipEventRecord(e0, stream));
     for (int r = 0; r < reps; r++) SNAP_TRY(sweep());
     SNAP_HIP_TRY(hipEventRecord(e1, stream));
@@ -365,6 +445,8 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
   b->precision = b->opt.precision;
   if (b->precision != SNAP_PRECISION_FP64 && b->precision != SNAP_PRECISION_MIXED)
     return fail(SNAP_ERR_INVALID, "batch precision must be FP64 or MIXED");
+  if (!(b->opt.fp64_polish >= 0.0)) return fail(SNAP_ERR_INVALID, "fp64_polish must be >= 0");
+  b->polish = b->precision == SNAP_PRECISION_MIXED && b->opt.fp64_polish > 0.0;
   if (b->nmom < 1 || b->nmom > 4 || b->ng < 1 || b->nang < 1)
     return fail(SNAP_ERR_INVALID, "bad ng / nang / nmom");
   if (b->nang > 128)
@@ -445,8 +527,11 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
   SNAP_TRY(b->q2grpm.alloc(sizeof(double) * gc * nm1));
   SNAP_TRY(b->part_flux.alloc(sizeof(double) * 8 * gc));
   SNAP_TRY(b->part_fluxm.alloc(sizeof(double) * 8 * gc * nm1));
-  for (Buf *x : {&b->active, &b->done, &b->conv, &b->inners, &b->outers, &b->converged})
+  for (Buf *x : {&b->active, &b->done, &b->conv, &b->inners, &b->outers, &b->converged, &b->prec,
+                 &b->n64})
     SNAP_TRY(x->alloc(sizeof(int) * count));
+  SNAP_TRY(b->change.alloc(sizeof(unsigned long long) * count));
+  SNAP_TRY(b->stamps.alloc(sizeof(unsigned long long) * (count + 2)));
   SNAP_TRY(b->otno.alloc(sizeof(int)));
 
   // ExpandScatteringCrossSection over the batch, geometry per problem
@@ -489,16 +574,36 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
                                  nmom, st));
   }
 
-  // ---- tile shape: the caller's, or the fastest candidate on this batch
+  // ---- tile shape: the caller's, an earlier batch's of the same shapes, or
+  // the fastest candidate on this batch
   int TX = b->opt.tile[0], TY = b->opt.tile[1], TZ = b->opt.tile[2];
   int waves = b->opt.waves_per_tile, apl = b->opt.angles_per_lane;
+  std::string sig = std::to_string(b->precision) + ":" + std::to_string(ng) + ":" +
+                    std::to_string(nang) + ":" + std::to_string(nmom) + ":" +
+                    std::to_string(waves) + ":" + std::to_string(apl);
+  for (const Extent3 &e : b->ext)
+    sig += ":" + std::to_string(e.nx) + "x" + std::to_string(e.ny) + "x" + std::to_string(e.nz);
+  bool cached = false;
+  TunedShape hit{};
+  if (!(TX > 0 && TY > 0 && TZ > 0)) {
+    std::lock_guard<std::mutex> g(g_tuned_lock);
+    auto it = g_tuned.find(sig);
+    if (it != g_tuned.end()) {
+      cached = true;
+      hit = it->second;
+    }
+  }
   if (TX > 0 && TY > 0 && TZ > 0) {
     SNAP_TRY(b->build_plan(TX, TY, TZ, waves, apl));
+  } else if (cached) {
+    SNAP_TRY(b->build_plan(hit.TX, hit.TY, hit.TZ, hit.waves, hit.apl));
+    b->plan_cached = true;
   } else {
     const auto t0 = std::chrono::steady_clock::now();
     static const int cand[][3] = {{8, 8, 4}, {8, 4, 4}, {4, 4, 4}, {8, 8, 8}, {16, 8, 4}};
     float best = 1e30f;
     int bi = -1, bapl = 0;
+    bool warmed[3] = {false, false, false};  // per lane layout (one kernel each)
     for (int c = 0; c < (int)(sizeof(cand) / sizeof(cand[0])); c++) {
       for (int a = 1; a <= 2; a++) {
         if (apl > 0 && a != apl) continue;
@@ -507,7 +612,8 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
         const int rc = b->build_plan(cand[c][0], cand[c][1], cand[c][2], w, a);
         if (rc != SNAP_OK) continue;  // shape does not fit (LDS, waves)
         float ms = 0.f;
-        SNAP_TRY(b->time_plan(2, &ms));
+        SNAP_TRY(b->time_plan(1, &ms, !warmed[a]));
+        warmed[a] = true;
         if (ms < best) {
           best = ms;
           bi = c;
@@ -519,6 +625,8 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
     SNAP_TRY(b->build_plan(cand[bi][0], cand[bi][1], cand[bi][2], waves, bapl));
     b->autotune_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::lock_guard<std::mutex> g(g_tuned_lock);
+    g_tuned[sig] = TunedShape{cand[bi][0], cand[bi][1], cand[bi][2], waves, bapl};
   }
   SNAP_HIP_TRY(hipStreamSynchronize(st));
   *out = b.release();
@@ -530,11 +638,17 @@ extern "C" int snap_hip_batch_run(snap_batch_t *b, snap_batch_stats_t *stats,
   if (!b) return fail(SNAP_ERR_INVALID, "NULL batch");
   const hipStream_t st = b->stream;
   const int n = b->count;
-  std::vector<int> active(n, 1), prev(n, 1), inners(n), outers(n), conv(n);
-  std::vector<double> t_done(n, 0.0);
+  std::vector<int> active(n, 1), inners(n), outers(n), conv(n), n64(n);
+  std::vector<unsigned long long> stamps(n + 2);
+  int dev = 0, khz = 0;
+  SNAP_HIP_TRY(hipGetDevice(&dev));
+  SNAP_HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  if (khz <= 0) return fail(SNAP_ERR_INVALID, "the device reports no wall-clock rate");
   SNAP_HIP_TRY(hipStreamSynchronize(st));
   const auto t0 = std::chrono::steady_clock::now();
   SNAP_TRY(b->reset());
+  unsigned long long *sp = b->stamps.as<unsigned long long>();
+  hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, st, sp, n);
   int otno = 0;
   for (; otno < b->oitm; otno++) {
     SNAP_HIP_TRY(hipMemcpyAsync(b->otno.p, &otno, sizeof(int), hipMemcpyHostToDevice, st));
@@ -558,24 +672,24 @@ extern "C" int snap_hip_batch_run(snap_batch_t *b, snap_batch_stats_t *stats,
     }
     SNAP_HIP_TRY(hipMemcpyAsync(active.data(), b->active.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
     SNAP_HIP_TRY(hipStreamSynchronize(st));
-    const double now =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     int left = 0;
-    for (int p = 0; p < n; p++) {
-      if (prev[p] && !active[p]) t_done[p] = now;
-      prev[p] = active[p];
-      left += active[p];
-    }
+    for (int p = 0; p < n; p++) left += active[p];
     if (!left) {
       otno++;
       break;
     }
   }
+  hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, st, sp, n + 1);
+  SNAP_HIP_TRY(hipStreamSynchronize(st));
   const double solve_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   SNAP_HIP_TRY(hipMemcpy(inners.data(), b->inners.p, sizeof(int) * n, hipMemcpyDeviceToHost));
   SNAP_HIP_TRY(hipMemcpy(outers.data(), b->outers.p, sizeof(int) * n, hipMemcpyDeviceToHost));
   SNAP_HIP_TRY(hipMemcpy(conv.data(), b->converged.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  SNAP_HIP_TRY(hipMemcpy(n64.data(), b->n64.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  SNAP_HIP_TRY(hipMemcpy(stamps.data(), sp, sizeof(unsigned long long) * (n + 2),
+                         hipMemcpyDeviceToHost));
+  const double tick_ms = 1.0 / (double)khz;  // khz ticks per ms
   double units = 0.0;
   int nconv = 0;
   for (int p = 0; p < n; p++) {
@@ -586,7 +700,8 @@ extern "C" int snap_hip_batch_run(snap_batch_t *b, snap_batch_stats_t *stats,
       per_problem[p].inner_total = inners[p];
       per_problem[p].outer_total = outers[p];
       per_problem[p].converged = conv[p];
-      per_problem[p].time_ms = conv[p] ? t_done[p] : solve_ms;
+      per_problem[p].sweeps_fp64 = n64[p];
+      per_problem[p].time_ms = (double)((conv[p] ? stamps[p] : stamps[n + 1]) - stamps[n]) * tick_ms;
     }
   }
   if (stats) {
@@ -602,6 +717,7 @@ extern "C" int snap_hip_batch_run(snap_batch_t *b, snap_batch_stats_t *stats,
     stats->waves = b->plan.waves;
     stats->angles_per_lane = b->plan.apl;
     stats->autotune_ms = b->autotune_ms;
+    stats->plan_cached = b->plan_cached ? 1 : 0;
   }
   return SNAP_OK;
 }

@@ -220,6 +220,16 @@ struct BatchDesc {
 
 // Tile-diagonal launches of a batch: launch D covers diagonal D of every
 // problem; batch_tiles holds (problem, packed tile) entries by diagonal.
+// Mixed-precision FP64 polish: an inner iteration sweeps in FP64 once the
+// previous inner of its outer changed the flux by at most this much (the
+// largest |flux0/flux0p - 1|).  Near convergence FP32 sweeps move the flux by
+// their own rounding, a few 1e-6 at the worst cell, so below the 1e-5 floor
+// they make no progress whatever the tolerance asks for.
+constexpr double kFp32ChangeFloor = 1.0e-5;
+inline double polish_threshold(double fp64_polish, double epsi) {
+  return fp64_polish * epsi > kFp32ChangeFloor ? fp64_polish * epsi : kFp32ChangeFloor;
+}
+
 struct BatchSweep {
   int nplanes = 0;
   std::vector<int> plane_off;       // host: offsets of each diagonal in batch_tiles
@@ -227,6 +237,8 @@ struct BatchSweep {
   const BatchDesc *descs = nullptr;
   const int *active = nullptr;      // device flags: problem still iterating
   const int *done = nullptr;        // device flags: inner iteration converged this outer
+  const int *prec = nullptr;        // device flags: sweep precision (1 = FP64); NULL = all
+  int prec_want = 0;                // this pass sweeps the problems with prec[p] == prec_want
   int64_t total_cells = 0;
 };
 

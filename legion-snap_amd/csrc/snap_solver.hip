@@ -445,8 +445,10 @@ extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) 
         // FP64 polish: an inner whose predecessor moved the flux by at most
         // fp64_polish * epsi is likely one of the outer's last (the change
         // contracts by the in-group scattering ratio each inner), so it runs
-        // in FP64 and the outer ends on FP64 sweeps
-        const bool polish_now = polish && change <= s->opt.fp64_polish * s->prob.epsi;
+        // in FP64 and the outer ends on FP64 sweeps; so does every inner once
+        // FP32 sweeps have reached their own rounding level
+        const bool polish_now =
+            polish && change <= polish_threshold(s->opt.fp64_polish, s->prob.epsi);
         const bool fp32 = mixed && !(s->opt.first_sweep_fp64 && st.inner_total == 0) &&
                           !(refine && inno == 0) && !polish_now;
         if (refine && fp32)  // FP64 defect of the source against the outer's base
@@ -600,7 +602,9 @@ extern "C" int snap_hip_solver_gradient(snap_solver_t *s, const double *resp, in
     return fail(SNAP_ERR_INVALID, "adjoint precision must be FP64 or MIXED (FP32 sweeps)");
   if (!(tol > 0.0) || maxit < 1) return fail(SNAP_ERR_INVALID, "bad tolerance or iteration cap");
   const bool fp32 = precision == SNAP_PRECISION_MIXED;
-  const SweepPlan &pl = s->plan_for(precision);
+  // the checkpoint and the reverse kernel's forward recompute are FP64 (the
+  // fixup's decisions of the FP64 solve); `fp32` sets the reverse recurrence
+  const SweepPlan &pl = s->plan_for(SNAP_PRECISION_FP64);
   {
     int cpw, waves, sb;
     SNAP_TRY(adjoint_plan_fits(pl, s->nang, fp32 ? 4 : 8, &cpw, &waves, &sb));
@@ -616,16 +620,12 @@ extern "C" int snap_hip_solver_gradient(snap_solver_t *s, const double *resp, in
   SNAP_TRY(norms.alloc(2 * sizeof(unsigned long long)));
   if (!s->part_grad.p) SNAP_TRY(s->part_grad.alloc(sizeof(double) * 8 * gc));
   if (!s->adj_faces.p) SNAP_TRY(s->adj_faces.alloc(s->faces.bytes));
-  if (fp32 && !s->txs32.p) {
-    SNAP_TRY(s->txs32.alloc(sizeof(float) * gc));
-    SNAP_TRY(launch_to_float(s->txs.as<double>(), s->txs32.as<float>(), (int64_t)gc, st));
-  }
   SNAP_HIP_TRY(hipMemcpyAsync(r.p, resp, sizeof(double) * gc, hipMemcpyHostToDevice, st));
   // forward source of the current flux, q = qi + S flux0, and one forward
   // sweep of it: its tile faces are the checkpoint the reverse sweeps start from
   SNAP_TRY(s->outer_source());
   SNAP_TRY(s->inner_source());
-  SNAP_TRY(s->sweep(precision, nullptr, true, true, true));
+  SNAP_TRY(s->sweep(SNAP_PRECISION_FP64, nullptr, true, true, true));
   SNAP_HIP_TRY(hipMemcpyAsync(zeta.p, r.p, sizeof(double) * gc, hipMemcpyDeviceToDevice, st));
   auto reverse = [&](const DeviceBuffer &z) -> int {
     SNAP_TRY(adjoint_enqueue(pl, s->quad, s->ng, s->prob.fixup ? 1 : 0, s->hi, s->hj, s->hk,
@@ -636,6 +636,7 @@ extern "C" int snap_hip_solver_gradient(snap_solver_t *s, const double *resp, in
   };
   int it = 0;
   bool done = false;
+  double prev = HUGE_VAL;
   while (!done && it < maxit) {
     SNAP_TRY(reverse(zeta));
     SNAP_TRY(launch_adjoint_update(r.as<double>(), u.as<double>(), s->slgg.as<double>(),
@@ -651,6 +652,10 @@ extern "C" int snap_hip_solver_gradient(snap_solver_t *s, const double *resp, in
     std::swap(zeta.p, zn.p);
     it++;
     done = dmax <= tol * zmax;
+    // FP32 reverse sweeps: stop at their rounding floor, where the change
+    // (below 1e-5 of max|zeta|) no longer shrinks
+    if (!done && fp32 && dmax <= 1.0e-5 * zmax && dmax >= prev) done = true;
+    prev = dmax;
   }
   // the gradient at the converged adjoint
   SNAP_TRY(reverse(zeta));

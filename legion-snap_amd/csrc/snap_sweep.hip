@@ -395,6 +395,8 @@ struct SweepArgs {
   const int2 *batch_tiles;
   const BatchDesc *descs;
   const int *active, *done;
+  const int *prec;  // with prec, only problems whose prec[p] == prec_want
+  int prec_want;
   int64_t total_cells;
   // time-dependent kernels (TD): vdelt per job group, or vdelt0 for the
   // general variant; time-edge fluxes and MMS source [8][ng_job][cells][nang]
@@ -505,6 +507,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
   if constexpr (BATCH) {
     const int2 bt = A.batch_tiles[plane_off + blockIdx.x];
     if (!A.active[bt.x] || A.done[bt.x]) return;  // the whole workgroup
+    if (A.prec && A.prec[bt.x] != A.prec_want) return;
     const BatchDesc &P = A.descs[bt.x];
     ext = P.ext;
     nTx = P.nTx;
@@ -1609,6 +1612,8 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
     A.descs = L.batch->descs;
     A.active = L.batch->active;
     A.done = L.batch->done;
+    A.prec = L.batch->prec;
+    A.prec_want = L.batch->prec_want;
     A.total_cells = L.batch->total_cells;
   }
   if (L.td) {

@@ -89,19 +89,21 @@ class SolveOptions(ctypes.Structure):
 class BatchOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("tile", ctypes.c_int * 3),
                 ("waves_per_tile", ctypes.c_int), ("angles_per_lane", ctypes.c_int),
-                ("use_graph", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("fp64_polish", ctypes.c_double)]
 
 
 class ProblemStats(ctypes.Structure):
     _fields_ = [("inner_total", ctypes.c_int), ("outer_total", ctypes.c_int),
-                ("converged", ctypes.c_int), ("time_ms", ctypes.c_double)]
+                ("converged", ctypes.c_int), ("sweeps_fp64", ctypes.c_int),
+                ("time_ms", ctypes.c_double)]
 
 
 class BatchStats(ctypes.Structure):
     _fields_ = [("outer_iterations", ctypes.c_int), ("converged", ctypes.c_int),
                 ("solve_ms", ctypes.c_double), ("units", ctypes.c_double),
                 ("grind_ns", ctypes.c_double), ("tile", ctypes.c_int * 3), ("waves", ctypes.c_int),
-                ("angles_per_lane", ctypes.c_int), ("autotune_ms", ctypes.c_double)]
+                ("angles_per_lane", ctypes.c_int), ("autotune_ms", ctypes.c_double),
+                ("plan_cached", ctypes.c_int)]
 
 
 class IterationRecord(ctypes.Structure):

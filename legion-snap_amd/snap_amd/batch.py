@@ -11,12 +11,16 @@ import numpy as np
 
 from . import _lib
 from .problems import Problem
-from .solver import _PREC, _ptr, problem_struct
+from .solver import _PREC, MIXED_FP64_POLISH, _ptr, problem_struct
 
 
 class Batch:
     def __init__(self, problems: List[Problem], precision: str = "fp64", tile=(0, 0, 0),
-                 waves_per_tile: int = 0, angles_per_lane: int = 0, use_graph: bool = True):
+                 waves_per_tile: int = 0, angles_per_lane: int = 0, use_graph: bool = True,
+                 fp64_polish: Optional[float] = None):
+        """fp64_polish (mixed): a problem's next inner sweep runs in FP64 once
+        its previous inner of the same outer changed its flux by at most
+        fp64_polish * epsi; None = the solver's default, 0 = FP32 throughout."""
         self.problems = [q.contiguous() for q in problems]
         for q in self.problems:
             q.validate()
@@ -30,6 +34,9 @@ class Batch:
         o.waves_per_tile = waves_per_tile
         o.angles_per_lane = angles_per_lane
         o.use_graph = int(bool(use_graph))
+        if fp64_polish is None:
+            fp64_polish = MIXED_FP64_POLISH if precision == "mixed" else 0.0
+        o.fp64_polish = float(fp64_polish)
         h = ctypes.c_void_p()
         _lib.check(self.lib.snap_hip_batch_create(arr, len(self.problems), ctypes.byref(o),
                                                   ctypes.byref(h)), "snap_hip_batch_create")

@@ -44,6 +44,13 @@ def problem_struct(p: Problem) -> "_lib.Problem":
     return s
 
 
+# The mixed-precision default: with the flux change contracting by about 0.2
+# per inner iteration, the last one or two inners of a converging outer run in
+# FP64 (5 of 23 sweeps on the 64^3 P3 workload), which brings the FP32
+# sweeps' error from about 4e-6 to below 1e-7 against the FP64 solve.
+MIXED_FP64_POLISH = 25.0
+
+
 class Solver:
     """One problem on the current HIP device.
 
@@ -55,13 +62,21 @@ class Solver:
     FP64 source defect in FP32 without the fixup and add it to the base.
     wide_sweep: one workgroup per cell and sweep, with angles across up to
     1024 lanes -- the form every solve past 128 angles per octant takes.
+    sweep_kernel: "auto", "tiled" (multi-wave workgroups per tile) or
+    "wave_tile" (one wavefront per 4 x 4 x TZ tile; steady, up to 64 angles).
+    fp64_polish (mixed only): an inner runs its sweeps in FP64 when the
+    previous inner of its outer changed the scalar flux by at most
+    fp64_polish * epsi (or by at most 1e-5, the FP32 sweeps' own rounding
+    level, when epsi asks for less), so the outers end on FP64 sweeps; the
+    default for mixed precision is MIXED_FP64_POLISH, 0 sweeps everything in
+    FP32.
     """
 
     def __init__(self, problem: Problem, precision: str = FP64, tile=(0, 0, 0),
                  waves_per_tile: int = 0, first_sweep_fp64: bool = False,
                  dinv: str = "default", angles_per_lane: int = 0, refine: bool = False,
                  outer_ordered: bool = False, wide_sweep: bool = False,
-                 sweep_kernel: str = "auto", fp64_polish: float = 0.0):
+                 sweep_kernel: str = "auto", fp64_polish: Optional[float] = None):
         problem.validate()
         self.problem = p = problem.contiguous()
         self.lib = _lib.load()
@@ -78,6 +93,8 @@ class Solver:
         o.outer_ordered = int(bool(outer_ordered))
         o.wide_sweep = int(bool(wide_sweep))
         o.sweep_kernel = {"auto": 0, "tiled": 1, "wave_tile": 2}[sweep_kernel]
+        if fp64_polish is None:
+            fp64_polish = MIXED_FP64_POLISH if precision == MIXED else 0.0
         o.fp64_polish = float(fp64_polish)
         self.precision = precision
         h = ctypes.c_void_p()
@@ -111,9 +128,11 @@ class Solver:
     def gradient(self, resp: np.ndarray, precision: Optional[str] = None,
                  tol: Optional[float] = None, maxit: int = 2000):
         """dR/dt_xs[g, z, y, x] of R = sum(resp * flux0) at the current flux
-        (run() first; nmom = 1).  precision "fp64" or "mixed" (FP32 sweeps);
-        tol defaults to 1e-12 (FP64) or 1e-6 (FP32, whose rounding stalls the
-        adjoint iteration near 1e-7).  Returns (grad, R, adjoint iterations)."""
+        (run() first; nmom = 1).  precision "fp64" or "mixed" (FP32 reverse
+        sweeps; the forward recompute and its fixup decisions stay FP64);
+        tol defaults to 1e-12 (FP64) or 1e-9 (FP32, whose iteration also ends
+        where its rounding stops the change from shrinking, near 1e-9 on the
+        48^3 gradient workload).  Returns (grad, R, adjoint iterations)."""
         p = self.problem
         r = np.ascontiguousarray(resp, dtype=np.float64).reshape(p.ng, p.nz, p.ny, p.nx)
         grad = np.zeros_like(r)
@@ -121,7 +140,7 @@ class Solver:
         it = ctypes.c_int()
         prec = _PREC[precision or ("mixed" if self.precision == "mixed" else "fp64")]
         if tol is None:
-            tol = 1e-6 if prec == _lib.PRECISION_MIXED else 1e-12
+            tol = 1e-9 if prec == _lib.PRECISION_MIXED else 1e-12
         _lib.check(self.lib.snap_hip_solver_gradient(self._h, _ptr(r), prec, tol, maxit, _ptr(grad),
                                                      ctypes.byref(R), ctypes.byref(it)),
                    "snap_hip_solver_gradient")

@@ -29,8 +29,13 @@ class OracleProblem(ctypes.Structure):
         ("mat", _ip), ("sigt", _dp), ("slgg", _dp), ("qi", _dp), ("sigt_cell", _dp),
         ("epsi", ctypes.c_double), ("iitm", ctypes.c_int), ("oitm", ctypes.c_int),
         ("fixup", ctypes.c_int), ("timedep", ctypes.c_int), ("tf", ctypes.c_double),
-        ("nsteps", ctypes.c_int), ("mms", ctypes.c_int),
+        ("nsteps", ctypes.c_int), ("mms", ctypes.c_int), ("sweep_order", ctypes.c_int),
     ]
+
+
+# operation orders of the sweep body: the reference's generic CPU variant
+# (sweep.cc:242-616) and its AVX variant (sweep.cc:1037-1520, nang % 4 == 0)
+ORDERS = {"generic": 0, "avx": 1}
 
 
 class OracleStats(ctypes.Structure):
@@ -73,6 +78,8 @@ def load():
         lib.snap_oracle_sweep.argtypes = (
             [ctypes.c_int] * 6 + [ctypes.c_double] * 3 + [_dp] * 8 + [ctypes.c_int] + [_dp] * 8)
         lib.snap_oracle_sweep.restype = ctypes.c_int
+        lib.snap_oracle_sweep_avx.argtypes = lib.snap_oracle_sweep.argtypes
+        lib.snap_oracle_sweep_avx.restype = ctypes.c_int
         lib.snap_oracle_sweep_td.argtypes = (
             [ctypes.c_int] * 6 + [ctypes.c_double] * 3 + [_dp] * 8 + [ctypes.c_int]
             + [ctypes.c_double] + [_dp] * 8)
@@ -177,24 +184,28 @@ def converged(flux0, flux0p, epsi) -> bool:
 
 
 def sweep(nx, ny, nz, nang, cmom, corner, hijk, quad, qtot_g, dinv_g, t_xs_g, fixup=True,
-          ghost_in=(None, None, None), want_ghost_out=False):
-    """One (corner, group) MiniKBA sweep (sweep.cc:242-616).  Returns
+          ghost_in=(None, None, None), want_ghost_out=False, order: str = "generic"):
+    """One (corner, group) MiniKBA sweep (sweep.cc:242-616, or the AVX
+    variant's operation order, sweep.cc:1037-1520, with order="avx").  Returns
     (part_flux [nz,ny,nx], part_fluxm [nz,ny,nx,cmom-1] or None, fixup_cells, ghost_out)."""
     lib = load()
+    fn = lib.snap_oracle_sweep_avx if ORDERS[order] else lib.snap_oracle_sweep
     mu, eta, xi, w, ec = quad
     pf = np.zeros((nz, ny, nx))
     pm = np.zeros((nz, ny, nx, cmom - 1)) if cmom > 1 else None
     outs = (np.zeros((nz, ny, nang)), np.zeros((nz, nx, nang)), np.zeros((ny, nx, nang))) \
         if want_ghost_out else (None, None, None)
-    n = lib.snap_oracle_sweep(nx, ny, nz, nang, cmom, corner, hijk[0], hijk[1], hijk[2],
-                              _p(mu), _p(eta), _p(xi), _p(w), _p(ec), _p(qtot_g), _p(dinv_g),
-                              _p(t_xs_g), int(bool(fixup)), _p(ghost_in[0]), _p(ghost_in[1]),
-                              _p(ghost_in[2]), _p(outs[0]), _p(outs[1]), _p(outs[2]),
-                              _p(pf), _p(pm))
+    n = fn(nx, ny, nz, nang, cmom, corner, hijk[0], hijk[1], hijk[2],
+           _p(mu), _p(eta), _p(xi), _p(w), _p(ec), _p(qtot_g), _p(dinv_g),
+           _p(t_xs_g), int(bool(fixup)), _p(ghost_in[0]), _p(ghost_in[1]),
+           _p(ghost_in[2]), _p(outs[0]), _p(outs[1]), _p(outs[2]),
+           _p(pf), _p(pm))
+    if n < 0:
+        raise ValueError("the AVX order needs nang % 4 == 0")
     return pf, pm, n, outs
 
 
-def _problem_struct(p, keep):
+def _problem_struct(p, keep, order: str = "generic"):
     q = p.contiguous()
     keep.append(q)
     s = OracleProblem()
@@ -208,6 +219,7 @@ def _problem_struct(p, keep):
     s.sigt_cell = _p(q.sigt_cell)
     s.epsi, s.iitm, s.oitm, s.fixup = q.epsi, q.iitm, q.oitm, int(bool(q.fixup))
     s.timedep, s.tf, s.nsteps, s.mms = int(bool(q.timedep)), q.tf, q.nsteps, int(bool(q.mms))
+    s.sweep_order = ORDERS[order]
     return s
 
 
@@ -256,13 +268,14 @@ def mms_compare(flux0, ref):
     return float(out[0]), float(out[1]), float(out[2])
 
 
-def solve(p, nthreads: int = 0):
-    """transport_solve (snap.cc:386-541), every time step.  Returns (flux0 [ng,nz,ny,nx],
-    fluxm [ng,nz,ny,nx,cmom-1] or None, stats dict)."""
+def solve(p, nthreads: int = 0, order: str = "generic"):
+    """transport_solve (snap.cc:386-541), every time step, with the sweeps in
+    the generic or the AVX variant's operation order.  Returns (flux0
+    [ng,nz,ny,nx], fluxm [ng,nz,ny,nx,cmom-1] or None, stats dict)."""
     lib = load()
     p.validate()
     keep = []
-    s = _problem_struct(p, keep)
+    s = _problem_struct(p, keep, order)
     flux0 = np.zeros((p.ng, p.nz, p.ny, p.nx))
     fluxm = np.zeros((p.ng, p.nz, p.ny, p.nx, p.cmom - 1)) if p.cmom > 1 else None
     st = OracleStats()
@@ -306,11 +319,11 @@ def adjoint(p, flux0, resp, tol: float = 1e-13, maxit: int = 500, nthreads: int 
     return grad, zeta, it
 
 
-def time_sweeps(p, ninner: int = 1, nthreads: int = 0) -> float:
+def time_sweeps(p, ninner: int = 1, nthreads: int = 0, order: str = "generic") -> float:
     """Seconds for `ninner` inner iterations' sweeps (all octants and groups)."""
     lib = load()
     keep = []
-    s = _problem_struct(p, keep)
+    s = _problem_struct(p, keep, order)
     t = lib.snap_oracle_time_sweeps(ctypes.byref(s), ninner, nthreads)
     if t < 0:
         raise RuntimeError("oracle timing failed")

@@ -14,7 +14,23 @@
  *    (sweep.cc:361 loops forever; sweep.cc:1135 and outer.cc:205 use nmom);
  *  - all flux moments are re-zeroed each sweep (snap.cc:494 zeroes flux0 only);
  *  - octant contributions are added in octant order 0..7, one deterministic
- *    instance of the reference's unordered SumReduction (snap.cc:1832-1880).
+ *    instance of the reference's unordered SumReduction (snap.cc:1832-1880);
+ *  - in the AVX order (below) the flux moments sum all four lanes,
+ *    (l0 + l1) + (l2 + l3): the AVX variant applies _mm256_hadd_pd twice and
+ *    keeps 2 (l0 + l1), dropping lanes 2 and 3 (sweep.cc:1485-1492).
+ *
+ * Two operation orders of the sweep body (the `avx` argument of sweep_impl):
+ *  - generic: MiniKBATask::cpu_implementation, sweep.cc:242-616;
+ *  - AVX: MiniKBATask::avx_implementation, sweep.cc:1037-1520, the variant
+ *    the reference registers on x86 with AVX (sweep.cc:165-179); it needs
+ *    nang % 4 == 0 (sweep.cc:1064).  Angles a = 4j + k sit in lane k of
+ *    vector j.  It differs from the generic order in (i) the fixup update's
+ *    products, psi_in * (mu * (hi * (1 + hv))) and a denominator
+ *    t_xs + ((mu hi hv_x + eta hj hv_y) + xi hk hv_z) (sweep.cc:1349-1393),
+ *    (ii) NaN tests (!(f >= 0) counts as negative, sweep.cc:1229-1232; a
+ *    denominator is kept only where pc > 0, sweep.cc:1371-1375), and (iii)
+ *    the angle sums: each lane adds its angles in order from 0, then
+ *    total = (l0 + l1) + (l2 + l3) (sweep.cc:1465-1475).
  */
 #include "snap_oracle.h"
 
@@ -215,7 +231,8 @@ static int sweep_impl(int nx, int ny, int nz, int nang, int cmom, int corner, do
                       const double *ghost_z_in, double *ghost_x_out, double *ghost_y_out,
                       double *ghost_z_out, double *part_flux, double *part_fluxm,
                       double vdelt, const double *time_flux_in, double *time_flux_out,
-                      const double *qim_g) {
+                      const double *qim_g, int avx) {
+  if (avx && nang % 4 != 0) return -1;
   const int sxp = (corner & 1) != 0, syp = (corner & 2) != 0, szp = (corner & 4) != 0;
   const size_t ab = (size_t)nang;
   double *buf = (double *)malloc(sizeof(double) * ab * 16);
@@ -297,21 +314,21 @@ static int sweep_impl(int nx, int ny, int nz, int nang, int cmom, int corner, do
             unsigned negative_fluxes = 0;
             for (int a = 0; a < nang; a++) {
               fx_hv_x[a] = 2.0 * pc[a] - psii[a];
-              if (fx_hv_x[a] < 0.0) {
+              if (avx ? !(fx_hv_x[a] >= 0.0) : fx_hv_x[a] < 0.0) {
                 hv_x[a] = 0.0;
                 negative_fluxes++;
               }
             }
             for (int a = 0; a < nang; a++) {
               fx_hv_y[a] = 2.0 * pc[a] - psij[a];
-              if (fx_hv_y[a] < 0.0) {
+              if (avx ? !(fx_hv_y[a] >= 0.0) : fx_hv_y[a] < 0.0) {
                 hv_y[a] = 0.0;
                 negative_fluxes++;
               }
             }
             for (int a = 0; a < nang; a++) {
               fx_hv_z[a] = 2.0 * pc[a] - psik[a];
-              if (fx_hv_z[a] < 0.0) {
+              if (avx ? !(fx_hv_z[a] >= 0.0) : fx_hv_z[a] < 0.0) {
                 hv_z[a] = 0.0;
                 negative_fluxes++;
               }
@@ -319,7 +336,7 @@ static int sweep_impl(int nx, int ny, int nz, int nang, int cmom, int corner, do
             if (vdelt != 0.0) {
               for (int a = 0; a < nang; a++) {
                 fx_hv_t[a] = 2.0 * pc[a] - tin[a];
-                if (fx_hv_t[a] < 0.0) {
+                if (avx ? !(fx_hv_t[a] >= 0.0) : fx_hv_t[a] < 0.0) {
                   hv_t[a] = 0.0;
                   negative_fluxes++;
                 }
@@ -328,7 +345,24 @@ static int sweep_impl(int nx, int ny, int nz, int nang, int cmom, int corner, do
             if (negative_fluxes == old_negative_fluxes) break;
             old_negative_fluxes = negative_fluxes;
             ran = 1;
-            if (vdelt != 0.0) {
+            if (avx) {
+              /* sweep.cc:1302-1393 */
+              for (int a = 0; a < nang; a++) {
+                double sum = psii[a] * (mu[a] * (hi * (1.0 + hv_x[a])));
+                sum = sum + psij[a] * (eta[a] * (hj * (1.0 + hv_y[a])));
+                sum = sum + psik[a] * (xi[a] * (hk * (1.0 + hv_z[a])));
+                double den3 = (mu[a] * hi) * hv_x[a] + (eta[a] * hj) * hv_y[a];
+                den3 = den3 + (xi[a] * hk) * hv_z[a];
+                if (vdelt != 0.0) {
+                  sum = sum + tin[a] * (vdelt * (1.0 + hv_t[a]));
+                  den3 = den3 + vdelt * hv_t[a];
+                }
+                pc[a] = psi[a] + 0.5 * sum;
+                double den = t_xs + den3;
+                if (!(pc[a] > 0.0)) den = 0.0;
+                pc[a] = (den >= tolr) ? pc[a] / den : 0.0;
+              }
+            } else if (vdelt != 0.0) {
               for (int a = 0; a < nang; a++) {
                 pc[a] = psi[a] + 0.5 * (psii[a] * mu[a] * hi * (1.0 + hv_x[a]) +
                                         psij[a] * eta[a] * hj * (1.0 + hv_y[a]) +
@@ -388,7 +422,22 @@ static int sweep_impl(int nx, int ny, int nz, int nang, int cmom, int corner, do
         } else {
           memcpy(zflux_plane + ((size_t)y * nx + x) * ab, psik, sizeof(double) * ab);
         }
-        /* moment reductions (sweep.cc:568-591) */
+        /* moment reductions (sweep.cc:568-591; AVX lanes, sweep.cc:1465-1493) */
+        if (avx) {
+          double lane[4] = {0.0, 0.0, 0.0, 0.0};
+          for (int a = 0; a < nang; a++) {
+            psi[a] = pc[a] * w[a];
+            lane[a & 3] = lane[a & 3] + psi[a];
+          }
+          part_flux[c] = (lane[0] + lane[1]) + (lane[2] + lane[3]);
+          for (int l = 1; l < cmom; l++) {
+            const double *e = ecb + (size_t)l * nang;
+            lane[0] = lane[1] = lane[2] = lane[3] = 0.0;
+            for (int a = 0; a < nang; a++) lane[a & 3] = lane[a & 3] + psi[a] * e[a];
+            part_fluxm[c * (cmom - 1) + (l - 1)] = (lane[0] + lane[1]) + (lane[2] + lane[3]);
+          }
+          continue;
+        }
         double total = 0.0;
         for (int a = 0; a < nang; a++) {
           psi[a] = w[a] * pc[a];
@@ -423,7 +472,21 @@ int snap_oracle_sweep(int nx, int ny, int nz, int nang, int cmom, int corner,
                       double *part_flux, double *part_fluxm) {
   return sweep_impl(nx, ny, nz, nang, cmom, corner, hi, hj, hk, mu, eta, xi, w, ec, qtot_g,
                     dinv_g, t_xs_g, fixup, ghost_x_in, ghost_y_in, ghost_z_in, ghost_x_out,
-                    ghost_y_out, ghost_z_out, part_flux, part_fluxm, 0.0, NULL, NULL, NULL);
+                    ghost_y_out, ghost_z_out, part_flux, part_fluxm, 0.0, NULL, NULL, NULL, 0);
+}
+
+int snap_oracle_sweep_avx(int nx, int ny, int nz, int nang, int cmom, int corner,
+                          double hi, double hj, double hk, const double *mu,
+                          const double *eta, const double *xi, const double *w,
+                          const double *ec, const double *qtot_g,
+                          const double *dinv_g, const double *t_xs_g, int fixup,
+                          const double *ghost_x_in, const double *ghost_y_in,
+                          const double *ghost_z_in, double *ghost_x_out,
+                          double *ghost_y_out, double *ghost_z_out,
+                          double *part_flux, double *part_fluxm) {
+  return sweep_impl(nx, ny, nz, nang, cmom, corner, hi, hj, hk, mu, eta, xi, w, ec, qtot_g,
+                    dinv_g, t_xs_g, fixup, ghost_x_in, ghost_y_in, ghost_z_in, ghost_x_out,
+                    ghost_y_out, ghost_z_out, part_flux, part_fluxm, 0.0, NULL, NULL, NULL, 1);
 }
 
 int snap_oracle_sweep_td(int nx, int ny, int nz, int nang, int cmom, int corner, double hi,
@@ -437,7 +500,7 @@ int snap_oracle_sweep_td(int nx, int ny, int nz, int nang, int cmom, int corner,
   return sweep_impl(nx, ny, nz, nang, cmom, corner, hi, hj, hk, mu, eta, xi, w, ec, qtot_g,
                     dinv_g, t_xs_g, fixup, NULL, NULL, NULL, ghost_x_out, ghost_y_out,
                     ghost_z_out, part_flux, part_fluxm, vdelt, time_flux_in, time_flux_out,
-                    qim_g);
+                    qim_g, 0);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -457,12 +520,15 @@ typedef struct {
   const double *tflux_in;
   double *tflux_out;
   const double *qim;              /* MMS angular source [8][ng][cells][nang] or NULL */
+  int avx;                        /* the AVX variant's operation order */
 } oracle_state_t;
 
 static int problem_steps(const oracle_problem_t *p) { return p->nsteps > 0 ? p->nsteps : 1; }
 
 static int state_init(oracle_state_t *s, const oracle_problem_t *p) {
   memset(s, 0, sizeof(*s));
+  if (p->sweep_order != 0 && (p->sweep_order != 1 || p->nang % 4 != 0)) return -1;
+  s->avx = p->sweep_order;
   s->nx = p->nx;
   s->ny = p->ny;
   s->nz = p->nz;
@@ -540,7 +606,7 @@ static int perform_sweeps(oracle_state_t *s, int fixup, const double *qtot,
         s->part_flux + (size_t)corner * gc + (size_t)g * nc,
         cm > 1 ? s->part_fluxm + ((size_t)corner * gc + (size_t)g * nc) * (cm - 1) : NULL, vd,
         vd != 0.0 ? s->tflux_in + ta : NULL, vd != 0.0 ? s->tflux_out + ta : NULL,
-        s->qim ? s->qim + ta : NULL);
+        s->qim ? s->qim + ta : NULL, s->avx);
   }
 #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < gc; i++) {

@@ -44,6 +44,8 @@ typedef struct {
   double tf;                /* total simulated time; dt = tf / nsteps (snap.cc:1174) */
   int nsteps;               /* time steps (0 counts as 1) */
   int mms;                  /* manufactured-solution source (src_opt 3): qi is ignored */
+  int sweep_order;          /* 0: generic variant's operation order (sweep.cc:242-616);
+                               1: AVX variant's (sweep.cc:1037-1520, nang % 4 == 0) */
 } oracle_problem_t;
 
 typedef struct {
@@ -104,6 +106,18 @@ int snap_oracle_sweep(int nx, int ny, int nz, int nang, int cmom, int corner,
                       const double *ghost_z_in, double *ghost_x_out,
                       double *ghost_y_out, double *ghost_z_out,
                       double *part_flux, double *part_fluxm);
+/* The same sweep in the AVX variant's operation order (sweep.cc:1037-1520;
+ * the differences are listed in snap_oracle.c).  Returns -1 unless
+ * nang % 4 == 0. */
+int snap_oracle_sweep_avx(int nx, int ny, int nz, int nang, int cmom, int corner,
+                          double hi, double hj, double hk, const double *mu,
+                          const double *eta, const double *xi, const double *w,
+                          const double *ec, const double *qtot_g,
+                          const double *dinv_g, const double *t_xs_g, int fixup,
+                          const double *ghost_x_in, const double *ghost_y_in,
+                          const double *ghost_z_in, double *ghost_x_out,
+                          double *ghost_y_out, double *ghost_z_out,
+                          double *part_flux, double *part_fluxm);
 
 /* The sweep above from vacuum inflow, with the time-dependent terms (vdelt
  * != 0: inflow time_flux_in and outflow time_flux_out, [cells][nang] of this

@@ -17,22 +17,41 @@ def _small_batch(count=5, seed=17):
 
 
 @pytest.mark.parametrize("use_graph", [True, False])
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_batch_equals_single_solves(gpu, precision, use_graph):
+@pytest.mark.parametrize("precision,polish", [("fp64", None), ("mixed", 0.0), ("mixed", None)])
+def test_batch_equals_single_solves(gpu, precision, polish, use_graph):
     probs = _small_batch()
-    with Batch(probs, precision=precision, use_graph=use_graph) as b:
+    with Batch(probs, precision=precision, use_graph=use_graph, fp64_polish=polish) as b:
         st, per = b.run()
         fluxes = [b.flux(i) for i in range(len(probs))]
-    print(st)
+    print(st, [(x["sweeps_fp64"], x["inner_total"], round(x["time_ms"], 3)) for x in per])
     assert st["converged"] == len(probs)
     for i, p in enumerate(probs):
-        with Solver(p, precision=precision) as s:
+        # the same polish rule (FP32 throughout with 0) picks the same sweeps
+        with Solver(p, precision=precision, fp64_polish=polish) as s:
             ss = s.run()
             f, fm = s.flux()
         assert (per[i]["inner_total"], per[i]["outer_total"]) == (ss["inner_total"], ss["outer_total"])
+        assert per[i]["sweeps_fp64"] == ss["sweeps_fp64"]
         assert np.array_equal(fluxes[i][0], f)
         if fm is not None:
             assert np.array_equal(fluxes[i][1], fm)
+        assert 0.0 < per[i]["time_ms"] <= st["solve_ms"]
+    if precision == "mixed" and polish is None:
+        assert any(x["sweeps_fp64"] > 0 for x in per)
+
+
+def test_batch_reuses_the_tuned_tile(gpu):
+    probs = _small_batch(count=3, seed=9)
+    with Batch(probs, precision="fp64") as b:
+        st1, _ = b.run()
+        f1 = b.flux(2)[0]
+    with Batch(probs, precision="fp64") as b:
+        st2, _ = b.run()
+        f2 = b.flux(2)[0]
+    assert st1["plan_cached"] == 0 and st1["autotune_ms"] > 0
+    assert st2["plan_cached"] == 1 and st2["autotune_ms"] == 0
+    assert st2["tile"] == st1["tile"] and st2["angles_per_lane"] == st1["angles_per_lane"]
+    assert np.array_equal(f1, f2)
 
 
 def test_batch_with_fixed_tile(gpu):

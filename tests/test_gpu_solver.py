@@ -4,9 +4,9 @@ Bars, from the requirement on this path:
 * FP64 in the reference's operation order ("fp64_exact"): bit-equal scalar
   flux and flux moments, identical inner/outer counts;
 * FP64 with fused multiply-adds ("fp64"): <= 1e-12 relative, identical counts;
-* mixed precision (FP32 sweep, FP64 moments/sources/tests): max relative
-  error <= 5e-6 and L2 relative error <= 1e-6 against the FP64 oracle,
-  identical counts on these cases (the measured errors are printed).
+* mixed precision (FP32 sweeps, FP64 moments/sources/tests, the inners that
+  end an outer in FP64): max and L2 relative error <= 1e-6 against the FP64
+  oracle, identical counts (the measured errors are printed).
 """
 import os
 
@@ -81,8 +81,22 @@ def test_mixed_precision_error(gpu, oracle_mod, name):
     f, fm, st = gpu_solve(CASES[name](), "mixed")
     mx, l2 = rel_errors(f, fo)
     print(name, "mixed", f"maxrel={mx:.3e} l2rel={l2:.3e}", st)
-    assert st["sweeps_fp32"] == st["inner_total"]
+    assert st["sweeps_fp32"] > 0 and st["sweeps_fp32"] + st["sweeps_fp64"] == st["inner_total"]
     assert (st["inner_total"], st["outer_total"]) == (sto["inner_total"], sto["outer_total"])
+    assert mx <= 1e-6 and l2 <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["config0", "config1_small", "config1_ragged"])
+def test_fp32_sweeps_without_polish(gpu, oracle_mod, name):
+    # every sweep in FP32 (fp64_polish = 0): a regression guard for the FP32
+    # kernels only -- this form's rounding error (about 4e-6 max on the 64^3
+    # workload) does not meet the 1e-6 requirement, which the polished form
+    # above is tested against
+    fo, _, sto = oracle_solution(oracle_mod, name)
+    f, _, st = gpu_solve(CASES[name](), "mixed", fp64_polish=0.0)
+    mx, l2 = rel_errors(f, fo)
+    print(name, "fp32 sweeps", f"maxrel={mx:.3e} l2rel={l2:.3e}")
+    assert st["sweeps_fp32"] == st["inner_total"]
     assert mx <= 5e-6 and l2 <= 1e-6
 
 
@@ -105,7 +119,7 @@ def test_mixed_defect_correction(gpu, oracle_mod):
     # config 0 has few fixed cells: sweeping only the FP64 source defect in
     # FP32 brings the FP32 error from ~1e-6 down by orders of magnitude
     fo, _, sto = oracle_solution(oracle_mod, "config0")
-    f, _, st = gpu_solve(CASES["config0"](), "mixed", refine=True)
+    f, _, st = gpu_solve(CASES["config0"](), "mixed", refine=True, fp64_polish=0.0)
     mx, l2 = rel_errors(f, fo)
     print("config0 mixed refine", f"maxrel={mx:.3e} l2rel={l2:.3e}", st)
     assert (st["inner_total"], st["outer_total"]) == (sto["inner_total"], sto["outer_total"])
@@ -151,8 +165,8 @@ def test_mixed_options(gpu, oracle_mod, kw):
     print(kw, f"maxrel={mx:.3e} l2rel={l2:.3e}", st)
     assert (st["inner_total"], st["outer_total"]) == (sto["inner_total"], sto["outer_total"])
     if kw.get("first_sweep_fp64"):
-        assert (st["sweeps_fp64"], st["sweeps_fp32"]) == (1, st["inner_total"] - 1)
-    assert mx <= 5e-6 and l2 <= 1e-6
+        assert st["sweeps_fp64"] >= 1
+    assert mx <= 1e-6 and l2 <= 1e-6
 
 
 def test_dinv_modes_agree(gpu):

@@ -77,7 +77,7 @@ def test_mixed_within_bar(gpu, oracle_mod, name):
     l2 = float(np.linalg.norm(f - fo) / np.linalg.norm(fo))
     print(name, f"mixed maxrel={rel:.2e} l2={l2:.2e}")
     assert (st["inner_total"], st["outer_total"]) == (sto["inner_total"], sto["outer_total"])
-    assert rel <= 5e-6 and l2 <= 1e-6
+    assert rel <= 1e-6 and l2 <= 1e-6
 
 
 def test_adjoint_with_wave_tile_solver(gpu, oracle_mod):

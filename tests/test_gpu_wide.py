@@ -92,7 +92,7 @@ def test_wide_mixed_precision(gpu, oracle_mod, name):
     mx = float(np.max(d / np.maximum(np.abs(fo), 1e-300)))
     l2 = float(np.linalg.norm(d) / np.linalg.norm(fo))
     print(name, f"mixed maxrel={mx:.3e} l2rel={l2:.3e}")
-    assert l2 <= 1e-5 and mx <= 1e-4
+    assert l2 <= 1e-6 and mx <= 1e-6
 
 
 @pytest.mark.parametrize("timedep", [False, True])
