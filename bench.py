@@ -39,13 +39,14 @@ TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "sweep_config1_fp64_traffic.jso
 
 
 def measured_traffic(size: int):
-    """HBM bytes per launch of the FP64 sweep kernel from the committed
+    """HBM bytes of one inner sweep of the FP64 sweep kernel (all its
+    launches, the unit `achieved` is priced over) from the committed
     FETCH_SIZE / WRITE_SIZE profile of this workload (calibrated against a
     plain copy), or None when no profile covers it."""
     if size != 64 or not os.path.exists(TRAFFIC_PROFILE):
         return None
     with open(TRAFFIC_PROFILE) as f:
-        return float(json.load(f)["bytes_per_launch"])
+        return float(json.load(f)["bytes_per_sweep"])
 
 
 def bytes_per_unit(nang: int, cmom: int, real_bytes: int) -> float:
@@ -355,11 +356,13 @@ def main() -> int:
                              "solve_s": st["solve_ms"] / 1e3, "inner": st["inner_total"],
                              "maxrel_vs_fp64_fused": float(np.max(np.abs(fex - f64) / np.abs(fex)))}
 
-    # ---- mixed precision (FP32 sweeps, FP64 moments / sources / tests): the
-    # default with FP64 polish sweeps at the end of each outer, FP32 sweeps
-    # throughout, and per-outer defect correction
-    for key, kw in (("mixed", {}), ("mixed_fp32_only", {"fp64_polish": 0.0}),
-                    ("mixed_refine", {"refine": True, "fp64_polish": 0.0})):
+    # ---- mixed precision (FP64 moments / sources / tests): the default
+    # (FP32 sweeps, FP64 polish sweeps at the end of each outer), per-outer
+    # defect correction with FP32 correction sweeps and FP64 polish, and the
+    # two without polish
+    for key, kw in (("mixed", {}), ("mixed_refine", {"refine": True}),
+                    ("mixed_fp32_only", {"fp64_polish": 0.0}),
+                    ("mixed_refine_no_polish", {"refine": True, "fp64_polish": 0.0})):
         with Solver(p, precision="mixed", **kw) as s:
             for _ in range(args.warmup):
                 s.run()

@@ -147,10 +147,13 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   const int tix = blockIdx.x * kWGW + wave;
   if (tix >= plane_count) return;
   // per wavefront: W: w*pc of the plane's cells [slot][angle]; P: their
-  // angular sources (fast path) or their source moments [slot][16] (general)
+  // angular sources (fast path) or their source moments [slot][16]
+  // (general); Z: the plane's z inflows [slot][angle] while the fixup may
+  // sweep it again (fast path)
   double *W = EA + (size_t)NST * 64 +
-              (size_t)wave * (kNP * RS + 64 + (kNP * RS > kNP * 16 ? kNP * RS : kNP * 16));
+              (size_t)wave * (2 * (kNP * RS + 64) + (kNP * RS > kNP * 16 ? kNP * RS : kNP * 16));
   double *P = W + kNP * RS + 64;
+  Real *Zb = (Real *)(P + (kNP * RS > kNP * 16 ? kNP * RS : kNP * 16));
   Real *qs = (Real *)P;
   const int tp = A.plane_tiles[plane_off + tix];
   const int tx = tp & 1023, ty = (tp >> 10) & 1023, tz = (tp >> 20) & 1023;
@@ -229,11 +232,16 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   // W slot of the lane in row n: its angle's column, or (lanes past the
   // row) its own slot of a junk row after the 16 rows, which nothing reads
   auto wslot = [&](int n) { return lane < RS ? n * RS + lane : kNP * RS + lane; };
+  // Z slot of cell n: by angle, so that lanes past nang (which mirror angle
+  // 0 and store its faces again) read and write angle 0's slot, with the
+  // values angle 0 has
+  auto zslot = [&](int n) { return n * RS + a; };
 
   // ---- one cell of the general path: DD update, the reference's fixup loop
-  // (sweep.cc:402-520) and the outflows; returns pc
+  // (sweep.cc:402-520) and the outflows; returns pc (fixed: some outflow of
+  // the cell came out negative, so the fixup loop ran)
   auto cell_step = [&](Real psi, Real d, Real pi_, Real pj_, Real pk_, int c, Real &ox, Real &oy,
-                       Real &oz) -> Real {
+                       Real &oz, bool &fixed) -> Real {
     Real pc;
     if constexpr (EXACT) {
       Real p = psi;
@@ -264,6 +272,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
                (unsigned)__popcll(__builtin_amdgcn_ballot_w64(aok & nzn));
       };
       unsigned old_neg = count_negative();
+      fixed = old_neg != 0;
       if (old_neg != 0) {
         const Real txs = tg[c];
         // a pass that clears no new flag leaves pc unchanged, so the pass
@@ -415,6 +424,11 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
     tv_load(0);
     dv_load(0);
     load_xy_inflows(0, xin, yin);
+    // the planes that need the fixup cluster (near-zero flux regions): after
+    // a plane swept again with the fixup, the next plane goes straight to the
+    // cell-by-cell pass (the same arithmetic, without the discarded
+    // branch-free pass); after one without any fixup, back to step (2)
+    bool predict_fixup = false;
     for (int k = 0; k < T.ez; k++) {
       // (1) psi[slot][angle] = sum_l q[slot][l] ec[l][angle] for each block of
       // 16 angles: A = q (cells x moments), B = ec (moments x angles), D[cell]
@@ -453,53 +467,57 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
       // independent cells are in flight (the z inflows are kept in LDS for
       // step 3)
       if (A.fixup) {
-        // the tile's own z outflow face holds the plane's z inflows until the
-        // tile's last plane writes its real outflows there
 #pragma unroll
-        for (int n = 0; n < kNP; n++) out_z[(size_t)n * nang + a] = zf[n];
+        for (int n = 0; n < kNP; n++) Zb[zslot(n)] = zf[n];
       }
-      bool neg = false;
+      bool redo = predict_fixup;
+      if (!predict_fixup) {
+        bool neg = false;
 #pragma unroll
-      for (int dg2 = 0; dg2 < kSX + kSY - 1; dg2++) {
+        for (int dg2 = 0; dg2 < kSX + kSY - 1; dg2++) {
 #pragma unroll
-        for (int i = 0; i < kSX; i++) {
-          const int j = dg2 - i;
-          if (j < 0 || j >= kSY) continue;  // resolved at compile time
-          const int n = j * kSX + i;
-          const Real psi = (Real)P[n * RS + a];
-          Real d;
-          if constexpr (REC)
-            d = dinv_of(bcast(tv_c, n));
-          else
-            d = dv_c[n];
-          const Real pi_ = (i == 0) ? xin[j] : xf[j];
-          const Real pj_ = (j == 0) ? yin[i] : yf[i];
-          const Real pk_ = zf[n];
-          Real p = fmar(pi_, cx, psi);
-          p = fmar(pj_, cy, p);
-          p = fmar(pk_, cz, p);
-          const Real pc = p * d;
-          const Real p2 = Real(2) * pc;
-          const Real ox = p2 - pi_, oy = p2 - pj_, oz = p2 - pk_;
-          neg = neg | (ox < Real(0)) | (oy < Real(0)) | (oz < Real(0));
-          xf[j] = ox;
-          yf[i] = oy;
-          zf[n] = oz;
-          // lanes past nang mirror lane 0 and store its values again
-          if (i == kSX - 1) out_x[(size_t)(k * kSY + j) * nang + a] = ox;
-          if (j == kSY - 1) out_y[(size_t)(k * kSX + i) * nang + a] = oy;
-          W[wslot(n)] = aok ? w_d * (double)pc : 0.0;
+          for (int i = 0; i < kSX; i++) {
+            const int j = dg2 - i;
+            if (j < 0 || j >= kSY) continue;  // resolved at compile time
+            const int n = j * kSX + i;
+            const Real psi = (Real)P[n * RS + a];
+            Real d;
+            if constexpr (REC)
+              d = dinv_of(bcast(tv_c, n));
+            else
+              d = dv_c[n];
+            const Real pi_ = (i == 0) ? xin[j] : xf[j];
+            const Real pj_ = (j == 0) ? yin[i] : yf[i];
+            const Real pk_ = zf[n];
+            Real p = fmar(pi_, cx, psi);
+            p = fmar(pj_, cy, p);
+            p = fmar(pk_, cz, p);
+            const Real pc = p * d;
+            const Real p2 = Real(2) * pc;
+            const Real ox = p2 - pi_, oy = p2 - pj_, oz = p2 - pk_;
+            neg = neg | (ox < Real(0)) | (oy < Real(0)) | (oz < Real(0));
+            xf[j] = ox;
+            yf[i] = oy;
+            zf[n] = oz;
+            // lanes past nang mirror lane 0 and store its values again
+            if (i == kSX - 1) out_x[(size_t)(k * kSY + j) * nang + a] = ox;
+            if (j == kSY - 1) out_y[(size_t)(k * kSX + i) * nang + a] = oy;
+            W[wslot(n)] = aok ? w_d * (double)pc : 0.0;
+          }
+          __builtin_amdgcn_sched_barrier(0);
         }
-        __builtin_amdgcn_sched_barrier(0);
+        redo = A.fixup && __builtin_amdgcn_ballot_w64(aok & neg) != 0;
       }
       // (3) a negative outflow anywhere in the plane: sweep it again cell by
       // cell with the fixup, from the plane's z inflows (zf) and tile inflows
-      if (A.fixup && __builtin_amdgcn_ballot_w64(aok & neg) != 0) {
-        // A rolled loop, so that this rarer path does not set the register
+      predict_fixup = false;
+      if (redo) {
+        // A rolled loop, so that this path does not set the register
         // budget: the y inflows of a row are the outflows of four cells
-        // earlier (a four-entry queue), the z inflows come from the backup
-        // in the tile's z face and each z outflow overwrites its own, now
-        // consumed, backup slot; tile inflows and dinv are read again.
+        // earlier (a four-entry queue), the z inflows come from the LDS
+        // backup and each z outflow overwrites its own, now consumed, backup
+        // slot; the tile inflows are picked from their registers by uniform
+        // selects and dinv is formed again.
         Real xc = Real(0), yq0 = Real(0), yq1 = Real(0), yq2 = Real(0), yq3 = Real(0);
 #pragma unroll 1
         for (int n = 0; n < kNP; n++) {
@@ -511,29 +529,26 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
             d = dinv_of(bcast(tv_c, n));
           else
             d = dg[(size_t)c * nang + a];
-          Real pi_ = xc, pj_ = yq0;
-          if (i == 0) {
-            const Real v = in_x_ld[(size_t)(k * kSY + j) * nang + a];
-            pi_ = has_x ? v : Real(0);
-          }
-          if (j == 0) {
-            const Real v = in_y_ld[(size_t)(k * kSX + i) * nang + a];
-            pj_ = has_y ? v : Real(0);
-          }
+          const Real xj = j == 0 ? xin[0] : j == 1 ? xin[1] : j == 2 ? xin[2] : xin[3];
+          const Real yi = i == 0 ? yin[0] : i == 1 ? yin[1] : i == 2 ? yin[2] : yin[3];
+          const Real pi_ = i == 0 ? xj : xc;
+          const Real pj_ = j == 0 ? yi : yq0;
           Real ox, oy, oz;
-          const Real pc = cell_step(psi, d, pi_, pj_, out_z[(size_t)n * nang + a], c, ox, oy, oz);
+          bool fixed = false;
+          const Real pc = cell_step(psi, d, pi_, pj_, Zb[zslot(n)], c, ox, oy, oz, fixed);
+          predict_fixup = predict_fixup || fixed;
           xc = ox;
           yq0 = yq1;
           yq1 = yq2;
           yq2 = yq3;
           yq3 = oy;
-          out_z[(size_t)n * nang + a] = oz;
+          Zb[zslot(n)] = oz;
           if (i == kSX - 1) out_x[(size_t)(k * kSY + j) * nang + a] = ox;
           if (j == kSY - 1) out_y[(size_t)(k * kSX + i) * nang + a] = oy;
           W[wslot(n)] = aok ? w_d * (double)pc : 0.0;
         }
 #pragma unroll
-        for (int n = 0; n < kNP; n++) zf[n] = out_z[(size_t)n * nang + a];
+        for (int n = 0; n < kNP; n++) zf[n] = Zb[zslot(n)];
       }
       wave_sync();
       // (4) flux moments
@@ -602,7 +617,8 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
         const Real pi_ = (i == 0) ? xin[j] : xf[j];
         const Real pj_ = (j == 0) ? yin[i] : yf[i];
         Real ox, oy, oz;
-        const Real pc = cell_step(psi, d, pi_, pj_, zf[n], c, ox, oy, oz);
+        bool fixed = false;
+        const Real pc = cell_step(psi, d, pi_, pj_, zf[n], c, ox, oy, oz, fixed);
         xf[j] = ox;
         yf[i] = oy;
         zf[n] = oz;
@@ -641,7 +657,7 @@ int seq_row_stride(int nang) { return ((nang + 3) & ~3) + 1; }
 size_t seq_lds_bytes(int nang, int cmom) {
   const int rs = seq_row_stride(nang);
   const size_t tables = (size_t)((nang + 15) / 16) * ((cmom + 3) / 4) * 64 + (size_t)((nang + 3) / 4) * 64;
-  const size_t per_wave = (size_t)kNP * rs + 64 + std::max(kNP * rs, kNP * 16);
+  const size_t per_wave = 2 * ((size_t)kNP * rs + 64) + std::max(kNP * rs, kNP * 16);
   return sizeof(double) * (tables + kWGW * per_wave);
 }
 }  // namespace

@@ -44,11 +44,17 @@ def problem_struct(p: Problem) -> "_lib.Problem":
     return s
 
 
-# The mixed-precision default: with the flux change contracting by about 0.2
-# per inner iteration, the last one or two inners of a converging outer run in
-# FP64 (5 of 23 sweeps on the 64^3 P3 workload), which brings the FP32
-# sweeps' error from about 4e-6 to below 1e-7 against the FP64 solve.
+# FP64 polish thresholds (in units of epsi).  FP32 sweeps throughout: with
+# the flux change contracting by about 0.2 per inner iteration, the last one
+# or two inners of a converging outer run in FP64 (5 of 23 sweeps on the
+# 64^3 P3 workload), which brings the FP32 sweeps' error from about 4e-6 to
+# below 2e-7 against the FP64 solve.  With defect correction (an option) the
+# FP32 sweeps carry only the source defect, so a lower threshold suffices on
+# that workload: 2 polish sweeps in all, 1.7e-7, 1.54x the FP64 grind; on
+# small problems where a fifth of the cell sweeps run the fixup its
+# correction sweeps (which skip the fixup) leave up to 1e-5.
 MIXED_FP64_POLISH = 25.0
+MIXED_REFINE_FP64_POLISH = 10.0
 
 
 class Solver:
@@ -59,7 +65,9 @@ class Solver:
     arithmetic) or "mixed" (FP32 sweeps, FP64 moments, sources and tests).
     refine (mixed only): defect correction per outer iteration -- the first
     inner sweep runs in FP64 and is kept as the base; later inners sweep the
-    FP64 source defect in FP32 without the fixup and add it to the base.
+    FP64 source defect in FP32 without the fixup and add it to the base (an
+    FP64 polish sweep becomes the new base).  Off by default (None); steady
+    problems only.
     wide_sweep: one workgroup per cell and sweep, with angles across up to
     1024 lanes -- the form every solve past 128 angles per octant takes.
     sweep_kernel: "auto", "tiled" (multi-wave workgroups per tile) or
@@ -68,13 +76,15 @@ class Solver:
     previous inner of its outer changed the scalar flux by at most
     fp64_polish * epsi (or by at most 1e-5, the FP32 sweeps' own rounding
     level, when epsi asks for less), so the outers end on FP64 sweeps; the
-    default for mixed precision is MIXED_FP64_POLISH, 0 sweeps everything in
-    FP32.
+    default for mixed precision is MIXED_REFINE_FP64_POLISH with defect
+    correction and MIXED_FP64_POLISH without; 0 sweeps every inner after the
+    first in FP32.
     """
 
     def __init__(self, problem: Problem, precision: str = FP64, tile=(0, 0, 0),
                  waves_per_tile: int = 0, first_sweep_fp64: bool = False,
-                 dinv: str = "default", angles_per_lane: int = 0, refine: bool = False,
+                 dinv: str = "default", angles_per_lane: int = 0,
+                 refine: Optional[bool] = None,
                  outer_ordered: bool = False, wide_sweep: bool = False,
                  sweep_kernel: str = "auto", fp64_polish: Optional[float] = None):
         problem.validate()
@@ -89,12 +99,15 @@ class Solver:
         o.dinv_mode = {"default": _lib.DINV_DEFAULT, "stream": _lib.DINV_STREAM,
                        "recompute": _lib.DINV_RECOMPUTE}[dinv]
         o.angles_per_lane = angles_per_lane
+        if refine is None:
+            refine = False
         o.refine = int(bool(refine))
         o.outer_ordered = int(bool(outer_ordered))
         o.wide_sweep = int(bool(wide_sweep))
         o.sweep_kernel = {"auto": 0, "tiled": 1, "wave_tile": 2}[sweep_kernel]
         if fp64_polish is None:
-            fp64_polish = MIXED_FP64_POLISH if precision == MIXED else 0.0
+            fp64_polish = (0.0 if precision != MIXED else
+                           MIXED_REFINE_FP64_POLISH if refine else MIXED_FP64_POLISH)
         o.fp64_polish = float(fp64_polish)
         self.precision = precision
         h = ctypes.c_void_p()

@@ -424,17 +424,29 @@ static int sweep_impl(int nx, int ny, int nz, int nang, int cmom, int corner, do
         }
         /* moment reductions (sweep.cc:568-591; AVX lanes, sweep.cc:1465-1493) */
         if (avx) {
-          double lane[4] = {0.0, 0.0, 0.0, 0.0};
-          for (int a = 0; a < nang; a++) {
+          /* lane k of vector j holds angle 4j + k */
+          double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+          for (int a = 0; a < nang; a += 4) {
             psi[a] = pc[a] * w[a];
-            lane[a & 3] = lane[a & 3] + psi[a];
+            psi[a + 1] = pc[a + 1] * w[a + 1];
+            psi[a + 2] = pc[a + 2] * w[a + 2];
+            psi[a + 3] = pc[a + 3] * w[a + 3];
+            l0 = l0 + psi[a];
+            l1 = l1 + psi[a + 1];
+            l2 = l2 + psi[a + 2];
+            l3 = l3 + psi[a + 3];
           }
-          part_flux[c] = (lane[0] + lane[1]) + (lane[2] + lane[3]);
+          part_flux[c] = (l0 + l1) + (l2 + l3);
           for (int l = 1; l < cmom; l++) {
             const double *e = ecb + (size_t)l * nang;
-            lane[0] = lane[1] = lane[2] = lane[3] = 0.0;
-            for (int a = 0; a < nang; a++) lane[a & 3] = lane[a & 3] + psi[a] * e[a];
-            part_fluxm[c * (cmom - 1) + (l - 1)] = (lane[0] + lane[1]) + (lane[2] + lane[3]);
+            l0 = l1 = l2 = l3 = 0.0;
+            for (int a = 0; a < nang; a += 4) {
+              l0 = l0 + psi[a] * e[a];
+              l1 = l1 + psi[a + 1] * e[a + 1];
+              l2 = l2 + psi[a + 2] * e[a + 2];
+              l3 = l3 + psi[a + 3] * e[a + 3];
+            }
+            part_fluxm[c * (cmom - 1) + (l - 1)] = (l0 + l1) + (l2 + l3);
           }
           continue;
         }

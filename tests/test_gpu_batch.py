@@ -26,8 +26,9 @@ def test_batch_equals_single_solves(gpu, precision, polish, use_graph):
     print(st, [(x["sweeps_fp64"], x["inner_total"], round(x["time_ms"], 3)) for x in per])
     assert st["converged"] == len(probs)
     for i, p in enumerate(probs):
-        # the same polish rule (FP32 throughout with 0) picks the same sweeps
-        with Solver(p, precision=precision, fp64_polish=polish) as s:
+        # the same polish rule (FP32 throughout with 0) picks the same sweeps;
+        # batches sweep the whole source (no defect correction)
+        with Solver(p, precision=precision, fp64_polish=polish, refine=False) as s:
             ss = s.run()
             f, fm = s.flux()
         assert (per[i]["inner_total"], per[i]["outer_total"]) == (ss["inner_total"], ss["outer_total"])

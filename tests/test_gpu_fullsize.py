@@ -90,11 +90,15 @@ def test_config1_fp64_fused_against_avx_order(gpu, oracle_mod):
     assert np.max(np.abs(fm - fmo)) <= 1e-12 * np.max(np.abs(fmo))
 
 
-def test_config2_mixed_within_1e6(gpu, oracle_mod):
+@pytest.mark.parametrize("form", ["fp32_sweeps", "refine"])
+def test_config2_mixed_within_1e6(gpu, oracle_mod, form):
+    # the default (FP32 sweeps + FP64 polish) and the faster defect
+    # correction + FP64 polish
     fo, _, sto = _oracle(oracle_mod, "c1", _c1)
-    f, _, st = _solve(problems.config2(), "mixed")
+    kw = {"refine": True} if form == "refine" else {}
+    f, _, st = _solve(problems.config2(), "mixed", **kw)
     mx, l2 = _rel(f, fo)
-    print("config2 mixed", f"maxrel={mx:.3e} l2rel={l2:.3e}", st)
+    print("config2 mixed", form, f"maxrel={mx:.3e} l2rel={l2:.3e}", st)
     assert (st["inner_total"], st["outer_total"]) == (sto["inner_total"], sto["outer_total"])
     assert st["sweeps_fp32"] > st["sweeps_fp64"] > 0
     assert mx <= 1e-6 and l2 <= 1e-6
