@@ -228,6 +228,15 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
       return __builtin_fmaf(r, __builtin_fmaf(-den, r, 1.0f), r);
     }
   };
+  // 1/x in FP64 to within an ulp or two: the hardware reciprocal refined by
+  // two Newton steps
+  auto recip = [&](double x) -> double {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+  };
   const int r16 = lane & 15, q4 = lane >> 4;  // matrix-core operand coordinates
   // W slot of the lane in row n: its angle's column, or (lanes past the
   // row) its own slot of a junk row after the 16 rows, which nothing reads
@@ -240,8 +249,8 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   // ---- one cell of the general path: DD update, the reference's fixup loop
   // (sweep.cc:402-520) and the outflows; returns pc (fixed: some outflow of
   // the cell came out negative, so the fixup loop ran)
-  auto cell_step = [&](Real psi, Real d, Real pi_, Real pj_, Real pk_, int c, Real &ox, Real &oy,
-                       Real &oz, bool &fixed) -> Real {
+  auto cell_step = [&](Real psi, Real d, Real txs, Real pi_, Real pj_, Real pk_, Real &ox,
+                       Real &oy, Real &oz, bool &fixed) -> Real {
     Real pc;
     if constexpr (EXACT) {
       Real p = psi;
@@ -274,7 +283,6 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
       unsigned old_neg = count_negative();
       fixed = old_neg != 0;
       if (old_neg != 0) {
-        const Real txs = tg[c];
         // a pass that clears no new flag leaves pc unchanged, so the pass
         // after it repeats the count: 3*nang+2 passes always reach the break
         for (int it = 1; it < 3 * nang + 2; ++it) {
@@ -291,7 +299,15 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
             p = psi + Real(0.5) * t;
             den = (p <= Real(0)) ? Real(0) : (txs + cx * vx + cy * vy + cz * vz);
           }
-          pc = (den < tolr) ? Real(0) : p / den;
+          if constexpr (EXACT || !kFP64) {
+            pc = (den < tolr) ? Real(0) : p / den;
+          } else {
+            // fused FP64: a refined hardware reciprocal (as for dinv) instead
+            // of the division; a zero den is never used.  (FP32 keeps the
+            // division: one Newton step's last-bit errors cost the mixed
+            // solve measurable accuracy.)
+            pc = (den < tolr) ? Real(0) : p * recip(den);
+          }
           const unsigned neg = count_negative();
           if (neg == old_neg) break;
           old_neg = neg;
@@ -519,14 +535,21 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
         // slot; the tile inflows are picked from their registers by uniform
         // selects and dinv is formed again.
         Real xc = Real(0), yq0 = Real(0), yq1 = Real(0), yq2 = Real(0), yq3 = Real(0);
+        // each cell's LDS operands are read one cell ahead
+        Real psi_nx = (Real)P[a], pk_nx = Zb[zslot(0)];
 #pragma unroll 1
         for (int n = 0; n < kNP; n++) {
           const int i = n & 3, j = n >> 2;
           const int c = T.c0 + i * T.sx + j * T.sy + k * T.sz;
-          const Real psi = (Real)P[n * RS + a];
+          const Real psi = psi_nx, pk_ = pk_nx;
+          if (n + 1 < kNP) {
+            psi_nx = (Real)P[(n + 1) * RS + a];
+            pk_nx = Zb[zslot(n + 1)];
+          }
+          const Real txs = bcast(tv_c, n);
           Real d;
           if constexpr (REC)
-            d = dinv_of(bcast(tv_c, n));
+            d = dinv_of(txs);
           else
             d = dg[(size_t)c * nang + a];
           const Real xj = j == 0 ? xin[0] : j == 1 ? xin[1] : j == 2 ? xin[2] : xin[3];
@@ -535,7 +558,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
           const Real pj_ = j == 0 ? yi : yq0;
           Real ox, oy, oz;
           bool fixed = false;
-          const Real pc = cell_step(psi, d, pi_, pj_, Zb[zslot(n)], c, ox, oy, oz, fixed);
+          const Real pc = cell_step(psi, d, txs, pi_, pj_, pk_, ox, oy, oz, fixed);
           predict_fixup = predict_fixup || fixed;
           xc = ox;
           yq0 = yq1;
@@ -613,12 +636,13 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
           }
           psi = CMOM > 2 ? p0 + p1 : p0;
         }
-        const Real d = REC ? dinv_of(tg[c]) : dg[(size_t)c * nang + a];
+        const Real txs = tg[c];
+        const Real d = REC ? dinv_of(txs) : dg[(size_t)c * nang + a];
         const Real pi_ = (i == 0) ? xin[j] : xf[j];
         const Real pj_ = (j == 0) ? yin[i] : yf[i];
         Real ox, oy, oz;
         bool fixed = false;
-        const Real pc = cell_step(psi, d, pi_, pj_, zf[n], c, ox, oy, oz, fixed);
+        const Real pc = cell_step(psi, d, txs, pi_, pj_, zf[n], ox, oy, oz, fixed);
         xf[j] = ox;
         yf[i] = oy;
         zf[n] = oz;
