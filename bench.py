@@ -186,6 +186,49 @@ def rocprof_summary(dirs, kernel_filter: str = "", sweeps_of: str = "",
     return "\n".join(out)
 
 
+def dropin_timing(torch, n: int = 64, ng: int = 32, nang: int = 36, nmom: int = 4,
+                  streams: int = 4, reps: int = 3) -> dict:
+    """The drop-in sweep driven as Legion's task bodies drive it: one inner
+    iteration is 8 MiniKBA corner tasks over all groups, the tasks spread over
+    `streams` streams, every corner accumulating into the same flux fields.
+    Three forms: run_gpu_sweep once per group (the reference's task-body
+    loop, exact kernel), one group-range call per corner (exact), and one
+    group-range call per corner with the fused kernel.  Synthetic inputs on
+    the device; wall time per inner iteration (best of `reps`, host enqueue
+    included) and the grind time over cells x 8 nang x ng."""
+    from snap_amd import tasks
+    cmom = nmom * nmom
+    quad = tasks.quadrature(nang, nmom)
+    tasks.initialize_gpu_context(quad, nang, nmom, (n, n, n))
+    h = (2.0 * n, 2.0 * n, 2.0 * n)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    txs = 0.5 + torch.rand((ng, n, n, n), dtype=torch.float64, device="cuda", generator=gen)
+    dinv = torch.zeros((ng, n, n, n, nang), dtype=torch.float64, device="cuda")
+    tasks.calculate_geometry_param(txs, dinv, h)
+    qtot = torch.rand((ng, n, n, n, cmom), dtype=torch.float64, device="cuda", generator=gen)
+    qtot[..., 1:] *= 0.1
+    flux = torch.zeros((ng, n, n, n), dtype=torch.float64, device="cuda")
+    fluxm = torch.zeros((ng, n, n, n, cmom - 1), dtype=torch.float64, device="cuda")
+    ss = [torch.cuda.Stream() for _ in range(streams)]
+    units = n ** 3 * 8 * nang * ng
+    out = {"streams": streams, "tasks": 8, "groups_per_task": ng}
+    for key, form in (("per_group_calls_exact", None), ("group_range_exact", "exact"),
+                      ("group_range_fused", "fused")):
+        best = float("inf")
+        for r in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for corner in range(8):
+                with torch.cuda.stream(ss[corner % streams]):
+                    tasks.mini_kba(corner, range(ng), qtot, flux, fluxm, dinv, txs, None, h, nang,
+                                   nmom, group_range=form)
+            torch.cuda.synchronize()
+            if r:  # the first pass sizes the workspaces and loads the kernels
+                best = min(best, time.perf_counter() - t0)
+        out[key] = {"ms_per_inner": best * 1e3, "grind_ns": best * 1e9 / units}
+    return out
+
+
 def deck_suite(names, cpu: bool) -> dict:
     """Every reference input deck through the GPU solver (FP64, fused): the
     decks' own sizes, time steps and iteration control.  The CPU column times
@@ -347,6 +390,11 @@ def main() -> int:
             "grind_ns": g_t, "timed_sweep_ms": ts_t["ms_per_sweep"], "plan": plan_t,
             "inner": st_t[-1]["inner_total"],
             "maxrel_vs_default": float(np.max(np.abs(f_t - f64) / np.abs(f64)))}
+
+    # ---- the drop-in sweep as Legion's task bodies would drive it
+    if not args.no_extra:
+        details["dropin_config1"] = dropin_timing(torch)
+        details["dropin_config1"]["solver_grind_ns"] = grind
 
     # ---- FP64 in the reference's operation order (bit-equal to the oracle)
     with Solver(p, precision="fp64_exact") as s:

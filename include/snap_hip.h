@@ -118,6 +118,20 @@ typedef struct {
  * SIMULTANEOUS read-write, sweep.cc:54-55, gpu_sweep.cu:1705). */
 int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream_t stream);
 
+/* One MiniKBA task's group range in one launch series (MiniKBAArgs
+ * {corner, group_start, group_stop}, sweep.h:29-37; the task body's loop that
+ * calls run_gpu_sweep once per group, sweep.cc:1581-1617).  groups[i] holds
+ * run_gpu_sweep's arguments for group group_start + i; all share the chunk,
+ * corner and controls, and each field's accessors must have equal strides
+ * and equally spaced bases over the groups (the group fields of one Legion
+ * SOA instance are) -- else SNAP_ERR_UNSUPPORTED, and the caller sweeps group
+ * by group.  Steady sweeps only (vdelt = 0, no MMS source).  precision:
+ * SNAP_PRECISION_FP64_EXACT (results bit-equal to one run_gpu_sweep per
+ * group) or SNAP_PRECISION_FP64 (fused multiply-adds).  All groups' sweeps of
+ * the range share every launch, as the solver's do. */
+int snap_hip_run_gpu_sweep_groups(const snap_sweep_args_t *groups, int num_groups, int precision,
+                                  snap_stream_t stream);
+
 /* snap_hip_run_gpu_sweep without any device allocation: the same kernels and
  * results, with the plan tables and tile-face workspace in a fixed scratch
  * array of 64 MiB that each device holds until the process ends (chunks whose

@@ -608,7 +608,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
         Real v = Real(0);
         if (j < ey && k < ez) {
           const int py = syp ? y0 + j : ny - 1 - (y0 + j), pz = szp ? z0 + k : nz - 1 - (z0 + k);
-          v = (Real)((const double *)G.gx.at(py, pz, 0))[a2];
+          v = (Real)((const double *)G.gx.at(py, pz, g))[a2];
         }
         fx[idx] = v;
       }
@@ -626,7 +626,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
         Real v = Real(0);
         if (i < ex && k < ez) {
           const int px = sxp ? x0 + i : nx - 1 - (x0 + i), pz = szp ? z0 + k : nz - 1 - (z0 + k);
-          v = (Real)((const double *)G.gy.at(px, pz, 0))[a2];
+          v = (Real)((const double *)G.gy.at(px, pz, g))[a2];
         }
         fy[idx] = v;
       }
@@ -644,7 +644,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
         Real v = Real(0);
         if (i < ex && j < ey) {
           const int px = sxp ? x0 + i : nx - 1 - (x0 + i), py = syp ? y0 + j : ny - 1 - (y0 + j);
-          v = (Real)((const double *)G.gz.at(px, py, 0))[a2];
+          v = (Real)((const double *)G.gz.at(px, py, g))[a2];
         }
         fz[idx] = v;
       }
@@ -697,8 +697,8 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
       const double *tp = nullptr, *qp = nullptr;
       if constexpr (GEN) {
         const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
-        if (has_time) tp = (const double *)G.tin.at(px, py, pz, 0);
-        if (has_qim) qp = (const double *)G.qim.at(px, py, pz, 0);
+        if (has_time) tp = (const double *)G.tin.at(px, py, pz, g);
+        if (has_qim) qp = (const double *)G.qim.at(px, py, pz, g);
       } else {
         if (has_time) tp = tig + (size_t)cell * nang;
         if (has_qim) qp = qmg + (size_t)cell * nang;
@@ -711,14 +711,14 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
     }
     if constexpr (GEN) {
       const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
-      const Real *dv = (const Real *)G.dinv.at(px, py, pz, 0);
-      const double *qc = (const double *)G.qtot.at(px, py, pz, 0);
+      const Real *dv = (const Real *)G.dinv.at(px, py, pz, g);
+      const double *qc = (const double *)G.qtot.at(px, py, pz, g);
 #pragma unroll
       for (int r = 0; r < APL; r++) n_dv[st][r] = dv[an[r]];
 #pragma unroll
       for (int m = 0; m < QS; m++)
         if (q_in_lanes && la + m * LPC < cmom) n_q[st][m] = qc[la + m * LPC];
-      n_tx[st] = *(const Real *)G.txs.at(px, py, pz, 0);
+      n_tx[st] = *(const Real *)G.txs.at(px, py, pz, g);
     } else {
       if constexpr (!REC) {
         const Real *dv = dg + (size_t)cell * nang;
@@ -779,8 +779,8 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
       auto store = [&](int l, double sum) {
         if constexpr (GEN) {
           const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
-          double *dst = (l == 0) ? (double *)G.flux.at(px, py, pz, 0)
-                                 : (double *)G.fluxm.at(px, py, pz, 0) + (l - 1);
+          double *dst = (l == 0) ? (double *)G.flux.at(px, py, pz, g)
+                                 : (double *)G.fluxm.at(px, py, pz, g) + (l - 1);
           atomicAdd(dst, sum);
         } else {
           if (l == 0)
@@ -836,7 +836,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
           if (cell >= 0) {
             if constexpr (GEN) {
               const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
-              v = ((const double *)G.qtot.at(px, py, pz, 0))[l];
+              v = ((const double *)G.qtot.at(px, py, pz, g))[l];
             } else {
               v = qg[(size_t)cell * cmom + l];
             }
@@ -1088,7 +1088,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
             double *to;
             if constexpr (GEN) {
               const int px = cell % nx, py = (cell / nx) % ny, pz = cell / (nx * ny);
-              to = (double *)G.tout.at(px, py, pz, 0);
+              to = (double *)G.tout.at(px, py, pz, g);
             } else {
               to = tog + (size_t)cell * nang;
             }
@@ -1122,7 +1122,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
         const int a2 = idx % nang, slot = idx / nang, j = slot % TY, k = slot / TY;
         if (j < ey && k < ez) {
           const int py = syp ? y0 + j : ny - 1 - (y0 + j), pz = szp ? z0 + k : nz - 1 - (z0 + k);
-          ((double *)G.gx.at(py, pz, 0))[a2] = (double)fx[idx];
+          ((double *)G.gx.at(py, pz, g))[a2] = (double)fx[idx];
         }
       }
     if (ty == nTy - 1 && G.gy.base)
@@ -1130,7 +1130,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
         const int a2 = idx % nang, slot = idx / nang, i = slot % TX, k = slot / TX;
         if (i < ex && k < ez) {
           const int px = sxp ? x0 + i : nx - 1 - (x0 + i), pz = szp ? z0 + k : nz - 1 - (z0 + k);
-          ((double *)G.gy.at(px, pz, 0))[a2] = (double)fy[idx];
+          ((double *)G.gy.at(px, pz, g))[a2] = (double)fy[idx];
         }
       }
     if (tz == nTz - 1 && G.gz.base)
@@ -1138,7 +1138,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
         const int a2 = idx % nang, slot = idx / nang, i = slot % TX, j = slot / TX;
         if (i < ex && j < ey) {
           const int px = sxp ? x0 + i : nx - 1 - (x0 + i), py = syp ? y0 + j : ny - 1 - (y0 + j);
-          ((double *)G.gz.at(px, py, 0))[a2] = (double)fz[idx];
+          ((double *)G.gz.at(px, py, g))[a2] = (double)fz[idx];
         }
       }
   }
@@ -1202,8 +1202,8 @@ __global__ void __launch_bounds__(1024) k_sweep_wide(SweepArgs A, SweepGeneral G
   const double *qc;
   Real txs;
   if constexpr (GEN) {
-    qc = (const double *)G.qtot.at(px, py, pz, 0);
-    txs = *(const Real *)G.txs.at(px, py, pz, 0);
+    qc = (const double *)G.qtot.at(px, py, pz, g);
+    txs = *(const Real *)G.txs.at(px, py, pz, g);
   } else {
     qc = A.q + ((size_t)g * cells + cell) * cmom;
     txs = ((const Real *)A.txs)[(size_t)g * cells + cell];
@@ -1240,7 +1240,7 @@ __global__ void __launch_bounds__(1024) k_sweep_wide(SweepArgs A, SweepGeneral G
       for (int l = 1; l < cmom; l++) ps = madd<EXACT>(ecc[(size_t)l * nang + a], qc[l], ps);
       if constexpr (TD) {
         if (has_qim)
-          ps += GEN ? ((const double *)G.qim.at(px, py, pz, 0))[a] : A.qim[ab + a];
+          ps += GEN ? ((const double *)G.qim.at(px, py, pz, g))[a] : A.qim[ab + a];
       }
       psi[r] = ps;
     } else {
@@ -1248,14 +1248,14 @@ __global__ void __launch_bounds__(1024) k_sweep_wide(SweepArgs A, SweepGeneral G
       for (int l = 1; l < cmom; l++) pf = maddf((float)ecc[(size_t)l * nang + a], (float)qc[l], pf);
       if constexpr (TD) {
         if (has_qim)
-          pf += (float)(GEN ? ((const double *)G.qim.at(px, py, pz, 0))[a] : A.qim[ab + a]);
+          pf += (float)(GEN ? ((const double *)G.qim.at(px, py, pz, g))[a] : A.qim[ab + a]);
       }
       psi[r] = pf;
     }
     ti[r] = Real(0);
     if constexpr (TD) {
       if (has_time)
-        ti[r] = (Real)(GEN ? ((const double *)G.tin.at(px, py, pz, 0))[a] : A.tin[ab + a]);
+        ti[r] = (Real)(GEN ? ((const double *)G.tin.at(px, py, pz, g))[a] : A.tin[ab + a]);
     }
     // inflow faces: the upstream cell's outflow, the ghost array at the
     // chunk edge, or vacuum
@@ -1263,19 +1263,19 @@ __global__ void __launch_bounds__(1024) k_sweep_wide(SweepArgs A, SweepGeneral G
     if (i > 0)
       vx = sxf[a];
     else if (GEN && G.gx.base)
-      vx = (Real)((const double *)G.gx.at(py, pz, 0))[a];
+      vx = (Real)((const double *)G.gx.at(py, pz, g))[a];
     if (j > 0)
       vy = syf[a];
     else if (GEN && G.gy.base)
-      vy = (Real)((const double *)G.gy.at(px, pz, 0))[a];
+      vy = (Real)((const double *)G.gy.at(px, pz, g))[a];
     if (k > 0)
       vz = szf[a];
     else if (GEN && G.gz.base)
-      vz = (Real)((const double *)G.gz.at(px, py, 0))[a];
+      vz = (Real)((const double *)G.gz.at(px, py, g))[a];
     psii[r] = vx;
     psij[r] = vy;
     psik[r] = vz;
-    const Real dv = GEN ? ((const Real *)G.dinv.at(px, py, pz, 0))[a]
+    const Real dv = GEN ? ((const Real *)G.dinv.at(px, py, pz, g))[a]
                         : ((const Real *)A.dinv)[((size_t)g * cells + cell) * nang + a];
     // initial diamond-difference solution (sweep.cc:352-423)
     Real p = psi[r];
@@ -1387,22 +1387,22 @@ __global__ void __launch_bounds__(1024) k_sweep_wide(SweepArgs A, SweepGeneral G
     pcw[r] = w_d[r] * (double)pc[r];
     if (!aok[r]) continue;
     if (GEN && i == nx - 1 && G.gx.base)
-      ((double *)G.gx.at(py, pz, 0))[a] = (double)ox;
+      ((double *)G.gx.at(py, pz, g))[a] = (double)ox;
     else
       sxf[a] = ox;
     if (GEN && j == ny - 1 && G.gy.base)
-      ((double *)G.gy.at(px, pz, 0))[a] = (double)oy;
+      ((double *)G.gy.at(px, pz, g))[a] = (double)oy;
     else
       syf[a] = oy;
     if (GEN && k == nz - 1 && G.gz.base)
-      ((double *)G.gz.at(px, py, 0))[a] = (double)oz;
+      ((double *)G.gz.at(px, py, g))[a] = (double)oz;
     else
       szf[a] = oz;
     if constexpr (TD) {
       if (has_time) {
         const double ot = (double)((p2 - ti[r]) * (ht[r] ? Real(1) : Real(0)));
         if constexpr (GEN)
-          ((double *)G.tout.at(px, py, pz, 0))[a] = ot;
+          ((double *)G.tout.at(px, py, pz, g))[a] = ot;
         else
           A.tout[ab + a] = ot;
       }
@@ -1412,8 +1412,8 @@ __global__ void __launch_bounds__(1024) k_sweep_wide(SweepArgs A, SweepGeneral G
   // flux moments phi_l = sum_a ec[l][a] * w[a] pc[a] (sweep.cc:568-591)
   auto store = [&](int l, double sum) {
     if constexpr (GEN) {
-      double *dst = (l == 0) ? (double *)G.flux.at(px, py, pz, 0)
-                             : (double *)G.fluxm.at(px, py, pz, 0) + (l - 1);
+      double *dst = (l == 0) ? (double *)G.flux.at(px, py, pz, g)
+                             : (double *)G.fluxm.at(px, py, pz, g) + (l - 1);
       atomicAdd(dst, sum);
     } else {
       const size_t sc = ((size_t)corner * A.ng_job + g) * cells + cell;
@@ -1556,8 +1556,8 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
   if (L.nsweeps < 1 || L.nsweeps > 65535) return fail(SNAP_ERR_INVALID, "bad sweep count");
   if (L.quad->nang != p.nang || L.quad->cmom != p.cmom)
     return fail(SNAP_ERR_INVALID, "quadrature does not match the sweep plan");
-  if (L.general && (L.nsweeps != 1 || L.ng_job != 1))
-    return fail(SNAP_ERR_INVALID, "the general sweep handles one (corner, group) per call");
+  if (L.general && L.nsweeps != L.ng_job)
+    return fail(SNAP_ERR_INVALID, "the general sweep handles one corner per call");
   if ((int64_t)p.ext.nx * p.ext.ny * p.ext.nz >= (int64_t)1 << 31)
     return fail(SNAP_ERR_UNSUPPORTED, "chunks of 2^31 cells or more");
   if (L.batch && (L.general || L.recompute_dinv))
@@ -1788,11 +1788,21 @@ int check_sweep_args(const snap_sweep_args_t &a) {
   return SNAP_OK;
 }
 
-// The launch of one drop-in sweep on `plan` with the given face workspace,
+// Byte distance between consecutive groups' fields of a group-range sweep
+// (0 for a single group).
+struct DropinGroupStrides {
+  int64_t qtot = 0, flux = 0, fluxm = 0, dinv = 0, txs = 0, gx = 0, gy = 0, gz = 0;
+};
+
+// The launch of drop-in sweeps on `plan` with the given face workspace,
 // from the reference's run_gpu_sweep arguments (accessors relative to the
-// chunk origin, ghost points absolute).
+// chunk origin, ghost points absolute): one (corner, group), or with
+// ngroups > 1 the groups ngroups of one corner, group g's fields at g times
+// the strides in gs from the first group's.  exact: the reference's CPU
+// operation order (bit-equal); else fused multiply-adds.
 int dropin_launch(const snap_sweep_args_t &a, const SweepPlan *plan, const QuadDev *quad,
-                  void *faces, hipStream_t st) {
+                  void *faces, hipStream_t st, int ngroups = 1,
+                  const DropinGroupStrides *gs = nullptr, bool exact = true) {
   const Extent3 e{a.x_range, a.y_range, a.z_range};
   const bool td = a.vdelt != 0.0 || a.mms_source;
   snap_rect3_t b;
@@ -1804,8 +1814,8 @@ int dropin_launch(const snap_sweep_args_t &a, const SweepPlan *plan, const QuadD
   memset(&L, 0, sizeof(L));
   L.plan = plan;
   L.quad = quad;
-  L.nsweeps = 1;
-  L.ng_job = 1;
+  L.nsweeps = ngroups;
+  L.ng_job = ngroups;
   L.corner0 = a.corner;
   L.fixup = a.fixup ? 1 : 0;
   L.hi = a.hi;
@@ -1830,9 +1840,19 @@ int dropin_launch(const snap_sweep_args_t &a, const SweepPlan *plan, const QuadD
   L.gx = ghost(a.ghostx, a.origin[1], a.origin[2]);
   L.gy = ghost(a.ghosty, a.origin[0], a.origin[2]);
   L.gz = ghost(a.ghostz, a.origin[0], a.origin[1]);
+  if (gs) {
+    L.qtot.sg = gs->qtot;
+    L.flux.sg = gs->flux;
+    L.fluxm.sg = gs->fluxm;
+    L.dinv.sg = gs->dinv;
+    L.txs.sg = gs->txs;
+    L.gx.sg = gs->gx;
+    L.gy.sg = gs->gy;
+    L.gz.sg = gs->gz;
+  }
   L.faces = faces;
   L.fp32 = false;
-  L.exact = true;
+  L.exact = exact;
   // time-dependent terms and the MMS source (gpu_sweep.cu:2102-2140 picks the
   // time-dependent kernel when vdelt != 0)
   if (td) {
@@ -1880,10 +1900,12 @@ __global__ void k_plan_tables(int nTx, int nTy, int nTz, int TX, int TY, int TZ,
 }  // namespace
 }  // namespace snap
 
-extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream_t stream) {
-  if (!args) return fail(SNAP_ERR_INVALID, "NULL sweep arguments");
-  const snap_sweep_args_t &a = *args;
-  SNAP_TRY(check_sweep_args(a));
+namespace snap {
+namespace {
+// run_gpu_sweep's body: the plan of the chunk shape and the stream's face
+// workspace (sized for ngroups sweeps), then the launches
+int dropin_run(const snap_sweep_args_t &a, snap_stream_t stream, int ngroups,
+               const DropinGroupStrides *gs, bool exact) {
   int dev;
   SNAP_TRY(current_device(&dev));
   const Extent3 e{a.x_range, a.y_range, a.z_range};
@@ -1932,7 +1954,7 @@ extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream
       wk->stream = st;
     }
   }
-  const size_t need = plan->face_doubles_per_sweep() * sizeof(double);
+  const size_t need = plan->face_doubles_per_sweep() * sizeof(double) * (size_t)ngroups;
   std::lock_guard<std::mutex> wlock(wk->lock);
   if (wk->bytes < need) {
     if (wk->faces) {
@@ -1945,7 +1967,90 @@ extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream
       return fail(SNAP_ERR_NO_MEMORY, "face workspace of %zu bytes", need);
     wk->bytes = need;
   }
-  return dropin_launch(a, plan, quad, wk->faces, st);
+  return dropin_launch(a, plan, quad, wk->faces, st, ngroups, gs, exact);
+}
+
+// the byte distance between consecutive groups' instances of one accessor
+// field: equal strides, equally spaced bases (or NULL for every group where
+// the field is optional)
+int group_step(const void *const *bases, const int64_t *const *strides, int nstr, int n,
+               bool allow_null, const char *what, int64_t *sg) {
+  *sg = 0;
+  const bool null0 = bases[0] == nullptr;
+  if (null0 && !allow_null) return fail(SNAP_ERR_INVALID, "%s accessor has a NULL base", what);
+  if (n > 1 && !null0) *sg = (const char *)bases[1] - (const char *)bases[0];
+  for (int g = 1; g < n; g++) {
+    if ((bases[g] == nullptr) != null0)
+      return fail(SNAP_ERR_INVALID, "%s: group %d is NULL where group 0 is not (or the reverse)",
+                  what, g);
+    if (null0) continue;
+    for (int i = 0; i < nstr; i++)
+      if (strides[g][i] != strides[0][i])
+        return fail(SNAP_ERR_UNSUPPORTED, "%s: group %d strides differ from group 0", what, g);
+    if ((const char *)bases[g] - (const char *)bases[0] != *sg * g)
+      return fail(SNAP_ERR_UNSUPPORTED,
+                  "%s: the groups' fields are not equally spaced in one instance (sweep them "
+                  "one group at a time)", what);
+  }
+  return SNAP_OK;
+}
+}  // namespace
+}  // namespace snap
+
+extern "C" int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream_t stream) {
+  if (!args) return fail(SNAP_ERR_INVALID, "NULL sweep arguments");
+  SNAP_TRY(check_sweep_args(*args));
+  return dropin_run(*args, stream, 1, nullptr, true);
+}
+
+extern "C" int snap_hip_run_gpu_sweep_groups(const snap_sweep_args_t *groups, int num_groups,
+                                             int precision, snap_stream_t stream) {
+  if (!groups || num_groups < 1) return fail(SNAP_ERR_INVALID, "no groups to sweep");
+  if (num_groups > 65535) return fail(SNAP_ERR_INVALID, "at most 65535 groups per call");
+  if (precision != SNAP_PRECISION_FP64 && precision != SNAP_PRECISION_FP64_EXACT)
+    return fail(SNAP_ERR_INVALID, "group-range sweeps are FP64 (fused or exact)");
+  const snap_sweep_args_t &a = groups[0];
+  for (int g = 0; g < num_groups; g++) {
+    const snap_sweep_args_t &b = groups[g];
+    SNAP_TRY(check_sweep_args(b));
+    if (b.vdelt != 0.0 || b.mms_source)
+      return fail(SNAP_ERR_UNSUPPORTED,
+                  "group-range sweeps are steady (call run_gpu_sweep per group for vdelt != 0 "
+                  "or an MMS source)");
+    if (b.origin[0] != a.origin[0] || b.origin[1] != a.origin[1] || b.origin[2] != a.origin[2] ||
+        b.x_range != a.x_range || b.y_range != a.y_range || b.z_range != a.z_range ||
+        b.corner != a.corner || b.num_moments != a.num_moments || b.num_angles != a.num_angles ||
+        b.fixup != a.fixup || b.hi != a.hi || b.hj != a.hj || b.hk != a.hk)
+      return fail(SNAP_ERR_INVALID, "group %d differs from group 0 in its chunk, corner or controls",
+                  g);
+  }
+  DropinGroupStrides gs;
+  std::vector<const void *> bases(num_groups);
+  std::vector<const int64_t *> strides(num_groups);
+  auto step3 = [&](snap_accessor3_t snap_sweep_args_t::*f, bool allow_null, const char *what,
+                   int64_t *out) -> int {
+    for (int g = 0; g < num_groups; g++) {
+      bases[g] = (groups[g].*f).base;
+      strides[g] = (groups[g].*f).stride;
+    }
+    return group_step(bases.data(), strides.data(), 3, num_groups, allow_null, what, out);
+  };
+  auto step2 = [&](snap_accessor2_t snap_sweep_args_t::*f, const char *what, int64_t *out) -> int {
+    for (int g = 0; g < num_groups; g++) {
+      bases[g] = (groups[g].*f).base;
+      strides[g] = (groups[g].*f).stride;
+    }
+    return group_step(bases.data(), strides.data(), 2, num_groups, true, what, out);
+  };
+  SNAP_TRY(step3(&snap_sweep_args_t::qtot, false, "qtot", &gs.qtot));
+  SNAP_TRY(step3(&snap_sweep_args_t::flux, false, "flux", &gs.flux));
+  SNAP_TRY(step3(&snap_sweep_args_t::fluxm, true, "fluxm", &gs.fluxm));
+  SNAP_TRY(step3(&snap_sweep_args_t::dinv, false, "dinv", &gs.dinv));
+  SNAP_TRY(step3(&snap_sweep_args_t::t_xs, false, "t_xs", &gs.txs));
+  SNAP_TRY(step2(&snap_sweep_args_t::ghostx, "ghostx", &gs.gx));
+  SNAP_TRY(step2(&snap_sweep_args_t::ghosty, "ghosty", &gs.gy));
+  SNAP_TRY(step2(&snap_sweep_args_t::ghostz, "ghostz", &gs.gz));
+  return dropin_run(a, stream, num_groups, &gs, precision == SNAP_PRECISION_FP64_EXACT);
 }
 
 extern "C" int snap_hip_run_gpu_sweep_noalloc(const snap_sweep_args_t *args,

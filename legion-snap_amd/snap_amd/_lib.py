@@ -134,6 +134,8 @@ SIGNATURES = {
         [_dp, _dp, _dp, _dp, _dp] + [ctypes.c_int] * 6, ctypes.c_int),
     "snap_hip_run_gpu_sweep": ([ctypes.POINTER(SweepArgs), _vp], ctypes.c_int),
     "snap_hip_run_gpu_sweep_noalloc": ([ctypes.POINTER(SweepArgs), _vp], ctypes.c_int),
+    "snap_hip_run_gpu_sweep_groups": ([ctypes.POINTER(SweepArgs), ctypes.c_int, ctypes.c_int, _vp],
+                                      ctypes.c_int),
     "snap_hip_run_flux0_outer_source": (
         [Rect3, ctypes.POINTER(Accessor3), ctypes.POINTER(Accessor3), ctypes.POINTER(Accessor2),
          ctypes.POINTER(Accessor3), Accessor3, ctypes.c_int, _vp], ctypes.c_int),

@@ -99,7 +99,7 @@ def quadrature(nang: int, nmom: int):
 def mini_kba(corner: int, groups: Sequence[int], qtot, flux, fluxm, dinv, t_xs,
              ghosts, hijk, nang: int, nmom: int, fixup: bool = True, origin=(0, 0, 0),
              vdelt=None, time_flux_in=None, time_flux_out=None, qim=None, extent=None,
-             noalloc: bool = False) -> None:
+             noalloc: bool = False, group_range: Optional[str] = None) -> None:
     """One corner over a group range: run_gpu_sweep once per group.
     ghosts: (gx [ng, nz, ny, nang], gy [ng, nz, nx, nang], gz [ng, ny, nx, nang]) or None,
     indexed by absolute points.  extent: (nx, ny, nz) of the chunk at `origin`
@@ -108,10 +108,14 @@ def mini_kba(corner: int, groups: Sequence[int], qtot, flux, fluxm, dinv, t_xs,
     [ng] host values, time_flux_in / time_flux_out [ng, nz, ny, nx, nang] of
     this corner; qim [ng, nz, ny, nx, nang] is the corner's MMS source.
     noalloc: call the allocation-free entry point (scratch array, no device
-    allocation) instead of run_gpu_sweep."""
+    allocation) instead of run_gpu_sweep.  group_range: "exact" or "fused" --
+    one snap_hip_run_gpu_sweep_groups call for the whole (contiguous) group
+    range instead of one run_gpu_sweep per group."""
     lib = _lib.load()
     nx, ny, nz = extent if extent is not None else flux.shape[1:4][::-1]
-    for g in groups:
+    groups = list(groups)
+    batch = (_lib.SweepArgs * len(groups))() if group_range else None
+    for gi, g in enumerate(groups):
         a = _lib.SweepArgs()
         a.origin[0], a.origin[1], a.origin[2] = origin
         a.qtot = acc3(qtot, g)
@@ -145,12 +149,19 @@ def mini_kba(corner: int, groups: Sequence[int], qtot, flux, fluxm, dinv, t_xs,
             a.mms_source = 1
         a.num_angles = nang
         a.fixup = int(bool(fixup))
+        if batch is not None:
+            batch[gi] = a
+            continue
         if noalloc:
             _lib.check(lib.snap_hip_run_gpu_sweep_noalloc(ctypes.byref(a), _stream()),
                        "snap_hip_run_gpu_sweep_noalloc")
         else:
             _lib.check(lib.snap_hip_run_gpu_sweep(ctypes.byref(a), _stream()),
                        "snap_hip_run_gpu_sweep")
+    if batch is not None:
+        prec = {"exact": _lib.PRECISION_FP64_EXACT, "fused": _lib.PRECISION_FP64}[group_range]
+        _lib.check(lib.snap_hip_run_gpu_sweep_groups(batch, len(groups), prec, _stream()),
+                   "snap_hip_run_gpu_sweep_groups")
 
 
 def calc_outer_source(qi, flux0, slgg, qo0, mat, fluxm=None, qom=None, nmom: int = 1) -> None:

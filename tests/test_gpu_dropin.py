@@ -139,6 +139,72 @@ def test_mini_kba_all_corners(setup, oracle_mod, with_ghosts):
     assert np.allclose(fluxm.cpu().numpy(), refm, rtol=1e-13, atol=1e-16)
 
 
+@pytest.mark.parametrize("with_ghosts", [False, True])
+@pytest.mark.parametrize("form", ["exact", "fused"])
+def test_group_range_matches_per_group_calls(setup, with_ghosts, form):
+    # one snap_hip_run_gpu_sweep_groups call for a corner's whole group range
+    # against run_gpu_sweep once per group (the reference's task-body loop):
+    # bit-equal in the exact form (one corner per fresh flux buffer, so every
+    # cell receives one addition), within rounding in the fused form
+    S = setup
+    torch, T = S["torch"], S["T"]
+    nx, ny, nz, ng, nang, nmom, cmom = S["dims"]
+    rng = np.random.default_rng(77)
+    mu, eta, xi, w, ec = S["quad"]
+    txs = rng.uniform(0.5, 3.0, (ng, nz, ny, nx))
+    dinv = T(_dinv_of(S, txs))
+    qtot = rng.uniform(0, 1, (ng, nz, ny, nx, cmom))
+    qtot[..., 1:] *= 0.2
+    qtot_d, txs_d = T(qtot), T(txs)
+    z = lambda *shape: torch.zeros(shape, dtype=torch.float64, device="cuda")  # noqa: E731
+    for corner in (0, 3, 5, 7):
+        fa, fb = z(ng, nz, ny, nx), z(ng, nz, ny, nx)
+        ma, mb = z(ng, nz, ny, nx, cmom - 1), z(ng, nz, ny, nx, cmom - 1)
+        ga = gb = None
+        if with_ghosts:
+            gin = (rng.uniform(0, 1, (ng, nz, ny, nang)), rng.uniform(0, 1, (ng, nz, nx, nang)),
+                   rng.uniform(0, 1, (ng, ny, nx, nang)))
+            ga, gb = tuple(T(x) for x in gin), tuple(T(x) for x in gin)
+        tasks.mini_kba(corner, range(ng), qtot_d, fa, ma, dinv, txs_d, ga, S["h"], nang, nmom)
+        tasks.mini_kba(corner, range(ng), qtot_d, fb, mb, dinv, txs_d, gb, S["h"], nang, nmom,
+                       group_range=form)
+        torch.cuda.synchronize()
+        A, B = fa.cpu().numpy(), fb.cpu().numpy()
+        AM, BM = ma.cpu().numpy(), mb.cpu().numpy()
+        if form == "exact":
+            assert np.array_equal(A, B) and np.array_equal(AM, BM), corner
+            if with_ghosts:
+                for k in range(3):
+                    assert torch.equal(ga[k], gb[k]), (corner, k)
+        else:
+            assert np.max(np.abs(B - A) / np.abs(A)) <= 1e-13, corner
+            assert np.max(np.abs(BM - AM)) <= 1e-13 * np.max(np.abs(AM)), corner
+            if with_ghosts:
+                for k in range(3):
+                    assert torch.allclose(ga[k], gb[k], rtol=1e-12, atol=1e-15), (corner, k)
+
+
+def _dinv_of(S, txs):
+    from oracle import oracle
+    mu, eta, xi, w, ec = S["quad"]
+    return oracle.geometry_param(txs, S["dims"][4], S["h"], mu, eta, xi)
+
+
+def test_group_range_refuses_unevenly_spaced_groups(setup):
+    from snap_amd import _lib
+    S = setup
+    torch, T = S["torch"], S["T"]
+    nx, ny, nz, ng, nang, nmom, cmom = S["dims"]
+    z = lambda *shape: torch.zeros(shape, dtype=torch.float64, device="cuda")  # noqa: E731
+    txs = T(np.ones((ng, nz, ny, nx)))
+    dinv = T(_dinv_of(S, np.ones((ng, nz, ny, nx))))
+    with pytest.raises(_lib.SnapError) as e:
+        tasks.mini_kba(0, [0, 2, 1], z(ng, nz, ny, nx, cmom), z(ng, nz, ny, nx),
+                       z(ng, nz, ny, nx, cmom - 1), dinv, txs, None, S["h"], nang, nmom,
+                       group_range="exact")
+    assert e.value.code == _lib.SNAP_ERR_UNSUPPORTED
+
+
 def test_uninitialized_device_context_is_an_error(setup):
     from snap_amd import _lib
     import ctypes
