@@ -251,7 +251,9 @@ def deck_suite(names, cpu: bool) -> dict:
                "inner": st["inner_total"], "outer": st["outer_total"],
                "converged": st["converged"], "grind_ns": st["grind_ns"],
                "solve_s": st["solve_ms"] / 1e3, "sweep_s": st["sweep_ms"] / 1e3,
-               "sweep": "one cell per workgroup" if plan["cells_per_wave"] == 0 else "tiled"}
+               "sweep": ("one cell per workgroup" if plan["cells_per_wave"] == 0 else
+                         "one wavefront per tile" if plan["waves"] == 1 and
+                         plan["cells_per_wave"] == 16 else "multi-wave tiles")}
         if mms is not None:
             row["mms_max_diff"] = mms[0]
         if cpu:

@@ -283,9 +283,10 @@ typedef struct {
                                128 angles per octant; up to 2048); 0 = tiled when it fits */
   int sweep_kernel;         /* SNAP_SWEEP_AUTO, _TILED (multi-wave workgroups per tile)
                                or _WAVE_TILE (one wavefront per 4x4xTZ tile, steady
-                               problems of up to 64 angles; tile[2] sets TZ).  AUTO takes
-                               the wave tile for steady problems of 33..64 angles unless
-                               a multi-wave option (tile x/y, waves, angles per lane) is set */
+                               problems of up to 64 angles; tile[2] sets TZ, default 8).
+                               AUTO takes the wave tile for steady problems of 33..64
+                               angles unless a multi-wave option (tile x/y, waves, angles
+                               per lane) is set */
   double fp64_polish;       /* MIXED: run an inner's sweep in FP64 when the previous inner
                                of the same outer changed the scalar flux by at most
                                fp64_polish * epsi (max relative change), or by at most
@@ -388,7 +389,9 @@ int snap_hip_solver_gradient(snap_solver_t *solver, const double *resp, int prec
                              int *iterations);
 /* The sweep plan the solver chose: plan[0..2] tile edges (x,y,z), plan[3]
  * waves per tile workgroup, plan[4] angles per lane, plan[5] cells per
- * wavefront (0 for the one-cell-per-workgroup sweep, whose tiles are 1x1x1). */
+ * wavefront (0 for the one-cell-per-workgroup sweep, whose tiles are 1x1x1;
+ * 16 with plan[3] = 1 for the one-wavefront tile sweep, whose wavefront
+ * holds a 4 x 4 plane of its tile). */
 int snap_hip_solver_plan(snap_solver_t *solver, int32_t plan[6]);
 void snap_hip_solver_destroy(snap_solver_t *solver);
 
@@ -413,6 +416,9 @@ typedef struct {
                                changed its flux by at most fp64_polish * epsi or
                                1e-5, whichever is larger (max |flux0/flux0p - 1|);
                                0 = FP32 sweeps throughout */
+  int sweep_kernel;         /* SNAP_SWEEP_AUTO (the autotuner times one-wavefront 4x4xTZ
+                               tiles, TZ 4 or 8, and multi-wave tiles), _TILED or
+                               _WAVE_TILE (up to 64 angles; tile[2] sets TZ, else timed) */
 } snap_batch_options_t;
 
 typedef struct {

@@ -184,6 +184,7 @@ int64_t blocks_for(int64_t n) { return std::min<int64_t>((n + 255) / 256, 65536)
 // nang, nmom, every problem's extent), for the life of the process.
 struct TunedShape {
   int TX, TY, TZ, waves, apl;
+  bool seq;  // the one-wavefront tile sweep (4 x 4 x TZ)
 };
 std::mutex g_tuned_lock;
 std::map<std::string, TunedShape> g_tuned;
@@ -239,16 +240,24 @@ struct snap_batch {
     return a;
   }
 
-  // tile shape (TX, TY, TZ, waves, apl) -> plan, per-problem descriptors,
-  // (problem, tile) table by diagonal, face buffer
-  int build_plan(int TX, int TY, int TZ, int waves, int apl) {
+  // tile shape (TX, TY, TZ, waves, apl), or the one-wavefront sweep's
+  // 4 x 4 x TZ tiles (seq) -> plan, per-problem descriptors, (problem, tile)
+  // table by diagonal, face buffer
+  int build_plan(int TX, int TY, int TZ, int waves, int apl, bool seq = false) {
     Extent3 big{1, 1, 1};
     for (auto &e : ext) {
       big.nx = std::max(big.nx, e.nx);
       big.ny = std::max(big.ny, e.ny);
       big.nz = std::max(big.nz, e.nz);
     }
-    SNAP_TRY(plan_create(&plan, big, nang, cmom, TX, TY, TZ, waves, apl));
+    if (seq) {
+      SNAP_TRY(plan_create_seq(&plan, big, nang, cmom, TZ));
+      TX = plan.TX;
+      TY = plan.TY;
+      TZ = plan.TZ;
+    } else {
+      SNAP_TRY(plan_create(&plan, big, nang, cmom, TX, TY, TZ, waves, apl));
+    }
     const int per_tile = (TY * TZ + TX * TZ + TX * TY) * nang;
     std::vector<BatchDesc> d(count);
     int64_t face_off = 0;
@@ -265,7 +274,12 @@ struct snap_batch {
       b.hi = hi[p];
       b.hj = hj[p];
       b.hk = hk[p];
-      face_off += (int64_t)8 * ng * b.ntiles * per_tile;
+      // the one-wavefront sweep keeps its faces in place, one slot per tile
+      // row, column and pencil
+      face_off += seq ? (int64_t)8 * ng * nang *
+                            ((int64_t)b.nTy * b.nTz * TY * TZ + (int64_t)b.nTx * b.nTz * TX * TZ +
+                             (int64_t)b.nTx * b.nTy * TX * TY)
+                      : (int64_t)8 * ng * b.ntiles * per_tile;
       maxd = std::max(maxd, b.nTx + b.nTy + b.nTz - 2);
     }
     std::vector<int2> tab;
@@ -307,6 +321,7 @@ struct snap_batch {
     BatchSweep b = bs;
     b.prec = prec;
     b.prec_want = want;
+    const bool seq = plan.seq;
     SweepLaunch L;
     memset(&L, 0, sizeof(L));
     L.plan = &plan;
@@ -322,6 +337,10 @@ struct snap_batch {
     L.faces = faces.p;
     L.fp32 = fp32;
     L.batch = &b;
+    if (seq) {
+      L.recompute_dinv = true;  // dinv formed from t_xs in the kernel
+      return sweep_seq_enqueue(L, stream, nullptr);
+    }
     return sweep_enqueue(L, stream);
   }
 
@@ -446,6 +465,10 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
   if (b->precision != SNAP_PRECISION_FP64 && b->precision != SNAP_PRECISION_MIXED)
     return fail(SNAP_ERR_INVALID, "batch precision must be FP64 or MIXED");
   if (!(b->opt.fp64_polish >= 0.0)) return fail(SNAP_ERR_INVALID, "fp64_polish must be >= 0");
+  if (b->opt.sweep_kernel < SNAP_SWEEP_AUTO || b->opt.sweep_kernel > SNAP_SWEEP_WAVE_TILE)
+    return fail(SNAP_ERR_INVALID, "unknown sweep kernel %d", b->opt.sweep_kernel);
+  if (b->opt.sweep_kernel == SNAP_SWEEP_WAVE_TILE && p0.nang > 64)
+    return fail(SNAP_ERR_UNSUPPORTED, "the one-wavefront tile sweep handles up to 64 angles");
   b->polish = b->precision == SNAP_PRECISION_MIXED && b->opt.fp64_polish > 0.0;
   if (b->nmom < 1 || b->nmom > 4 || b->ng < 1 || b->nang < 1)
     return fail(SNAP_ERR_INVALID, "bad ng / nang / nmom");
@@ -578,14 +601,17 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
   // the fastest candidate on this batch
   int TX = b->opt.tile[0], TY = b->opt.tile[1], TZ = b->opt.tile[2];
   int waves = b->opt.waves_per_tile, apl = b->opt.angles_per_lane;
-  std::string sig = std::to_string(b->precision) + ":" + std::to_string(ng) + ":" +
+  const int kern = b->opt.sweep_kernel;
+  std::string sig = std::to_string(b->precision) + ":" + std::to_string(kern) + ":" +
+                    std::to_string(ng) + ":" +
                     std::to_string(nang) + ":" + std::to_string(nmom) + ":" +
                     std::to_string(waves) + ":" + std::to_string(apl);
   for (const Extent3 &e : b->ext)
     sig += ":" + std::to_string(e.nx) + "x" + std::to_string(e.ny) + "x" + std::to_string(e.nz);
   bool cached = false;
   TunedShape hit{};
-  if (!(TX > 0 && TY > 0 && TZ > 0)) {
+  const bool fixed_seq = kern == SNAP_SWEEP_WAVE_TILE && TZ > 0;
+  if (!(TX > 0 && TY > 0 && TZ > 0) && !fixed_seq) {
     std::lock_guard<std::mutex> g(g_tuned_lock);
     auto it = g_tuned.find(sig);
     if (it != g_tuned.end()) {
@@ -593,10 +619,12 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
       hit = it->second;
     }
   }
-  if (TX > 0 && TY > 0 && TZ > 0) {
+  if (fixed_seq) {
+    SNAP_TRY(b->build_plan(4, 4, TZ, 1, 1, true));
+  } else if (TX > 0 && TY > 0 && TZ > 0 && kern != SNAP_SWEEP_WAVE_TILE) {
     SNAP_TRY(b->build_plan(TX, TY, TZ, waves, apl));
   } else if (cached) {
-    SNAP_TRY(b->build_plan(hit.TX, hit.TY, hit.TZ, hit.waves, hit.apl));
+    SNAP_TRY(b->build_plan(hit.TX, hit.TY, hit.TZ, hit.waves, hit.apl, hit.seq));
     b->plan_cached = true;
   } else {
     const auto t0 = std::chrono::steady_clock::now();
@@ -604,7 +632,24 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
     float best = 1e30f;
     int bi = -1, bapl = 0;
     bool warmed[3] = {false, false, false};  // per lane layout (one kernel each)
-    for (int c = 0; c < (int)(sizeof(cand) / sizeof(cand[0])); c++) {
+    // the one-wavefront tile sweep first (steady problems of up to 64 angles,
+    // any options that only the multi-wave tiles have left unset)
+    int best_seq_tz = 0;
+    if (kern != SNAP_SWEEP_TILED && nang <= 64 && apl <= 1 && waves <= 1) {
+      bool warm = true;
+      for (int tz : {4, 8}) {
+        if (b->build_plan(4, 4, tz, 1, 1, true) != SNAP_OK) continue;
+        float ms = 0.f;
+        SNAP_TRY(b->time_plan(1, &ms, warm));
+        warm = false;
+        if (ms < best) {
+          best = ms;
+          best_seq_tz = tz;
+        }
+      }
+    }
+    const int ncand = kern == SNAP_SWEEP_WAVE_TILE ? 0 : (int)(sizeof(cand) / sizeof(cand[0]));
+    for (int c = 0; c < ncand; c++) {
       for (int a = 1; a <= 2; a++) {
         if (apl > 0 && a != apl) continue;
         if (a == 2 && !fp32 && nang <= 64) continue;  // FP64 two-angle lanes halve occupancy
@@ -621,12 +666,21 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
         }
       }
     }
-    if (bi < 0) return fail(SNAP_ERR_INVALID, "no candidate tile fits this batch");
-    SNAP_TRY(b->build_plan(cand[bi][0], cand[bi][1], cand[bi][2], waves, bapl));
+    if (bi >= 0) {  // a multi-wave shape beat the one-wavefront sweep
+      best_seq_tz = 0;
+    } else if (best_seq_tz == 0) {
+      return fail(SNAP_ERR_INVALID, "no candidate tile fits this batch");
+    }
+    TunedShape t;
+    if (best_seq_tz)
+      t = TunedShape{4, 4, best_seq_tz, 1, 1, true};
+    else
+      t = TunedShape{cand[bi][0], cand[bi][1], cand[bi][2], waves, bapl, false};
+    SNAP_TRY(b->build_plan(t.TX, t.TY, t.TZ, t.waves, t.apl, t.seq));
     b->autotune_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     std::lock_guard<std::mutex> g(g_tuned_lock);
-    g_tuned[sig] = TunedShape{cand[bi][0], cand[bi][1], cand[bi][2], waves, bapl};
+    g_tuned[sig] = t;
   }
   SNAP_HIP_TRY(hipStreamSynchronize(st));
   *out = b.release();

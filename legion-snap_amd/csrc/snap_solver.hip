@@ -264,8 +264,8 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
     if (s->opt.precision == SNAP_PRECISION_MIXED)
       SNAP_TRY(plan_create_wide(&s->plan64, s->e, s->nang, s->cmom, s->td));
   } else {
-    // the one-wavefront tile sweep for steady problems of 33..64 angles
-    // (asked for: any steady problem of up to 64 angles)
+    // the one-wavefront tile sweep: the default for steady problems of
+    // 33..64 angles, asked for on any steady problem of up to 64 angles
     const bool seq_ok = !s->td && s->nang <= 64;
     if (s->opt.sweep_kernel == SNAP_SWEEP_WAVE_TILE && !seq_ok)
       return fail(SNAP_ERR_UNSUPPORTED,
@@ -675,6 +675,9 @@ extern "C" int snap_hip_solver_gradient(snap_solver_t *s, const double *resp, in
   return SNAP_OK;
 }
 
+// cells per wavefront of the one-wavefront tile sweep: one 4 x 4 plane
+constexpr int kNP_wave_tile = 16;
+
 extern "C" int snap_hip_solver_plan(snap_solver_t *s, int32_t plan[6]) {
   if (!s || !plan) return fail(SNAP_ERR_INVALID, "null argument");
   const SweepPlan &p = s->sweep_plan(s->opt.precision);
@@ -683,7 +686,7 @@ extern "C" int snap_hip_solver_plan(snap_solver_t *s, int32_t plan[6]) {
   plan[2] = p.TZ;
   plan[3] = p.waves;
   plan[4] = p.apl;
-  plan[5] = p.cpw;
+  plan[5] = p.seq ? kNP_wave_tile : p.cpw;
   return SNAP_OK;
 }
 

@@ -111,6 +111,9 @@ size_t SweepPlan::lds_bytes(int real_bytes, bool recompute_dinv) const {
 }
 
 size_t SweepPlan::face_doubles_per_sweep() const {
+  if (seq)  // in-place pencils: x faces per (ty, tz), y per (tx, tz), z per (tx, ty)
+    return ((size_t)nTy * nTz * TY * TZ + (size_t)nTx * nTz * TX * TZ +
+            (size_t)nTx * nTy * TX * TY) * nang;
   if (wide)  // in-place x, y and z face planes
     return ((size_t)ext.ny * ext.nz + (size_t)ext.nx * ext.nz + (size_t)ext.nx * ext.ny) * nang;
   return (size_t)ntiles * (TY * TZ + TX * TZ + TX * TY) * nang;

@@ -10,8 +10,9 @@
 // never leave registers: the x inflows are one carry per row, the y inflows
 // a 4-entry pencil and the z inflows a 16-entry plane, all indexed at compile
 // time (the loops over a plane are unrolled).  Faces reach HBM only at tile
-// entry and exit.  Lanes carry angles (up to 64 per octant, one per lane), so
-// the fixup's negative count is a wavefront popcount.
+// entry and exit, in place: one slot per tile row, column and pencil.
+// Lanes carry angles (up to 64 per octant, one per lane), so the fixup's
+// negative count is a wavefront popcount.
 //
 // Full tiles of the fused and FP32 sweeps take the fast path, per z plane:
 //  1. the angular sources of the plane's 16 cells, psi[cell][a] =
@@ -77,8 +78,16 @@ struct SeqArgs {
   const void *txs;      // [ng_job][cells] of Real
   double *part_flux;    // [8][ng_job][cells]
   double *part_fluxm;   // [8][ng_job][cells][cmom-1]
-  void *faces;          // [nsweeps][ntiles][x|y|z face slots][nang] of Real
+  void *faces;          // [nsweeps][x faces per (ty,tz) | y per (tx,tz) | z per (tx,ty)] of Real
   const int *plane_tiles;
+  // a batch of problems (BATCH kernels): per-launch (problem, tile) entries,
+  // per-problem chunks (their faces in [nsweeps][pencils] from face_off),
+  // and the predicates that skip finished problems or the other precision
+  const int2 *batch_tiles;
+  const BatchDesc *descs;
+  const int *active, *done, *prec;
+  int prec_want;
+  int64_t total_cells;
 };
 
 __device__ inline void wave_sync() {
@@ -111,7 +120,7 @@ struct TileGeom {
 
 // CMOM: flux moments (1, 4, 9 or 16), a compile-time count so that every loop
 // over moments unrolls with register operands at fixed places.
-template <typename Real, bool EXACT, bool REC, int CMOM>
+template <typename Real, bool EXACT, bool REC, int CMOM, bool BATCH = false>
 __global__ void __launch_bounds__(64 * kWGW) __attribute__((amdgpu_waves_per_eu(SEQ_WAVES, 8)))
 k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   constexpr bool kFP64 = std::is_same<Real, double>::value;
@@ -155,14 +164,39 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   double *P = W + kNP * RS + 64;
   Real *Zb = (Real *)(P + (kNP * RS > kNP * 16 ? kNP * RS : kNP * 16));
   Real *qs = (Real *)P;
-  const int tp = A.plane_tiles[plane_off + tix];
+  // the chunk: the launch's own, or one problem of a batch
+  Extent3 ext = A.ext;
+  int nTx = A.nTx, nTy = A.nTy, nTz = A.nTz;
+  double dhi = A.hi, dhj = A.hj, dhk = A.hk;
+  size_t coff = 0, face_base = 0, gstride;
+  int tp;
+  if constexpr (BATCH) {
+    const int2 bt = A.batch_tiles[plane_off + tix];
+    if (!A.active[bt.x] || A.done[bt.x]) return;  // the whole wavefront
+    if (A.prec && A.prec[bt.x] != A.prec_want) return;
+    const BatchDesc &Pd = A.descs[bt.x];
+    ext = Pd.ext;
+    nTx = Pd.nTx;
+    nTy = Pd.nTy;
+    nTz = Pd.nTz;
+    dhi = Pd.hi;
+    dhj = Pd.hj;
+    dhk = Pd.hk;
+    coff = (size_t)Pd.cell_off;
+    face_base = (size_t)Pd.face_off;
+    gstride = (size_t)A.total_cells;
+    tp = bt.y;
+  } else {
+    gstride = (size_t)ext.nx * ext.ny * ext.nz;
+    tp = A.plane_tiles[plane_off + tix];
+  }
   const int tx = tp & 1023, ty = (tp >> 10) & 1023, tz = (tp >> 20) & 1023;
   constexpr int cmom = CMOM;
   const int TZ = A.TZ;
   TileGeom T;
-  T.nx = A.ext.nx;
-  T.ny = A.ext.ny;
-  T.nz = A.ext.nz;
+  T.nx = ext.nx;
+  T.ny = ext.ny;
+  T.nz = ext.nz;
   T.x0 = tx * kSX;
   T.y0 = ty * kSY;
   T.z0 = tz * TZ;
@@ -170,26 +204,29 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   T.ey = min(kSY, T.ny - T.y0);
   T.ez = min(TZ, T.nz - T.z0);
   T.init(corner);
-  const size_t gstride = (size_t)T.nx * T.ny * T.nz;
 
-  // tile faces: this tile's outflow regions and its upstream tiles' ones
+  // tile faces, in place: the x faces of a tile row (ty, tz) share one slot,
+  // which the upstream tile (tx-1) wrote one launch earlier and this tile
+  // reads before it overwrites it with its own outflow (only one tile of the
+  // row is on a tile diagonal); likewise the y faces per (tx, tz) and the z
+  // faces per (tx, ty)
   const int fxn = kSY * TZ * nang, fyn = kSX * TZ * nang, fzn = kSX * kSY * nang;
-  const int per_tile = fxn + fyn + fzn;
-  const int tile_id = (tz * A.nTy + ty) * A.nTx + tx;
-  Real *faces_s = (Real *)A.faces + (size_t)s * A.ntiles * per_tile;
-  Real *out_x = faces_s + (size_t)tile_id * per_tile;
-  Real *out_y = out_x + fxn;
-  Real *out_z = out_y + fyn;
-  const Real *in_x = tx > 0 ? faces_s + (size_t)(tile_id - 1) * per_tile : nullptr;
-  const Real *in_y = ty > 0 ? faces_s + (size_t)(tile_id - A.nTx) * per_tile + fxn : nullptr;
-  const Real *in_z =
-      tz > 0 ? faces_s + (size_t)(tile_id - A.nTx * A.nTy) * per_tile + fxn + fyn : nullptr;
+  const size_t nfx = (size_t)nTy * nTz * fxn, nfy = (size_t)nTx * nTz * fyn,
+               nfz = (size_t)nTx * nTy * fzn;
+  Real *faces_s = (Real *)A.faces + face_base + (size_t)s * (nfx + nfy + nfz);
+  Real *out_x = faces_s + (size_t)(tz * nTy + ty) * fxn;
+  Real *out_y = faces_s + nfx + (size_t)(tz * nTx + tx) * fyn;
+  Real *out_z = faces_s + nfx + nfy + (size_t)(ty * nTx + tx) * fzn;
+  const Real *in_x = tx > 0 ? out_x : nullptr;
+  const Real *in_y = ty > 0 ? out_y : nullptr;
+  const Real *in_z = tz > 0 ? out_z : nullptr;
 
-  const double *qg = A.q + (size_t)g * gstride * cmom;
-  const Real *dg = (const Real *)A.dinv + (size_t)g * gstride * nang;
-  const Real *tg = (const Real *)A.txs + (size_t)g * gstride;
-  double *pfg = A.part_flux + ((size_t)corner * A.ng_job + g) * gstride;
-  double *pmg = A.part_fluxm + ((size_t)corner * A.ng_job + g) * gstride * (size_t)(cmom - 1);
+  const double *qg = A.q + ((size_t)g * gstride + coff) * cmom;
+  const Real *dg = (const Real *)A.dinv + ((size_t)g * gstride + coff) * nang;
+  const Real *tg = (const Real *)A.txs + (size_t)g * gstride + coff;
+  double *pfg = A.part_flux + ((size_t)corner * A.ng_job + g) * gstride + coff;
+  double *pmg =
+      A.part_fluxm + (((size_t)corner * A.ng_job + g) * gstride + coff) * (size_t)(cmom - 1);
 
   // ---- per-lane constants (lane = angle; lanes past nang carry angle 0
   // with zero weight and never store)
@@ -199,14 +236,14 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   const double w_d = aok ? A.w[a] : 0.0;
   // EXACT keeps (psi*mu)*hi as the reference writes it; the others fold the
   // spacing into the direction cosine
-  const Real cx = EXACT ? (Real)mu_d : (Real)(mu_d * A.hi);
-  const Real cy = EXACT ? (Real)eta_d : (Real)(eta_d * A.hj);
-  const Real cz = EXACT ? (Real)xi_d : (Real)(xi_d * A.hk);
-  const Real hi = (Real)A.hi, hj = (Real)A.hj, hk = (Real)A.hk;
+  const Real cx = EXACT ? (Real)mu_d : (Real)(mu_d * dhi);
+  const Real cy = EXACT ? (Real)eta_d : (Real)(eta_d * dhj);
+  const Real cz = EXACT ? (Real)xi_d : (Real)(xi_d * dhk);
+  const Real hi = (Real)dhi, hj = (Real)dhj, hk = (Real)dhk;
   const double *ecc = A.ec + (size_t)corner * cmom * nang;
   // dinv formed in the kernel: CalculateGeometryParam's sum (expxs.cc:378-381)
-  const double dsum_x = A.hi * mu_d, dsum_y = A.hj * eta_d, dsum_z = A.hk * xi_d;
-  const float gsum32 = (float)((A.hi * mu_d + A.hj * eta_d) + A.hk * xi_d);
+  const double dsum_x = dhi * mu_d, dsum_y = dhj * eta_d, dsum_z = dhk * xi_d;
+  const float gsum32 = (float)((dhi * mu_d + dhj * eta_d) + dhk * xi_d);
   const Real tolr = (Real)1.0e-12;
   auto dinv_of = [&](Real t) -> Real {
     if constexpr (kFP64 && EXACT)
@@ -743,11 +780,25 @@ static SeqKernel seq_kernel(int cmom, bool rec) {
   }
 }
 
+// Batch kernels: fused FP64 or FP32, dinv formed in the kernel.
+template <typename Real>
+static SeqKernel seq_kernel_batch(int cmom) {
+  switch (cmom) {
+    case 1: return k_sweep_seq<Real, false, true, 1, true>;
+    case 4: return k_sweep_seq<Real, false, true, 4, true>;
+    case 9: return k_sweep_seq<Real, false, true, 9, true>;
+    case 16: return k_sweep_seq<Real, false, true, 16, true>;
+    default: return nullptr;
+  }
+}
+
 int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
   const SweepPlan &p = *L.plan;
   if (!p.seq) return fail(SNAP_ERR_INVALID, "not a one-wavefront tile plan");
-  if (L.general || L.batch || L.td)
+  if (L.general || L.td)
     return fail(SNAP_ERR_INVALID, "the one-wavefront tile sweep is dense and steady");
+  if (L.batch && (L.exact || !L.recompute_dinv))
+    return fail(SNAP_ERR_INVALID, "one-wavefront batch sweeps are fused and form dinv in the kernel");
   if (!L.q || !L.txs_dense || !L.part_flux || (p.cmom > 1 && !L.part_fluxm) ||
       (!L.recompute_dinv && !L.dinv_dense))
     return fail(SNAP_ERR_INVALID, "dense sweep inputs missing");
@@ -781,18 +832,31 @@ int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) 
   A.part_fluxm = L.part_fluxm;
   A.faces = L.faces;
   A.plane_tiles = p.plane_tiles;
-  const SeqKernel k = L.fp32 ? seq_kernel<float, false>(p.cmom, L.recompute_dinv)
+  if (L.batch) {
+    A.batch_tiles = L.batch->batch_tiles;
+    A.descs = L.batch->descs;
+    A.active = L.batch->active;
+    A.done = L.batch->done;
+    A.prec = L.batch->prec;
+    A.prec_want = L.batch->prec_want;
+    A.total_cells = L.batch->total_cells;
+  }
+  const SeqKernel k = L.batch ? (L.fp32 ? seq_kernel_batch<float>(p.cmom)
+                                        : seq_kernel_batch<double>(p.cmom))
+                   : L.fp32 ? seq_kernel<float, false>(p.cmom, L.recompute_dinv)
                    : L.exact ? seq_kernel<double, true>(p.cmom, L.recompute_dinv)
                              : seq_kernel<double, false>(p.cmom, L.recompute_dinv);
   if (!k) return fail(SNAP_ERR_UNSUPPORTED, "no one-wavefront tile sweep for %d moments", p.cmom);
   const size_t lds = seq_lds_bytes(p.nang, p.cmom);
   SNAP_TRY(ensure_dynamic_lds((const void *)k, lds));
-  for (int D = 0; D < p.nplanes; D++) {
-    const int count = p.plane_off[D + 1] - p.plane_off[D];
+  const int nplanes = L.batch ? L.batch->nplanes : p.nplanes;
+  const std::vector<int> &poff = L.batch ? L.batch->plane_off : p.plane_off;
+  for (int D = 0; D < nplanes; D++) {
+    const int count = poff[D + 1] - poff[D];
     if (count == 0) continue;
     if (events) SNAP_HIP_TRY(hipEventRecord(events[2 * D], st));
     hipLaunchKernelGGL(k, dim3((count + kWGW - 1) / kWGW, L.nsweeps), dim3(64 * kWGW), lds, st, A,
-                       p.plane_off[D], count);
+                       poff[D], count);
     if (events) SNAP_HIP_TRY(hipEventRecord(events[2 * D + 1], st));
   }
   SNAP_HIP_TRY(hipGetLastError());

@@ -89,7 +89,8 @@ class SolveOptions(ctypes.Structure):
 class BatchOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("tile", ctypes.c_int * 3),
                 ("waves_per_tile", ctypes.c_int), ("angles_per_lane", ctypes.c_int),
-                ("use_graph", ctypes.c_int), ("fp64_polish", ctypes.c_double)]
+                ("use_graph", ctypes.c_int), ("fp64_polish", ctypes.c_double),
+                ("sweep_kernel", ctypes.c_int)]
 
 
 class ProblemStats(ctypes.Structure):

@@ -41,6 +41,31 @@ def test_batch_equals_single_solves(gpu, precision, polish, use_graph):
         assert any(x["sweeps_fp64"] > 0 for x in per)
 
 
+@pytest.mark.parametrize("precision,polish", [("fp64", None), ("mixed", 0.0), ("mixed", None)])
+@pytest.mark.parametrize("tz", [4, 8])
+def test_wave_tile_batch_equals_single_solves(gpu, precision, polish, tz):
+    # the batch on one-wavefront 4 x 4 x TZ tiles against single solves on
+    # the same kernel: the batch form changes addressing and predication only
+    probs = _small_batch()
+    with Batch(probs, precision=precision, fp64_polish=polish, sweep_kernel="wave_tile",
+               tile=(0, 0, tz)) as b:
+        st, per = b.run()
+        fluxes = [b.flux(i) for i in range(len(probs))]
+    print(st)
+    assert st["waves"] == 1 and tuple(st["tile"][:2]) == (4, 4)
+    assert st["converged"] == len(probs)
+    for i, p in enumerate(probs):
+        with Solver(p, precision=precision, fp64_polish=polish, refine=False,
+                    sweep_kernel="wave_tile", tile=(0, 0, tz)) as s:
+            ss = s.run()
+            f, fm = s.flux()
+        assert (per[i]["inner_total"], per[i]["outer_total"]) == (ss["inner_total"], ss["outer_total"])
+        assert per[i]["sweeps_fp64"] == ss["sweeps_fp64"]
+        assert np.array_equal(fluxes[i][0], f)
+        if fm is not None:
+            assert np.array_equal(fluxes[i][1], fm)
+
+
 def test_batch_reuses_the_tuned_tile(gpu):
     probs = _small_batch(count=3, seed=9)
     with Batch(probs, precision="fp64") as b:
