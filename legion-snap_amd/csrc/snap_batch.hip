@@ -300,7 +300,11 @@ struct snap_batch {
     SNAP_HIP_TRY(hipMemcpy(descs.p, d.data(), sizeof(BatchDesc) * count, hipMemcpyHostToDevice));
     SNAP_HIP_TRY(hipMemcpy(tiles.p, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice));
     const size_t face_bytes = (size_t)face_off * sizeof(double);
-    if (faces.bytes < face_bytes) SNAP_TRY(faces.alloc(face_bytes));
+    // (a much larger buffer left by an autotuner candidate is given back)
+    if (faces.bytes < face_bytes || faces.bytes > 2 * face_bytes) {
+      SNAP_HIP_TRY(hipStreamSynchronize(stream));
+      SNAP_TRY(faces.alloc(face_bytes));
+    }
     bs.batch_tiles = tiles.as<int2>();
     bs.descs = descs.as<BatchDesc>();
     bs.active = active.as<int>();
