@@ -48,6 +48,19 @@ constexpr int kSX = 4, kSY = 4, kNP = kSX * kSY;  // tile cross-section, cells p
 // they share the sweep's matrix-core operand tables in LDS.
 constexpr int kWGW = 4;
 
+// Per-wavefront LDS, in bytes, for row stride rs and a sweep type of rsz
+// bytes: W (w*pc, FP64 moment-product operand, 16 rows plus a junk row of 64
+// slots), P (the plane's angular sources, or its 16 x 16 source moments on
+// the general path, in the sweep type) and Zb (the plane's z inflows, in the
+// sweep type).  FP32 sweeps thus take about a third less than FP64 ones.
+__host__ __device__ constexpr size_t seq_w_bytes(int rs) { return 8 * ((size_t)kNP * rs + 64); }
+__host__ __device__ constexpr size_t seq_p_bytes(int rs, int rsz) {
+  return (size_t)rsz * (kNP * rs > kNP * 16 ? kNP * rs : kNP * 16);
+}
+__host__ __device__ constexpr size_t seq_wave_bytes(int rs, int rsz) {
+  return seq_w_bytes(rs) + seq_p_bytes(rs, rsz) + (size_t)rsz * kNP * rs;
+}
+
 // a*b + c with one rounding in the sweep's type
 __device__ inline double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ inline float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -159,11 +172,12 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   // angular sources (fast path) or their source moments [slot][16]
   // (general); Z: the plane's z inflows [slot][angle] while the fixup may
   // sweep it again (fast path)
-  double *W = EA + (size_t)NST * 64 +
-              (size_t)wave * (2 * (kNP * RS + 64) + (kNP * RS > kNP * 16 ? kNP * RS : kNP * 16));
-  double *P = W + kNP * RS + 64;
-  Real *Zb = (Real *)(P + (kNP * RS > kNP * 16 ? kNP * RS : kNP * 16));
-  Real *qs = (Real *)P;
+  unsigned char *wbase =
+      (unsigned char *)(EA + (size_t)NST * 64) + (size_t)wave * seq_wave_bytes(RS, sizeof(Real));
+  double *W = (double *)wbase;
+  Real *P = (Real *)(wbase + seq_w_bytes(RS));
+  Real *Zb = (Real *)(wbase + seq_w_bytes(RS) + seq_p_bytes(RS, sizeof(Real)));
+  Real *qs = P;
   // the chunk: the launch's own, or one problem of a batch
   Extent3 ext = A.ext;
   int nTx = A.nTx, nTy = A.nTy, nTz = A.nTz;
@@ -495,7 +509,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
                                                      0, 0, 0);
         if (ang < RS) {
 #pragma unroll
-          for (int r = 0; r < 4; r++) P[(4 * r + q4) * RS + ang] = acc[r];
+          for (int r = 0; r < 4; r++) P[(4 * r + q4) * RS + ang] = (Real)acc[r];
         }
       }
       // the next plane's operands, in flight while this plane is swept
@@ -533,7 +547,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
             const int j = dg2 - i;
             if (j < 0 || j >= kSY) continue;  // resolved at compile time
             const int n = j * kSX + i;
-            const Real psi = (Real)P[n * RS + a];
+            const Real psi = P[n * RS + a];
             Real d;
             if constexpr (REC)
               d = dinv_of(bcast(tv_c, n));
@@ -573,14 +587,14 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
         // selects and dinv is formed again.
         Real xc = Real(0), yq0 = Real(0), yq1 = Real(0), yq2 = Real(0), yq3 = Real(0);
         // each cell's LDS operands are read one cell ahead
-        Real psi_nx = (Real)P[a], pk_nx = Zb[zslot(0)];
+        Real psi_nx = P[a], pk_nx = Zb[zslot(0)];
 #pragma unroll 1
         for (int n = 0; n < kNP; n++) {
           const int i = n & 3, j = n >> 2;
           const int c = T.c0 + i * T.sx + j * T.sy + k * T.sz;
           const Real psi = psi_nx, pk_ = pk_nx;
           if (n + 1 < kNP) {
-            psi_nx = (Real)P[(n + 1) * RS + a];
+            psi_nx = P[(n + 1) * RS + a];
             pk_nx = Zb[zslot(n + 1)];
           }
           const Real txs = bcast(tv_c, n);
@@ -715,11 +729,10 @@ namespace {
 // LDS row stride of the [cell][angle] tables: the angles rounded up to the
 // four-angle matrix steps, plus one so the 16 rows start on different banks
 int seq_row_stride(int nang) { return ((nang + 3) & ~3) + 1; }
-size_t seq_lds_bytes(int nang, int cmom) {
+size_t seq_lds_bytes(int nang, int cmom, bool fp32) {
   const int rs = seq_row_stride(nang);
   const size_t tables = (size_t)((nang + 15) / 16) * ((cmom + 3) / 4) * 64 + (size_t)((nang + 3) / 4) * 64;
-  const size_t per_wave = 2 * ((size_t)kNP * rs + 64) + std::max(kNP * rs, kNP * 16);
-  return sizeof(double) * (tables + kWGW * per_wave);
+  return sizeof(double) * tables + kWGW * seq_wave_bytes(rs, fp32 ? 4 : 8);
 }
 }  // namespace
 
@@ -847,7 +860,7 @@ int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) 
                    : L.exact ? seq_kernel<double, true>(p.cmom, L.recompute_dinv)
                              : seq_kernel<double, false>(p.cmom, L.recompute_dinv);
   if (!k) return fail(SNAP_ERR_UNSUPPORTED, "no one-wavefront tile sweep for %d moments", p.cmom);
-  const size_t lds = seq_lds_bytes(p.nang, p.cmom);
+  const size_t lds = seq_lds_bytes(p.nang, p.cmom, L.fp32);
   SNAP_TRY(ensure_dynamic_lds((const void *)k, lds));
   const int nplanes = L.batch ? L.batch->nplanes : p.nplanes;
   const std::vector<int> &poff = L.batch ? L.batch->plane_off : p.plane_off;
