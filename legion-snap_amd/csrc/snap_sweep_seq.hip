@@ -49,16 +49,20 @@ constexpr int kSX = 4, kSY = 4, kNP = kSX * kSY;  // tile cross-section, cells p
 constexpr int kWGW = 4;
 
 // Per-wavefront LDS, in bytes, for row stride rs and a sweep type of rsz
-// bytes: W (w*pc, FP64 moment-product operand, 16 rows plus a junk row of 64
-// slots), P (the plane's angular sources, or its 16 x 16 source moments on
-// the general path, in the sweep type) and Zb (the plane's z inflows, in the
-// sweep type).  FP32 sweeps thus take about a third less than FP64 ones.
-__host__ __device__ constexpr size_t seq_w_bytes(int rs) { return 8 * ((size_t)kNP * rs + 64); }
+// bytes: W (the moment product's cell operand, 16 rows plus a junk row of 64
+// slots: w*pc in FP64 for the exact kernels, pc in the sweep type for the
+// others, whose moment table carries the weights), P (the plane's angular
+// sources, or its 16 x 16 source moments on the general path, in the sweep
+// type) and Zb (the plane's z inflows, in the sweep type).  An FP32 sweep
+// thus takes half the per-wavefront LDS of an FP64 one.
+__host__ __device__ constexpr size_t seq_w_bytes(int rs, int wsz) {
+  return (size_t)wsz * ((size_t)kNP * rs + 64);
+}
 __host__ __device__ constexpr size_t seq_p_bytes(int rs, int rsz) {
   return (size_t)rsz * (kNP * rs > kNP * 16 ? kNP * rs : kNP * 16);
 }
-__host__ __device__ constexpr size_t seq_wave_bytes(int rs, int rsz) {
-  return seq_w_bytes(rs) + seq_p_bytes(rs, rsz) + (size_t)rsz * kNP * rs;
+__host__ __device__ constexpr size_t seq_wave_bytes(int rs, int rsz, int wsz) {
+  return seq_w_bytes(rs, wsz) + seq_p_bytes(rs, rsz) + (size_t)rsz * kNP * rs;
 }
 
 // a*b + c with one rounding in the sweep's type
@@ -149,6 +153,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   // the sweep's operand tables, shared by the workgroup's wavefronts:
   // EB[nb][kk][lane] = ec[4kk + lane/16][16nb + lane%16] (the source
   // product's B, ec[0] = 1) and EA[st][lane] = ec[lane%16][4st + lane/16]
+  // (the moment product's A, times the angle's weight unless EXACT)
   // (the moment product's A), zero outside the moments and angles
   double *EB = (double *)smem_raw;
   double *EA = EB + (size_t)NB * KS * 64;
@@ -162,21 +167,23 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
     for (int idx = threadIdx.x; idx < NST * 64; idx += 64 * kWGW) {
       const int L = idx & 63, st = idx >> 6;
       const int l = L & 15, ang = 4 * st + (L >> 4);
-      EA[idx] = (l < CMOM && ang < nang) ? ((l == 0) ? 1.0 : ecc0[(size_t)l * nang + ang]) : 0.0;
+      const double e = (l < CMOM && ang < nang) ? ((l == 0) ? 1.0 : ecc0[(size_t)l * nang + ang]) : 0.0;
+      EA[idx] = (EXACT || ang >= nang) ? e : e * A.w[ang];
     }
   }
   __syncthreads();  // the only workgroup barrier: before any wavefront can leave
   const int tix = blockIdx.x * kWGW + wave;
   if (tix >= plane_count) return;
-  // per wavefront: W: w*pc of the plane's cells [slot][angle]; P: their
+  // per wavefront: W: (w*)pc of the plane's cells [slot][angle]; P: their
   // angular sources (fast path) or their source moments [slot][16]
   // (general); Z: the plane's z inflows [slot][angle] while the fixup may
   // sweep it again (fast path)
-  unsigned char *wbase =
-      (unsigned char *)(EA + (size_t)NST * 64) + (size_t)wave * seq_wave_bytes(RS, sizeof(Real));
-  double *W = (double *)wbase;
-  Real *P = (Real *)(wbase + seq_w_bytes(RS));
-  Real *Zb = (Real *)(wbase + seq_w_bytes(RS) + seq_p_bytes(RS, sizeof(Real)));
+  using WT = typename std::conditional<EXACT, double, Real>::type;
+  unsigned char *wbase = (unsigned char *)(EA + (size_t)NST * 64) +
+                         (size_t)wave * seq_wave_bytes(RS, sizeof(Real), sizeof(WT));
+  WT *W = (WT *)wbase;
+  Real *P = (Real *)(wbase + seq_w_bytes(RS, sizeof(WT)));
+  Real *Zb = (Real *)(wbase + seq_w_bytes(RS, sizeof(WT)) + seq_p_bytes(RS, sizeof(Real)));
   Real *qs = P;
   // the chunk: the launch's own, or one problem of a batch
   Extent3 ext = A.ext;
@@ -296,6 +303,14 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   // 0 and store its faces again) read and write angle 0's slot, with the
   // values angle 0 has
   auto zslot = [&](int n) { return n * RS + a; };
+  // a cell's W entry: w*pc (EXACT: summed in the reference's order) or pc
+  // (the moment table carries the weights); zero for lanes past nang
+  auto w_of = [&](Real pc) -> WT {
+    if constexpr (EXACT)
+      return aok ? w_d * (double)pc : 0.0;
+    else
+      return aok ? pc : Real(0);
+  };
 
   // ---- one cell of the general path: DD update, the reference's fixup loop
   // (sweep.cc:402-520) and the outflows; returns pc (fixed: some outflow of
@@ -397,20 +412,20 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
       }
     } else {
       // phi[l][cell] over four angles per product: A[l][kk] = ec[l][4s+kk]
-      // (ec[0] = 1) in lane l + 16 kk, B[kk][cell] = w*pc in lane cell + 16 kk,
+      // (ec[0] = 1) times w in lane l + 16 kk, B[kk][cell] = pc in lane cell + 16 kk,
       // D[l][cell] in lane cell + 16 (l % 4), register l / 4
       // two accumulators halve the dependent-product chain
       f64x4 acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
       int st = 0;
       for (; st + 1 < NST; st += 2) {
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(EA[st * 64 + lane], W[r16 * RS + 4 * st + q4],
-                                                   acc, 0, 0, 0);
-        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(EA[(st + 1) * 64 + lane],
-                                                    W[r16 * RS + 4 * st + 4 + q4], acc2, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
+            EA[st * 64 + lane], (double)W[r16 * RS + 4 * st + q4], acc, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(
+            EA[(st + 1) * 64 + lane], (double)W[r16 * RS + 4 * st + 4 + q4], acc2, 0, 0, 0);
       }
       if (st < NST)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(EA[st * 64 + lane], W[r16 * RS + 4 * st + q4],
-                                                   acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
+            EA[st * 64 + lane], (double)W[r16 * RS + 4 * st + q4], acc, 0, 0, 0);
       acc += acc2;
       const int n = r16;
       const int c = T.cell(n & 3, n >> 2, k);
@@ -569,7 +584,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
             // lanes past nang mirror lane 0 and store its values again
             if (i == kSX - 1) out_x[(size_t)(k * kSY + j) * nang + a] = ox;
             if (j == kSY - 1) out_y[(size_t)(k * kSX + i) * nang + a] = oy;
-            W[wslot(n)] = aok ? w_d * (double)pc : 0.0;
+            W[wslot(n)] = w_of(pc);
           }
           __builtin_amdgcn_sched_barrier(0);
         }
@@ -619,7 +634,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
           Zb[zslot(n)] = oz;
           if (i == kSX - 1) out_x[(size_t)(k * kSY + j) * nang + a] = ox;
           if (j == kSY - 1) out_y[(size_t)(k * kSX + i) * nang + a] = oy;
-          W[wslot(n)] = aok ? w_d * (double)pc : 0.0;
+          W[wslot(n)] = w_of(pc);
         }
 #pragma unroll
         for (int n = 0; n < kNP; n++) zf[n] = Zb[zslot(n)];
@@ -701,7 +716,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
           if (i == T.ex - 1) out_x[(size_t)(k * kSY + j) * nang + a] = ox;
           if (j == T.ey - 1) out_y[(size_t)(k * kSX + i) * nang + a] = oy;
         }
-        W[wslot(n)] = aok ? w_d * (double)pc : 0.0;
+        W[wslot(n)] = w_of(pc);
       }
       wave_sync();
       moments(k);
@@ -729,10 +744,11 @@ namespace {
 // LDS row stride of the [cell][angle] tables: the angles rounded up to the
 // four-angle matrix steps, plus one so the 16 rows start on different banks
 int seq_row_stride(int nang) { return ((nang + 3) & ~3) + 1; }
-size_t seq_lds_bytes(int nang, int cmom, bool fp32) {
+size_t seq_lds_bytes(int nang, int cmom, bool fp32, bool exact) {
   const int rs = seq_row_stride(nang);
   const size_t tables = (size_t)((nang + 15) / 16) * ((cmom + 3) / 4) * 64 + (size_t)((nang + 3) / 4) * 64;
-  return sizeof(double) * tables + kWGW * seq_wave_bytes(rs, fp32 ? 4 : 8);
+  const int rsz = fp32 ? 4 : 8;
+  return sizeof(double) * tables + kWGW * seq_wave_bytes(rs, rsz, exact ? 8 : rsz);
 }
 }  // namespace
 
@@ -860,7 +876,7 @@ int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) 
                    : L.exact ? seq_kernel<double, true>(p.cmom, L.recompute_dinv)
                              : seq_kernel<double, false>(p.cmom, L.recompute_dinv);
   if (!k) return fail(SNAP_ERR_UNSUPPORTED, "no one-wavefront tile sweep for %d moments", p.cmom);
-  const size_t lds = seq_lds_bytes(p.nang, p.cmom, L.fp32);
+  const size_t lds = seq_lds_bytes(p.nang, p.cmom, L.fp32, L.exact && !L.fp32);
   SNAP_TRY(ensure_dynamic_lds((const void *)k, lds));
   const int nplanes = L.batch ? L.batch->nplanes : p.nplanes;
   const std::vector<int> &poff = L.batch ? L.batch->plane_off : p.plane_off;
