@@ -365,8 +365,10 @@ int snap_hip_solver_get_flux(snap_solver_t *solver, double *flux0, double *fluxm
  * on the source left by the last run, after one untimed sweep.  Writes the
  * mean GPU time of one inner sweep, including the sum of its octant partials
  * into the flux moments (which the reference does inside its sweep kernel),
- * and the mean duration of one launch of the sweep kernel.  MIXED needs a
- * solver created with precision MIXED. */
+ * the mean duration of one launch of the sweep kernel and the launches per
+ * inner sweep (one per tile diagonal and stream: the one-wavefront tile sweep
+ * runs half of the corners on a second stream).  MIXED needs a solver
+ * created with precision MIXED. */
 int snap_hip_solver_time_sweeps(snap_solver_t *solver, int count, int precision,
                                 double *ms_per_sweep, double *ms_per_launch,
                                 int *launches_per_sweep);

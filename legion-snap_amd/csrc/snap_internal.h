@@ -264,6 +264,11 @@ struct SweepLaunch {
   bool exact;                  // FP64 in the reference's operation order, no fused ops
   bool recompute_dinv;         // form dinv from t_xs in the kernel instead of reading it
   const BatchSweep *batch;     // non-NULL: a batch of problems (plan gives the tile shape)
+  // one-wavefront tile sweeps of one problem: a second stream that takes half
+  // of the corners (used when side, fork and join are all set), so that the
+  // tail of each launch on one stream overlaps work on the other
+  hipStream_t side;
+  hipEvent_t fork, join;
   // time-dependent terms (td): vdelt per job group (dense) or vdelt0 (general);
   // time-edge fluxes in / out and the MMS angular source (NULL = none), dense
   // [8][ng_job][cells][nang] doubles or per-cell accessors of nang doubles
@@ -281,8 +286,12 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events = nul
 // The plan of the one-wavefront tile sweep (snap_sweep_seq.hip): 4 x 4 x TZ
 // tiles, up to 64 angles, steady dense sweeps.
 int plan_create_seq(SweepPlan *p, Extent3 e, int nang, int cmom, int TZ);
-// Enqueue every tile diagonal of a one-wavefront tile sweep (dense, steady).
+// Enqueue every tile diagonal of a one-wavefront tile sweep (dense, steady):
+// nplanes launches per stream it uses (sweep_seq_streams).  When `events` is
+// given (2 * nplanes * streams events) launch q of diagonal D is bracketed by
+// events[2 (D streams + q)] and the next one, on its own stream.
 int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events = nullptr);
+int sweep_seq_streams(const SweepLaunch &L);
 // Quadrature of the drop-in context on device `dev` (NULL before init).
 const QuadDev *context_quad(int dev);
 

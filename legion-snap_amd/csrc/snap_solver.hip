@@ -69,6 +69,9 @@ struct snap_solver {
   // every inner sweep when present; the adjoint keeps the multi-wave plan
   SweepPlan plan_seq;
   hipStream_t stream = nullptr;
+  // the one-wavefront tile sweep's second stream and its fork / join events
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int32_t *h_flag = nullptr;  // pinned: [0] the test's flag, [2..3] max change bits
   DeviceBuffer mat, sigt, slgg, qi, txs, txs32, sxs, dinv, dinv32;
@@ -94,7 +97,10 @@ struct snap_solver {
     quad_free(&quad);
     for (auto &x : ev)
       if (x) (void)hipEventDestroy(x);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
     if (h_flag) (void)hipHostFree(h_flag);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -113,6 +119,8 @@ struct snap_solver {
   const SweepPlan &sweep_plan(int precision) const {
     return plan_seq.nplanes ? plan_seq : plan_for(precision);
   }
+  // streams an inner sweep's launches use (launches per tile diagonal)
+  int sweep_streams() const { return (plan_seq.nplanes && side) ? 2 : 1; }
 
   // tiled_only: the multi-wave tile plan even when the one-wavefront plan
   // exists (the adjoint's checkpoint faces are in that plan's layout)
@@ -155,6 +163,9 @@ struct snap_solver {
     }
     if (plan_seq.nplanes && !tiled_only) {
       L.plan = &plan_seq;
+      L.side = side;
+      L.fork = fork_ev;
+      L.join = join_ev;
       return sweep_seq_enqueue(L, stream, events);
     }
     return sweep_enqueue(L, stream, events);
@@ -296,6 +307,11 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
 
   SNAP_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   for (auto &x : s->ev) SNAP_HIP_TRY(hipEventCreate(&x));
+  if (s->plan_seq.nplanes) {
+    SNAP_HIP_TRY(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+    SNAP_HIP_TRY(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
+    SNAP_HIP_TRY(hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming));
+  }
   SNAP_HIP_TRY(hipHostMalloc((void **)&s->h_flag, 16, hipHostMallocDefault));
 
   const size_t gc = (size_t)s->ng * s->cells;
@@ -565,7 +581,7 @@ extern "C" int snap_hip_solver_time_sweeps(snap_solver_t *s, int count, int prec
   const bool fp32 = precision == SNAP_PRECISION_MIXED;
   if (fp32 && !s->have_fp32) return fail(SNAP_ERR_INVALID, "solver was created without mixed precision");
   if (precision < 0 || precision > 2) return fail(SNAP_ERR_INVALID, "unknown precision %d", precision);
-  const int np = s->sweep_plan(precision).nplanes;
+  const int np = s->sweep_plan(precision).nplanes * s->sweep_streams();
   std::vector<hipEvent_t> evs((size_t)2 * np * count);
   for (auto &x : evs) SNAP_HIP_TRY(hipEventCreate(&x));
   SNAP_TRY(s->sweep(precision, nullptr));  // first launches load the code object

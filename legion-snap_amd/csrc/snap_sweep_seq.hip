@@ -87,7 +87,8 @@ struct SeqArgs {
   Extent3 ext;
   int TZ, nTx, nTy, nTz, ntiles;
   int nang, cmom, ng_job, corner0, fixup;
-  int rs;               // LDS row stride (doubles) of the per-plane [cell][angle] tables
+  int s0;               // sweep of blockIdx.y = 0 (corner = corner0 + s / ng_job)
+  int rs;               // LDS row stride (elements) of the per-plane [cell][angle] tables
   double hi, hj, hk;
   const double *mu, *eta, *xi, *w, *ec;
   const double *q;      // [ng_job][cells][cmom]
@@ -148,7 +149,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   const int NB = (nang + 15) >> 4;  // 16-angle blocks of the source product
   const int NST = (nang + 3) >> 2;  // four-angle steps of the moment product
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int s = blockIdx.y;
+  const int s = A.s0 + blockIdx.y;
   const int corner = A.corner0 + s / A.ng_job, g = s % A.ng_job;
   // the sweep's operand tables, shared by the workgroup's wavefronts:
   // EB[nb][kk][lane] = ec[4kk + lane/16][16nb + lane%16] (the source
@@ -821,6 +822,14 @@ static SeqKernel seq_kernel_batch(int cmom) {
   }
 }
 
+int sweep_seq_streams(const SweepLaunch &L) {
+  const int ncorner = L.ng_job > 0 ? L.nsweeps / L.ng_job : 0;
+  return (L.side && L.fork && L.join && !L.batch && ncorner * L.ng_job == L.nsweeps &&
+          ncorner % 2 == 0)
+             ? 2
+             : 1;
+}
+
 int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
   const SweepPlan &p = *L.plan;
   if (!p.seq) return fail(SNAP_ERR_INVALID, "not a one-wavefront tile plan");
@@ -880,13 +889,34 @@ int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) 
   SNAP_TRY(ensure_dynamic_lds((const void *)k, lds));
   const int nplanes = L.batch ? L.batch->nplanes : p.nplanes;
   const std::vector<int> &poff = L.batch ? L.batch->plane_off : p.plane_off;
+  // Two streams, when the caller gives one: each takes half of the corners
+  // (sweeps s0 .. s0 + part - 1, their own faces and partials), and the
+  // launches alternate between them, so that the drain of one stream's
+  // launch overlaps the other's; measured 5% off the 64^3 FP64 solve (more
+  // streams were slower).  The side stream starts after, and the caller's
+  // stream resumes after, everything enqueued on the other.
+  const int nstr = sweep_seq_streams(L);
+  const int part = L.nsweeps / nstr;
+  const hipStream_t ss[2] = {st, nstr > 1 ? L.side : st};
+  if (nstr > 1) {
+    SNAP_HIP_TRY(hipEventRecord(L.fork, st));
+    SNAP_HIP_TRY(hipStreamWaitEvent(L.side, L.fork, 0));
+  }
   for (int D = 0; D < nplanes; D++) {
     const int count = poff[D + 1] - poff[D];
     if (count == 0) continue;
-    if (events) SNAP_HIP_TRY(hipEventRecord(events[2 * D], st));
-    hipLaunchKernelGGL(k, dim3((count + kWGW - 1) / kWGW, L.nsweeps), dim3(64 * kWGW), lds, st, A,
-                       poff[D], count);
-    if (events) SNAP_HIP_TRY(hipEventRecord(events[2 * D + 1], st));
+    for (int q = 0; q < nstr; q++) {
+      A.s0 = q * part;
+      hipEvent_t *ev = events ? events + 2 * ((size_t)D * nstr + q) : nullptr;
+      if (ev) SNAP_HIP_TRY(hipEventRecord(ev[0], ss[q]));
+      hipLaunchKernelGGL(k, dim3((count + kWGW - 1) / kWGW, part), dim3(64 * kWGW), lds, ss[q], A,
+                         poff[D], count);
+      if (ev) SNAP_HIP_TRY(hipEventRecord(ev[1], ss[q]));
+    }
+  }
+  if (nstr > 1) {
+    SNAP_HIP_TRY(hipEventRecord(L.join, L.side));
+    SNAP_HIP_TRY(hipStreamWaitEvent(st, L.join, 0));
   }
   SNAP_HIP_TRY(hipGetLastError());
   return SNAP_OK;
