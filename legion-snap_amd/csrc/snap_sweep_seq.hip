@@ -45,8 +45,11 @@ constexpr int kSX = 4, kSY = 4, kNP = kSX * kSY;  // tile cross-section, cells p
 #define SEQ_WAVES 2  // waves per SIMD the register allocation aims for
 #endif
 // Wavefronts per workgroup: each takes its own tile of the same sweep, so
-// they share the sweep's matrix-core operand tables in LDS.
-constexpr int kWGW = 4;
+// they share the sweep's matrix-core operand tables in LDS.  Two, not four:
+// a workgroup's LDS is freed only when its slowest tile (the one with the
+// most fixup re-sweeps) ends, and pairs hold less of it idle (the 64^3
+// FP64 solve 3% faster); one per workgroup repeats the tables too often.
+constexpr int kWGW = 2;
 
 // Per-wavefront LDS, in bytes, for row stride rs and a sweep type of rsz
 // bytes: W (the moment product's cell operand, 16 rows plus a junk row of 64
