@@ -1856,6 +1856,13 @@ int dropin_launch(const snap_sweep_args_t &a, const SweepPlan *plan, const QuadD
   L.faces = faces;
   L.fp32 = false;
   L.exact = exact;
+  if (plan->seq) {
+    // the fused steady form on the one-wavefront tiles (dinv formed in the
+    // kernel from t_xs, as CalculateGeometryParam forms the caller's dinv)
+    if (exact || td) return fail(SNAP_ERR_INVALID, "one-wavefront drop-in plans are fused and steady");
+    L.recompute_dinv = true;
+    return sweep_seq_enqueue(L, st, nullptr);
+  }
   // time-dependent terms and the MMS source (gpu_sweep.cu:2102-2140 picks the
   // time-dependent kernel when vdelt != 0)
   if (td) {
@@ -1912,7 +1919,8 @@ int dropin_run(const snap_sweep_args_t &a, snap_stream_t stream, int ngroups,
   int dev;
   SNAP_TRY(current_device(&dev));
   const Extent3 e{a.x_range, a.y_range, a.z_range};
-  // plan per chunk shape, one for steady and one for time-dependent sweeps
+  // plan per chunk shape, one for steady and one for time-dependent sweeps;
+  // fused steady sweeps of up to 64 angles take the one-wavefront tiles
   const bool td = a.vdelt != 0.0 || a.mms_source;
   hipStream_t st = (hipStream_t)stream;
   SweepPlan *plan = nullptr;
@@ -1929,14 +1937,18 @@ int dropin_run(const snap_sweep_args_t &a, snap_stream_t stream, int ngroups,
     if (a.num_angles != c.quad.nang || a.num_moments != c.quad.nmom)
       return fail(SNAP_ERR_INVALID, "angles/moments differ from the context");
     quad = &c.quad;
+    const bool seq = !exact && !td && c.quad.nang <= 64;
     for (auto &pp : c.plans)
-      if (pp.first.nx == e.nx && pp.first.ny == e.ny && pp.first.nz == e.nz && pp.second->td == td)
+      if (pp.first.nx == e.nx && pp.first.ny == e.ny && pp.first.nz == e.nz &&
+          pp.second->td == td && pp.second->seq == seq)
         plan = pp.second;
     if (!plan) {
       int TX = 0, TY = 0, TZ = 0, waves = 0, apl = 0;
       plan = new SweepPlan();
       int rc;
-      if (c.quad.nang > 128) {  // past two angles per lane of a wavefront
+      if (seq) {
+        rc = plan_create_seq(plan, e, c.quad.nang, c.quad.cmom, 8);
+      } else if (c.quad.nang > 128) {  // past two angles per lane of a wavefront
         rc = plan_create_wide(plan, e, c.quad.nang, c.quad.cmom, td);
       } else {
         plan_choose(e, c.quad.nang, c.quad.cmom, 8, &TX, &TY, &TZ, &waves, &apl, td);

@@ -109,6 +109,11 @@ struct SeqArgs {
   const int *active, *done, *prec;
   int prec_want;
   int64_t total_cells;
+  // the drop-in sweep of one chunk (GEN kernels): accessors of the source
+  // moments, t_xs and the fluxes the sweep adds to (chunk-local cells,
+  // group g at g * sg), and the ghost faces (base NULL = vacuum)
+  Acc3 gq, gtxs, gflux, gfluxm;
+  Acc2 ggx, ggy, ggz;
 };
 
 __device__ inline void wave_sync() {
@@ -123,14 +128,23 @@ __device__ inline void wave_sync() {
 struct TileGeom {
   int x0, y0, z0, ex, ey, ez, nx, ny, nz;
   int c0, sx, sy, sz;
+  int px0, py0, pz0, dx, dy, dz;  // chunk coordinates of tile cell (0,0,0), steps
   __device__ inline void init(int corner) {
     const bool sxp = (corner & 1) != 0, syp = (corner & 2) != 0, szp = (corner & 4) != 0;
-    const int px = sxp ? x0 : nx - 1 - x0, py = syp ? y0 : ny - 1 - y0, pz = szp ? z0 : nz - 1 - z0;
-    c0 = (pz * ny + py) * nx + px;
-    sx = sxp ? 1 : -1;
-    sy = syp ? nx : -nx;
-    sz = szp ? nx * ny : -nx * ny;
+    px0 = sxp ? x0 : nx - 1 - x0;
+    py0 = syp ? y0 : ny - 1 - y0;
+    pz0 = szp ? z0 : nz - 1 - z0;
+    dx = sxp ? 1 : -1;
+    dy = syp ? 1 : -1;
+    dz = szp ? 1 : -1;
+    c0 = (pz0 * ny + py0) * nx + px0;
+    sx = dx;
+    sy = dy * nx;
+    sz = dz * nx * ny;
   }
+  __device__ inline int px(int i) const { return px0 + dx * i; }
+  __device__ inline int py(int j) const { return py0 + dy * j; }
+  __device__ inline int pz(int k) const { return pz0 + dz * k; }
   __device__ inline int cell(int i, int j, int k) const {
     if (i >= ex || j >= ey || k >= ez) return -1;
     return c0 + i * sx + j * sy + k * sz;
@@ -140,11 +154,16 @@ struct TileGeom {
 }  // namespace
 
 // CMOM: flux moments (1, 4, 9 or 16), a compile-time count so that every loop
-// over moments unrolls with register operands at fixed places.
-template <typename Real, bool EXACT, bool REC, int CMOM, bool BATCH = false>
+// over moments unrolls with register operands at fixed places.  GEN: the
+// drop-in form (fused FP64, dinv formed in the kernel): sources and t_xs
+// through accessors, ghost faces at the chunk's sides, and the moments added
+// to the caller's fluxes by device atomics (the caller's other corner tasks
+// add to the same cells concurrently).
+template <typename Real, bool EXACT, bool REC, int CMOM, bool BATCH = false, bool GEN = false>
 __global__ void __launch_bounds__(64 * kWGW) __attribute__((amdgpu_waves_per_eu(SEQ_WAVES, 8)))
 k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   constexpr bool kFP64 = std::is_same<Real, double>::value;
+  static_assert(!GEN || (kFP64 && !EXACT && REC && !BATCH), "GEN is the fused FP64 drop-in form");
   constexpr int KS = (CMOM + 3) / 4;  // four-moment steps of the source product
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int RS = A.rs;
@@ -157,8 +176,8 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   // the sweep's operand tables, shared by the workgroup's wavefronts:
   // EB[nb][kk][lane] = ec[4kk + lane/16][16nb + lane%16] (the source
   // product's B, ec[0] = 1) and EA[st][lane] = ec[lane%16][4st + lane/16]
-  // (the moment product's A, times the angle's weight unless EXACT)
-  // (the moment product's A), zero outside the moments and angles
+  // (the moment product's A, times the angle's weight unless EXACT), zero
+  // outside the moments and angles
   double *EB = (double *)smem_raw;
   double *EA = EB + (size_t)NB * KS * 64;
   {
@@ -242,9 +261,13 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   Real *out_x = faces_s + (size_t)(tz * nTy + ty) * fxn;
   Real *out_y = faces_s + nfx + (size_t)(tz * nTx + tx) * fyn;
   Real *out_z = faces_s + nfx + nfy + (size_t)(ty * nTx + tx) * fzn;
-  const Real *in_x = tx > 0 ? out_x : nullptr;
-  const Real *in_y = ty > 0 ? out_y : nullptr;
-  const Real *in_z = tz > 0 ? out_z : nullptr;
+  // (GEN: with a ghost array on the chunk's upstream side, the tiles on that
+  // side read their slot too, which the ghost staging filled before the
+  // sweep; the last tiles' outflows stay in their slots for the staging
+  // back to the ghost arrays)
+  const Real *in_x = (tx > 0 || (GEN && A.ggx.base)) ? out_x : nullptr;
+  const Real *in_y = (ty > 0 || (GEN && A.ggy.base)) ? out_y : nullptr;
+  const Real *in_z = (tz > 0 || (GEN && A.ggz.base)) ? out_z : nullptr;
 
   const double *qg = A.q + ((size_t)g * gstride + coff) * cmom;
   const Real *dg = (const Real *)A.dinv + ((size_t)g * gstride + coff) * nang;
@@ -257,6 +280,10 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   // with zero weight and never store)
   const bool aok = lane < nang;
   const int a = aok ? lane : 0;
+  // GEN: a cell's record in an accessor
+  auto acc_cell = [&](const Acc3 &X, int i, int j, int k) -> const char * {
+    return X.at(T.px(i), T.py(j), T.pz(k), g);
+  };
   const double mu_d = A.mu[a], eta_d = A.eta[a], xi_d = A.xi[a];
   const double w_d = aok ? A.w[a] : 0.0;
   // EXACT keeps (psi*mu)*hi as the reference writes it; the others fold the
@@ -438,10 +465,15 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
         for (int r = 0; r < 4; r++) {
           const int l = 4 * r + q4;
           if (l >= cmom) break;
-          if (l == 0)
+          if constexpr (GEN) {
+            double *dst = l == 0 ? (double *)acc_cell(A.gflux, n & 3, n >> 2, k)
+                                 : (double *)acc_cell(A.gfluxm, n & 3, n >> 2, k) + (l - 1);
+            atomicAdd(dst, acc[r]);
+          } else if (l == 0) {
             pfg[c] = acc[r];
-          else
+          } else {
             pmg[(size_t)c * (cmom - 1) + (l - 1)] = acc[r];
+          }
         }
       }
     }
@@ -486,14 +518,20 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
 #pragma unroll
       for (int kk = 0; kk < KS; kk++) {
         const int l = 4 * kk + q4;
-        qa[kk] = (l < CMOM) ? qg[(size_t)c * cmom + l] : 0.0;
+        if constexpr (GEN)
+          qa[kk] = (l < CMOM) ? ((const double *)acc_cell(A.gq, r16 & 3, r16 >> 2, k))[l] : 0.0;
+        else
+          qa[kk] = (l < CMOM) ? qg[(size_t)c * cmom + l] : 0.0;
       }
     };
     // t_xs of slot L%16 in lane L (read by the cells with a lane broadcast)
     Real tv = Real(0);
     auto tv_load = [&](int k) {
       const int c = T.c0 + (r16 & 3) * T.sx + (r16 >> 2) * T.sy + k * T.sz;
-      tv = tg[c];
+      if constexpr (GEN)
+        tv = *(const Real *)acc_cell(A.gtxs, r16 & 3, r16 >> 2, k);
+      else
+        tv = tg[c];
     };
     Real dv[kNP];
     auto dv_load = [&](int k) {
@@ -667,7 +705,10 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
         const int idx = lane + 64 * r;  // = slot * 16 + l
         const int slot = idx >> 4, l = idx & 15;
         const int c = (k < T.ez && l < cmom) ? T.cell(slot & 3, slot >> 2, k) : -1;
-        qn[r] = c >= 0 ? qg[(size_t)c * cmom + l] : 0.0;
+        if constexpr (GEN)
+          qn[r] = c >= 0 ? ((const double *)acc_cell(A.gq, slot & 3, slot >> 2, k))[l] : 0.0;
+        else
+          qn[r] = c >= 0 ? qg[(size_t)c * cmom + l] : 0.0;
       }
     };
     auto q_store = [&]() {
@@ -706,7 +747,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
           }
           psi = CMOM > 2 ? p0 + p1 : p0;
         }
-        const Real txs = tg[c];
+        const Real txs = GEN ? *(const Real *)acc_cell(A.gtxs, i, j, k) : tg[c];
         const Real d = REC ? dinv_of(txs) : dg[(size_t)c * nang + a];
         const Real pi_ = (i == 0) ? xin[j] : xf[j];
         const Real pj_ = (j == 0) ? yin[i] : yf[i];
@@ -736,9 +777,56 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
 #pragma unroll
     for (int n = 0; n < kNP; n++) {
       const int i = n & 3, j = n >> 2;
-      if (i < T.ex && j < T.ey) out_z[(size_t)(j * kSX + i) * nang + a] = zf[n];
+      if (i < T.ex && j < T.ey) {
+        out_z[(size_t)(j * kSX + i) * nang + a] = zf[n];
+      }
     }
   }
+}
+
+// Ghost faces of the drop-in form, through the pencil slots: before the
+// sweep an upstream ghost array fills the slots its side's tiles read, and
+// after it the downstream side's slots, which then hold the last tiles'
+// outflows, go back to the ghost array -- read at the upstream face, the
+// outflow written over it, as the reference keeps its ghost regions
+// (sweep.cc:536-566).  One thread per (face point, angle) of sweep
+// blockIdx.y; axis 0: the x faces over (y, z), 1: y over (x, z), 2: z over
+// (x, y), in sweep order (p, q).
+__global__ void k_seq_ghosts(SeqArgs A, int axis, int to_ghost) {
+  const int nang = A.nang, TZ = A.TZ;
+  const Extent3 e = A.ext;
+  const int np = axis == 0 ? e.ny : e.nx;
+  const int nq = axis == 2 ? e.ny : e.nz;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)np * nq * nang) return;
+  const int a = (int)(idx % nang);
+  const int pq = (int)(idx / nang), p = pq % np, q = pq / np;
+  const int s = A.s0 + blockIdx.y;
+  const int corner = A.corner0 + s / A.ng_job, g = s % A.ng_job;
+  const bool sxp = (corner & 1) != 0, syp = (corner & 2) != 0, szp = (corner & 4) != 0;
+  const size_t fxn = (size_t)kSY * TZ * nang, fyn = (size_t)kSX * TZ * nang,
+               fzn = (size_t)kSX * kSY * nang;
+  const size_t nfx = (size_t)A.nTy * A.nTz * fxn, nfy = (size_t)A.nTx * A.nTz * fyn,
+               nfz = (size_t)A.nTx * A.nTy * fzn;
+  double *faces_s = (double *)A.faces + (size_t)s * (nfx + nfy + nfz);
+  double *slot, *gp;
+  if (axis == 0) {  // (y', z')
+    const int ty = p / kSY, j = p % kSY, tz = q / TZ, k = q % TZ;
+    slot = faces_s + (size_t)(tz * A.nTy + ty) * fxn + (size_t)(k * kSY + j) * nang + a;
+    gp = (double *)A.ggx.at(syp ? p : e.ny - 1 - p, szp ? q : e.nz - 1 - q, g) + a;
+  } else if (axis == 1) {  // (x', z')
+    const int tx = p / kSX, i = p % kSX, tz = q / TZ, k = q % TZ;
+    slot = faces_s + nfx + (size_t)(tz * A.nTx + tx) * fyn + (size_t)(k * kSX + i) * nang + a;
+    gp = (double *)A.ggy.at(sxp ? p : e.nx - 1 - p, szp ? q : e.nz - 1 - q, g) + a;
+  } else {  // (x', y')
+    const int tx = p / kSX, i = p % kSX, ty = q / kSY, j = q % kSY;
+    slot = faces_s + nfx + nfy + (size_t)(ty * A.nTx + tx) * fzn + (size_t)(j * kSX + i) * nang + a;
+    gp = (double *)A.ggz.at(sxp ? p : e.nx - 1 - p, syp ? q : e.ny - 1 - q, g) + a;
+  }
+  if (to_ghost)
+    *gp = *slot;
+  else
+    *slot = *gp;
 }
 
 // ---------------------------------------------------------------------------
@@ -813,6 +901,17 @@ static SeqKernel seq_kernel(int cmom, bool rec) {
   }
 }
 
+// Drop-in kernels: fused FP64 through accessors, dinv formed in the kernel.
+static SeqKernel seq_kernel_gen(int cmom) {
+  switch (cmom) {
+    case 1: return k_sweep_seq<double, false, true, 1, false, true>;
+    case 4: return k_sweep_seq<double, false, true, 4, false, true>;
+    case 9: return k_sweep_seq<double, false, true, 9, false, true>;
+    case 16: return k_sweep_seq<double, false, true, 16, false, true>;
+    default: return nullptr;
+  }
+}
+
 // Batch kernels: fused FP64 or FP32, dinv formed in the kernel.
 template <typename Real>
 static SeqKernel seq_kernel_batch(int cmom) {
@@ -836,13 +935,19 @@ int sweep_seq_streams(const SweepLaunch &L) {
 int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
   const SweepPlan &p = *L.plan;
   if (!p.seq) return fail(SNAP_ERR_INVALID, "not a one-wavefront tile plan");
-  if (L.general || L.td)
-    return fail(SNAP_ERR_INVALID, "the one-wavefront tile sweep is dense and steady");
+  if (L.td) return fail(SNAP_ERR_INVALID, "the one-wavefront tile sweep is steady");
   if (L.batch && (L.exact || !L.recompute_dinv))
     return fail(SNAP_ERR_INVALID, "one-wavefront batch sweeps are fused and form dinv in the kernel");
-  if (!L.q || !L.txs_dense || !L.part_flux || (p.cmom > 1 && !L.part_fluxm) ||
-      (!L.recompute_dinv && !L.dinv_dense))
+  if (L.general) {
+    if (L.exact || L.fp32 || L.batch || !L.recompute_dinv)
+      return fail(SNAP_ERR_INVALID,
+                  "the one-wavefront drop-in sweep is fused FP64 and forms dinv in the kernel");
+    if (!L.qtot.base || !L.txs.base || !L.flux.base || (p.cmom > 1 && !L.fluxm.base))
+      return fail(SNAP_ERR_INVALID, "drop-in sweep accessors missing");
+  } else if (!L.q || !L.txs_dense || !L.part_flux || (p.cmom > 1 && !L.part_fluxm) ||
+             (!L.recompute_dinv && !L.dinv_dense)) {
     return fail(SNAP_ERR_INVALID, "dense sweep inputs missing");
+  }
   if (L.nsweeps < 1 || L.nsweeps > 65535) return fail(SNAP_ERR_INVALID, "bad sweep count");
   SeqArgs A;
   memset(&A, 0, sizeof(A));
@@ -873,6 +978,15 @@ int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) 
   A.part_fluxm = L.part_fluxm;
   A.faces = L.faces;
   A.plane_tiles = p.plane_tiles;
+  if (L.general) {
+    A.gq = L.qtot;
+    A.gtxs = L.txs;
+    A.gflux = L.flux;
+    A.gfluxm = L.fluxm;
+    A.ggx = L.gx;
+    A.ggy = L.gy;
+    A.ggz = L.gz;
+  }
   if (L.batch) {
     A.batch_tiles = L.batch->batch_tiles;
     A.descs = L.batch->descs;
@@ -882,7 +996,8 @@ int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) 
     A.prec_want = L.batch->prec_want;
     A.total_cells = L.batch->total_cells;
   }
-  const SeqKernel k = L.batch ? (L.fp32 ? seq_kernel_batch<float>(p.cmom)
+  const SeqKernel k = L.general ? seq_kernel_gen(p.cmom)
+                   : L.batch  ? (L.fp32 ? seq_kernel_batch<float>(p.cmom)
                                         : seq_kernel_batch<double>(p.cmom))
                    : L.fp32 ? seq_kernel<float, false>(p.cmom, L.recompute_dinv)
                    : L.exact ? seq_kernel<double, true>(p.cmom, L.recompute_dinv)
@@ -901,6 +1016,20 @@ int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) 
   const int nstr = sweep_seq_streams(L);
   const int part = L.nsweeps / nstr;
   const hipStream_t ss[2] = {st, nstr > 1 ? L.side : st};
+  // the drop-in form's ghost faces (the drop-in runs on one stream)
+  const Acc2 *ghosts[3] = {&L.gx, &L.gy, &L.gz};
+  auto stage_ghosts = [&](int to_ghost) {
+    if (!L.general) return;
+    A.s0 = 0;
+    for (int axis = 0; axis < 3; axis++) {
+      if (!ghosts[axis]->base) continue;
+      const int np = axis == 0 ? p.ext.ny : p.ext.nx, nq = axis == 2 ? p.ext.ny : p.ext.nz;
+      const int64_t n = (int64_t)np * nq * p.nang;
+      hipLaunchKernelGGL(k_seq_ghosts, dim3((unsigned)((n + 255) / 256), L.nsweeps), dim3(256), 0,
+                         st, A, axis, to_ghost);
+    }
+  };
+  stage_ghosts(0);
   if (nstr > 1) {
     SNAP_HIP_TRY(hipEventRecord(L.fork, st));
     SNAP_HIP_TRY(hipStreamWaitEvent(L.side, L.fork, 0));
@@ -921,6 +1050,7 @@ int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) 
     SNAP_HIP_TRY(hipEventRecord(L.join, L.side));
     SNAP_HIP_TRY(hipStreamWaitEvent(st, L.join, 0));
   }
+  stage_ghosts(1);
   SNAP_HIP_TRY(hipGetLastError());
   return SNAP_OK;
 }

@@ -249,3 +249,48 @@ def test_fixup_count_is_cell_global(gpu, oracle_mod, w):
     assert flux.cpu().numpy()[0, 0, 0, 0] == pf[0, 0, 0]
     for k in range(3):
         assert np.array_equal(gd[k][0].cpu().numpy(), outs[k]), k
+
+
+@pytest.mark.parametrize("with_ghosts", [False, True])
+def test_group_range_fused_sub_chunk(setup, with_ghosts):
+    # the fused group range (one-wavefront tiles, accessors and ghost faces
+    # staged through the tile faces) on a chunk inside larger arrays -- the
+    # whole arrays' strides, an origin, three tiles along every axis (the
+    # last one ragged) -- against run_gpu_sweep once per group
+    S = setup
+    torch, T = S["torch"], S["T"]
+    _, _, _, ng, nang, nmom, cmom = S["dims"]
+    NX, NY, NZ = 12, 11, 21
+    origin, ext = (1, 2, 1), (10, 9, 19)
+    tasks.initialize_gpu_context(S["quad"], nang, nmom, (NX, NY, NZ))  # other tests re-initialize
+    rng = np.random.default_rng(2024)
+    txs = rng.uniform(0.5, 3.0, (ng, NZ, NY, NX))
+    dinv = T(_dinv_of(S, txs))
+    qtot = rng.uniform(0, 1, (ng, NZ, NY, NX, cmom))
+    qtot[..., 1:] *= 0.2
+    qtot_d, txs_d = T(qtot), T(txs)
+    z = lambda *shape: torch.zeros(shape, dtype=torch.float64, device="cuda")  # noqa: E731
+    inside = np.zeros((ng, NZ, NY, NX), dtype=bool)
+    inside[:, origin[2]:origin[2] + ext[2], origin[1]:origin[1] + ext[1],
+           origin[0]:origin[0] + ext[0]] = True
+    for corner in (0, 3, 5, 6):
+        fa, fb = z(ng, NZ, NY, NX), z(ng, NZ, NY, NX)
+        ma, mb = z(ng, NZ, NY, NX, cmom - 1), z(ng, NZ, NY, NX, cmom - 1)
+        ga = gb = None
+        if with_ghosts:
+            gin = (rng.uniform(0, 1, (ng, NZ, NY, nang)), rng.uniform(0, 1, (ng, NZ, NX, nang)),
+                   rng.uniform(0, 1, (ng, NY, NX, nang)))
+            ga, gb = tuple(T(x) for x in gin), tuple(T(x) for x in gin)
+        kw = dict(origin=origin, extent=ext)
+        tasks.mini_kba(corner, range(ng), qtot_d, fa, ma, dinv, txs_d, ga, S["h"], nang, nmom, **kw)
+        tasks.mini_kba(corner, range(ng), qtot_d, fb, mb, dinv, txs_d, gb, S["h"], nang, nmom,
+                       group_range="fused", **kw)
+        torch.cuda.synchronize()
+        A, B = fa.cpu().numpy(), fb.cpu().numpy()
+        AM, BM = ma.cpu().numpy(), mb.cpu().numpy()
+        assert np.all(A[inside] > 0) and not np.any(B[~inside]), corner
+        assert np.max(np.abs(B[inside] - A[inside]) / A[inside]) <= 1e-13, corner
+        assert np.max(np.abs(BM - AM)) <= 1e-13 * np.max(np.abs(AM)), corner
+        if with_ghosts:
+            for k in range(3):
+                assert torch.allclose(ga[k], gb[k], rtol=1e-12, atol=1e-15), (corner, k)
