@@ -127,8 +127,10 @@ int snap_hip_run_gpu_sweep(const snap_sweep_args_t *args, snap_stream_t stream);
  * SOA instance are) -- else SNAP_ERR_UNSUPPORTED, and the caller sweeps group
  * by group.  Steady sweeps only (vdelt = 0, no MMS source).  precision:
  * SNAP_PRECISION_FP64_EXACT (results bit-equal to one run_gpu_sweep per
- * group) or SNAP_PRECISION_FP64 (fused multiply-adds).  All groups' sweeps of
- * the range share every launch, as the solver's do. */
+ * group) or SNAP_PRECISION_FP64 (fused multiply-adds; up to 64 angles on the
+ * one-wavefront tile kernel, which forms dinv from t_xs and does not read the
+ * dinv accessor).  All groups' sweeps of the range share every launch, as
+ * the solver's do. */
 int snap_hip_run_gpu_sweep_groups(const snap_sweep_args_t *groups, int num_groups, int precision,
                                   snap_stream_t stream);
 
