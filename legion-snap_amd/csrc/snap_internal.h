@@ -128,13 +128,16 @@ int launch_outer_source_m(Acc3 fluxm, Acc2 slgg, Acc3 qom, Acc3 mat, Extent3 e, 
                           int cmom, const int *lma, int nmom, hipStream_t st);
 // CalcInnerSource over dense [g][cell][...] arrays (cg = ng * cells
 // cell-groups, the Legendre moment counts 1, 3, 5, 7)
+// Kernels of the solver's inner iteration take `skip`: when it is given and
+// *skip != 0 (the previous inner test passed; the host enqueues one inner
+// iteration ahead of its reading of the test) every block returns at once.
 int launch_inner_source_dense(const double *sxs, const double *flux0, const double *q2grp0,
                               const double *fluxm, const double *q2grpm, double *qtot, int64_t cg,
-                              int nmom, hipStream_t st);
+                              int nmom, hipStream_t st, const int32_t *skip = nullptr);
 int launch_inner_source(Acc3 sxs, Acc3 flux0, Acc3 q2grp0, Acc3 fluxm, Acc3 q2grpm, Acc3 qtot,
                         Extent3 e, int ng, int cmom, const int *lma, int nmom, hipStream_t st);
 int launch_convergence(Acc3 flux0, Acc3 flux0p, Extent3 e, int ng, double epsi, int32_t *result,
-                       hipStream_t st);
+                       hipStream_t st, const int32_t *skip = nullptr);
 // Outer source of the solver's dense layout: ordered = the CPU's summation
 // order (one thread per cell and moment), else FP64 matrix cores.
 int launch_outer_dense(const double *qi, const double *flux0, const double *fluxm,
@@ -150,13 +153,16 @@ int launch_adjoint_update(const double *r, const double *u, const double *slgg,
 int launch_to_float(const double *a, float *b, int64_t n, hipStream_t st);
 // out = (base ? base : 0) + sum over the 8 octant partials, octants in order
 int launch_octant_reduce(const double *part, double *out, int64_t n, hipStream_t st,
-                         const double *base = nullptr);
+                         const double *base = nullptr, const int32_t *skip = nullptr);
+// dst = src over n doubles
+int launch_copy(double *dst, const double *src, int64_t n, hipStream_t st,
+                const int32_t *skip = nullptr);
 // *out = bit pattern of max |f/fp - 1| over n values (|f| where |fp| < 1e-12)
 int launch_max_change(const double *f, const double *fp, int64_t n, unsigned long long *out,
                       hipStream_t st);
 // y -= x
 int launch_subtract(double *y, const double *x, int64_t n, hipStream_t st);
-int launch_fill_int(int32_t *p, int32_t v, hipStream_t st);
+int launch_fill_int(int32_t *p, int32_t v, hipStream_t st, const int32_t *skip = nullptr);
 
 // ---------------------------------------------------------------------------
 // tiled wavefront sweep (snap_sweep.hip)
@@ -269,6 +275,7 @@ struct SweepLaunch {
   // tail of each launch on one stream overlaps work on the other
   hipStream_t side;
   hipEvent_t fork, join;
+  const int32_t *skip;         // solver inner iterations enqueued ahead (see launch_inner_source_dense)
   // time-dependent terms (td): vdelt per job group (dense) or vdelt0 (general);
   // time-edge fluxes in / out and the MMS angular source (NULL = none), dense
   // [8][ng_job][cells][nang] doubles or per-cell accessors of nang doubles

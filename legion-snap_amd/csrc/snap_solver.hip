@@ -72,6 +72,12 @@ struct snap_solver {
   // the one-wavefront tile sweep's second stream and its fork / join events
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // inner iterations enqueued one ahead (FP64 solves): two slots, each with
+  // the test's device flag, its pinned host copy, and events around the
+  // sweep and after the copy
+  DeviceBuffer ra_flag;            // [2] int32
+  int32_t *h_ra = nullptr;         // pinned [2]
+  hipEvent_t ra_ev[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int32_t *h_flag = nullptr;  // pinned: [0] the test's flag, [2..3] max change bits
   DeviceBuffer mat, sigt, slgg, qi, txs, txs32, sxs, dinv, dinv32;
@@ -97,6 +103,10 @@ struct snap_solver {
     quad_free(&quad);
     for (auto &x : ev)
       if (x) (void)hipEventDestroy(x);
+    for (auto &sl : ra_ev)
+      for (auto &x : sl)
+        if (x) (void)hipEventDestroy(x);
+    if (h_ra) (void)hipHostFree(h_ra);
     if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (join_ev) (void)hipEventDestroy(join_ev);
     if (h_flag) (void)hipHostFree(h_flag);
@@ -125,7 +135,7 @@ struct snap_solver {
   // tiled_only: the multi-wave tile plan even when the one-wavefront plan
   // exists (the adjoint's checkpoint faces are in that plan's layout)
   int sweep(int precision, hipEvent_t *events, bool fixup = true, bool recompute = false,
-            bool tiled_only = false) {
+            bool tiled_only = false, const int32_t *skip = nullptr) {
     const bool fp32 = precision == SNAP_PRECISION_MIXED;
     SweepLaunch L;
     memset(&L, 0, sizeof(L));
@@ -135,6 +145,7 @@ struct snap_solver {
     L.ng_job = ng;
     L.corner0 = 0;
     L.fixup = (prob.fixup && fixup) ? 1 : 0;
+    L.skip = skip;
     L.hi = hi;
     L.hj = hj;
     L.hk = hk;
@@ -172,13 +183,13 @@ struct snap_solver {
   }
 
   // flux = sum of the octant partials (+ the base fluxes of a correction sweep)
-  int reduce_octants(bool add_base = false) {
+  int reduce_octants(bool add_base = false, const int32_t *skip = nullptr) {
     SNAP_TRY(launch_octant_reduce(part_flux.as<double>(), flux0.as<double>(), (int64_t)ng * cells,
-                                  stream, add_base ? flux0base.as<double>() : nullptr));
+                                  stream, add_base ? flux0base.as<double>() : nullptr, skip));
     if (cmom > 1)
       SNAP_TRY(launch_octant_reduce(part_fluxm.as<double>(), fluxm.as<double>(),
                                     (int64_t)ng * cells * (cmom - 1), stream,
-                                    add_base ? fluxmbase.as<double>() : nullptr));
+                                    add_base ? fluxmbase.as<double>() : nullptr, skip));
     return SNAP_OK;
   }
 
@@ -193,11 +204,11 @@ struct snap_solver {
                               ordered, stream);
   }
 
-  int inner_source() {
+  int inner_source(const int32_t *skip = nullptr) {
     return launch_inner_source_dense(sxs.as<double>(), flux0.as<double>(), q2grp0.as<double>(),
                                      cmom > 1 ? fluxm.as<double>() : nullptr,
                                      cmom > 1 ? q2grpm.as<double>() : nullptr, qtot.as<double>(),
-                                     (int64_t)ng * cells, nmom, stream);
+                                     (int64_t)ng * cells, nmom, stream, skip);
   }
 
   // change: when not NULL, also the largest relative change of the test's
@@ -313,6 +324,10 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
     SNAP_HIP_TRY(hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming));
   }
   SNAP_HIP_TRY(hipHostMalloc((void **)&s->h_flag, 16, hipHostMallocDefault));
+  SNAP_HIP_TRY(hipHostMalloc((void **)&s->h_ra, 2 * sizeof(int32_t), hipHostMallocDefault));
+  SNAP_TRY(s->ra_flag.alloc(2 * sizeof(int32_t)));
+  for (auto &sl : s->ra_ev)
+    for (auto &x : sl) SNAP_HIP_TRY(hipEventCreate(&x));
 
   const size_t gc = (size_t)s->ng * s->cells;
   const size_t nm1 = s->cmom > 1 ? (size_t)(s->cmom - 1) : 0;
@@ -454,7 +469,48 @@ extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) 
       inner_converged = false;
       // the previous inner's largest relative change, within this outer
       double change = HUGE_VAL;
-      for (int inno = 0; inno < s->prob.iitm; inno++) {
+      if (!mixed) {
+        // FP64: inner iteration i+1 is enqueued before the host reads
+        // iteration i's test, every kernel of it predicated on that test's
+        // device flag (set: the loop has ended, and the iteration does
+        // nothing), so the device does not wait for the host between inner
+        // iterations.  Two slots of flags and events alternate.
+        auto enqueue_inner = [&](int inno, const int32_t *skip) -> int {
+          const int sl = inno & 1;
+          int32_t *conv = s->ra_flag.as<int32_t>() + sl;
+          SNAP_TRY(s->inner_source(skip));
+          SNAP_TRY(launch_copy(s->flux0pi.as<double>(), s->flux0.as<double>(), (int64_t)gc, stream,
+                               skip));
+          SNAP_HIP_TRY(hipEventRecord(s->ra_ev[sl][0], stream));
+          SNAP_TRY(s->sweep(s->opt.precision, nullptr, true, false, false, skip));
+          // the octant sum is part of the sweep: the reference accumulates
+          // the flux moments inside its sweep kernel (gpu_sweep.cu:1680-1733)
+          SNAP_TRY(s->reduce_octants(false, skip));
+          SNAP_HIP_TRY(hipEventRecord(s->ra_ev[sl][1], stream));
+          SNAP_TRY(launch_convergence(s->g_acc(s->flux0, 1, 8), s->g_acc(s->flux0pi, 1, 8), s->e,
+                                      s->ng, s->prob.epsi, conv, stream, skip));
+          SNAP_HIP_TRY(hipMemcpyAsync(s->h_ra + sl, conv, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                      stream));
+          SNAP_HIP_TRY(hipEventRecord(s->ra_ev[sl][2], stream));
+          return SNAP_OK;
+        };
+        SNAP_TRY(enqueue_inner(0, nullptr));
+        for (int inno = 0; inno < s->prob.iitm; inno++) {
+          const int sl = inno & 1;
+          if (inno + 1 < s->prob.iitm)
+            SNAP_TRY(enqueue_inner(inno + 1, s->ra_flag.as<int32_t>() + sl));
+          SNAP_HIP_TRY(hipEventSynchronize(s->ra_ev[sl][2]));
+          inner_converged = s->h_ra[sl] != 0;
+          st.sweeps_fp64++;
+          st.inner_total++;
+          record(0, cy, otno, inno, inner_converged);
+          float ms = 0.f;
+          SNAP_HIP_TRY(hipEventElapsedTime(&ms, s->ra_ev[sl][0], s->ra_ev[sl][1]));
+          sweep_ms += ms;
+          if (inner_converged) break;  // the iteration enqueued after it does nothing
+        }
+      }
+      for (int inno = 0; mixed && inno < s->prob.iitm; inno++) {
         SNAP_TRY(s->inner_source());
         SNAP_TRY(s->copy(s->flux0pi, s->flux0, sizeof(double) * gc));
         const bool refine = mixed && s->opt.refine;

@@ -144,8 +144,9 @@ __global__ void k_inner_source_dense(const double *__restrict__ sxs, const doubl
                                      const double *__restrict__ q2grp0,
                                      const double *__restrict__ fluxm,
                                      const double *__restrict__ q2grpm, double *__restrict__ qtot,
-                                     Index n, int nmom, OrderTable ord) {
+                                     Index n, int nmom, OrderTable ord, const int32_t *skip) {
   constexpr int NM = CMOM - 1;
+  if (skip && *skip) return;
   for (Index t = (Index)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (Index)gridDim.x * blockDim.x) {
     const Index cg = t / CMOM;
@@ -163,19 +164,20 @@ __global__ void k_inner_source_dense(const double *__restrict__ sxs, const doubl
 template <int CMOM>
 static void enqueue_inner_dense(const double *sxs, const double *flux0, const double *q2grp0,
                                 const double *fluxm, const double *q2grpm, double *qtot, int64_t n,
-                                int nmom, const OrderTable &ord, hipStream_t st) {
+                                int nmom, const OrderTable &ord, hipStream_t st,
+                                const int32_t *skip) {
   const dim3 grid(grid_for(n, 256)), block(256);
   if (n <= (int64_t)UINT32_MAX - 65536 * 8 * 256)
     hipLaunchKernelGGL((k_inner_source_dense<CMOM, uint32_t>), grid, block, 0, st, sxs, flux0,
-                       q2grp0, fluxm, q2grpm, qtot, (uint32_t)n, nmom, ord);
+                       q2grp0, fluxm, q2grpm, qtot, (uint32_t)n, nmom, ord, skip);
   else
     hipLaunchKernelGGL((k_inner_source_dense<CMOM, int64_t>), grid, block, 0, st, sxs, flux0,
-                       q2grp0, fluxm, q2grpm, qtot, n, nmom, ord);
+                       q2grp0, fluxm, q2grpm, qtot, n, nmom, ord, skip);
 }
 
 int launch_inner_source_dense(const double *sxs, const double *flux0, const double *q2grp0,
                               const double *fluxm, const double *q2grpm, double *qtot, int64_t cg,
-                              int nmom, hipStream_t st) {
+                              int nmom, hipStream_t st, const int32_t *skip) {
   const int cmom = nmom * nmom;
   if (nmom < 1 || nmom > 4) return fail(SNAP_ERR_INVALID, "nmom %d outside 1..4", nmom);
   if (cmom > 1 && (!fluxm || !q2grpm)) return fail(SNAP_ERR_INVALID, "moment arrays missing");
@@ -184,10 +186,10 @@ int launch_inner_source_dense(const double *sxs, const double *flux0, const doub
   const int64_t n = cg * cmom;
   if (n <= 0) return SNAP_OK;
   switch (cmom) {
-    case 1: enqueue_inner_dense<1>(sxs, flux0, q2grp0, fluxm, q2grpm, qtot, n, nmom, ord, st); break;
-    case 4: enqueue_inner_dense<4>(sxs, flux0, q2grp0, fluxm, q2grpm, qtot, n, nmom, ord, st); break;
-    case 9: enqueue_inner_dense<9>(sxs, flux0, q2grp0, fluxm, q2grpm, qtot, n, nmom, ord, st); break;
-    default: enqueue_inner_dense<16>(sxs, flux0, q2grp0, fluxm, q2grpm, qtot, n, nmom, ord, st); break;
+    case 1: enqueue_inner_dense<1>(sxs, flux0, q2grp0, fluxm, q2grpm, qtot, n, nmom, ord, st, skip); break;
+    case 4: enqueue_inner_dense<4>(sxs, flux0, q2grp0, fluxm, q2grpm, qtot, n, nmom, ord, st, skip); break;
+    case 9: enqueue_inner_dense<9>(sxs, flux0, q2grp0, fluxm, q2grpm, qtot, n, nmom, ord, st, skip); break;
+    default: enqueue_inner_dense<16>(sxs, flux0, q2grp0, fluxm, q2grpm, qtot, n, nmom, ord, st, skip); break;
   }
   SNAP_HIP_TRY(hipGetLastError());
   return SNAP_OK;
@@ -198,7 +200,8 @@ int launch_inner_source_dense(const double *sxs, const double *flux0, const doub
 // a cell fails when |flux0/flux0p - 1| > epsi (|flux0| when |flux0p| < 1e-12).
 // *result must hold 1 on entry; any failing block clears it.
 __global__ void k_convergence(Acc3 flux0, Acc3 flux0p, Extent3 e, int ng, double epsi,
-                              int32_t *result) {
+                              int32_t *result, const int32_t *skip) {
+  if (skip && *skip) return;
   const int64_t cells = e.cells();
   const int64_t n = cells * ng;
   int bad = 0;
@@ -220,22 +223,23 @@ __global__ void k_convergence(Acc3 flux0, Acc3 flux0p, Extent3 e, int ng, double
   if (__syncthreads_or(bad) && threadIdx.x == 0) atomicAnd(result, 0);
 }
 
-__global__ void k_fill_int(int32_t *p, int32_t v) {
+__global__ void k_fill_int(int32_t *p, int32_t v, const int32_t *skip) {
+  if (skip && *skip) return;
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
-int launch_fill_int(int32_t *p, int32_t v, hipStream_t st) {
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, p, v);
+int launch_fill_int(int32_t *p, int32_t v, hipStream_t st, const int32_t *skip) {
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, p, v, skip);
   SNAP_HIP_TRY(hipGetLastError());
   return SNAP_OK;
 }
 
 int launch_convergence(Acc3 flux0, Acc3 flux0p, Extent3 e, int ng, double epsi, int32_t *result,
-                       hipStream_t st) {
-  SNAP_TRY(launch_fill_int(result, 1, st));
+                       hipStream_t st, const int32_t *skip) {
+  SNAP_TRY(launch_fill_int(result, 1, st, skip));
   const int64_t n = e.cells() * ng;
   hipLaunchKernelGGL(k_convergence, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, flux0,
-                     flux0p, e, ng, epsi, result);
+                     flux0p, e, ng, epsi, result, skip);
   SNAP_HIP_TRY(hipGetLastError());
   return SNAP_OK;
 }
@@ -279,7 +283,8 @@ int launch_max_change(const double *f, const double *fp, int64_t n, unsigned lon
 
 // Sum of the eight octant partials in octant order 0..7.
 __global__ void k_octant_reduce(const double *__restrict__ part, double *__restrict__ out,
-                                const double *__restrict__ base, int64_t n) {
+                                const double *__restrict__ base, int64_t n, const int32_t *skip) {
+  if (skip && *skip) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     double acc = 0.0;
@@ -290,9 +295,25 @@ __global__ void k_octant_reduce(const double *__restrict__ part, double *__restr
 }
 
 int launch_octant_reduce(const double *part, double *out, int64_t n, hipStream_t st,
-                         const double *base) {
+                         const double *base, const int32_t *skip) {
   if (n <= 0) return SNAP_OK;
-  hipLaunchKernelGGL(k_octant_reduce, dim3(grid_for(n, 256)), dim3(256), 0, st, part, out, base, n);
+  hipLaunchKernelGGL(k_octant_reduce, dim3(grid_for(n, 256)), dim3(256), 0, st, part, out, base, n,
+                     skip);
+  SNAP_HIP_TRY(hipGetLastError());
+  return SNAP_OK;
+}
+
+__global__ void k_copy_doubles(double *__restrict__ dst, const double *__restrict__ src, int64_t n,
+                               const int32_t *skip) {
+  if (skip && *skip) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+int launch_copy(double *dst, const double *src, int64_t n, hipStream_t st, const int32_t *skip) {
+  if (n <= 0) return SNAP_OK;
+  hipLaunchKernelGGL(k_copy_doubles, dim3(grid_for(n, 256)), dim3(256), 0, st, dst, src, n, skip);
   SNAP_HIP_TRY(hipGetLastError());
   return SNAP_OK;
 }

@@ -408,6 +408,7 @@ struct SweepArgs {
   const double *tin;
   double *tout;
   const double *qim;
+  const int32_t *skip;  // non-NULL and set: every workgroup returns at once
 };
 
 // Arguments of the general variant (the drop-in entry point): Realm affine
@@ -476,6 +477,7 @@ template <typename Real, int CPW, int APL, bool EXACT, bool REC, bool GEN, bool 
           bool TD = false>
 __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs A, SweepGeneral G, int plane_off) {
   constexpr bool kFP64 = std::is_same<Real, double>::value;
+  if (A.skip && *A.skip) return;  // the whole workgroup: an inner iteration enqueued ahead
   constexpr int MS = max_steps(APL);  // cell groups per wave and diagonal
   constexpr int QS = APL;             // source moments held per lane
   constexpr int ECR = ec_row(APL);    // row stride of the ec table
@@ -1163,6 +1165,7 @@ __global__ void __launch_bounds__(max_threads(APL, TD)) k_sweep_tiles(SweepArgs 
 template <typename Real, int APL, bool EXACT, bool GEN, bool TD>
 __global__ void __launch_bounds__(1024) k_sweep_wide(SweepArgs A, SweepGeneral G, int plane_off) {
   constexpr bool kFP64 = std::is_same<Real, double>::value;
+  if (A.skip && *A.skip) return;  // the whole workgroup: an inner iteration enqueued ahead
   extern __shared__ __align__(16) unsigned char smem_raw[];
   int *cnt = (int *)smem_raw;                       // [2][16] per-wave negative counts
   double *part = (double *)(smem_raw + 2 * 16 * 4);  // [16 waves][16 moments]
@@ -1607,6 +1610,7 @@ int sweep_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
   A.part_flux = L.part_flux;
   A.part_fluxm = L.part_fluxm;
   A.faces = L.faces;
+  A.skip = L.skip;
   A.plane_tiles = p.plane_tiles;
   A.cell_off = p.cell_off;
   A.cells = p.cells;

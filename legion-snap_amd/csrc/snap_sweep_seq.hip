@@ -91,6 +91,7 @@ struct SeqArgs {
   int TZ, nTx, nTy, nTz, ntiles;
   int nang, cmom, ng_job, corner0, fixup;
   int s0;               // sweep of blockIdx.y = 0 (corner = corner0 + s / ng_job)
+  const int32_t *skip;  // non-NULL and set: every workgroup returns at once
   int rs;               // LDS row stride (elements) of the per-plane [cell][angle] tables
   double hi, hj, hk;
   const double *mu, *eta, *xi, *w, *ec;
@@ -164,6 +165,7 @@ __global__ void __launch_bounds__(64 * kWGW) __attribute__((amdgpu_waves_per_eu(
 k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   constexpr bool kFP64 = std::is_same<Real, double>::value;
   static_assert(!GEN || (kFP64 && !EXACT && REC && !BATCH), "GEN is the fused FP64 drop-in form");
+  if (A.skip && *A.skip) return;  // the whole workgroup: an inner iteration enqueued ahead
   constexpr int KS = (CMOM + 3) / 4;  // four-moment steps of the source product
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int RS = A.rs;
@@ -978,6 +980,7 @@ int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) 
   A.part_fluxm = L.part_fluxm;
   A.faces = L.faces;
   A.plane_tiles = p.plane_tiles;
+  A.skip = L.skip;
   if (L.general) {
     A.gq = L.qtot;
     A.gtxs = L.txs;
