@@ -209,6 +209,10 @@ struct snap_batch {
   SweepPlan plan;        // tile shape and lane layout (its extent: the largest problem)
   BatchSweep bs;
   hipStream_t stream = nullptr;
+  // the one-wavefront sweep's second stream (half of the corners) and its
+  // fork / join events; a captured outer iteration holds both branches
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   Buf mat, slgg, qi, txs, txs32, sxs, dinv, dinv32, flux0, flux0pi, flux0po, fluxm, q2grp0,
@@ -223,6 +227,9 @@ struct snap_batch {
     if (graph) (void)hipGraphDestroy(graph);
     plan_free(&plan);
     quad_free(&quad);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -343,6 +350,9 @@ struct snap_batch {
     L.batch = &b;
     if (seq) {
       L.recompute_dinv = true;  // dinv formed from t_xs in the kernel
+      L.side = side;
+      L.fork = fork_ev;
+      L.join = join_ev;
       return sweep_seq_enqueue(L, stream, nullptr);
     }
     return sweep_enqueue(L, stream);
@@ -535,6 +545,9 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
   SNAP_TRY(quad_compute(nang, nmom, mu, eta, xi, w, ec));
   SNAP_TRY(quad_upload(&b->quad, ec.data(), mu.data(), eta.data(), xi.data(), w.data(), nang, nmom));
   SNAP_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  SNAP_HIP_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+  SNAP_HIP_TRY(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
+  SNAP_HIP_TRY(hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming));
   const hipStream_t st = b->stream;
   auto up = [&](Buf &d, const void *h, size_t n) -> int {
     SNAP_TRY(d.alloc(n));

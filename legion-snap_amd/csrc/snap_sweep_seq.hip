@@ -928,8 +928,7 @@ static SeqKernel seq_kernel_batch(int cmom) {
 
 int sweep_seq_streams(const SweepLaunch &L) {
   const int ncorner = L.ng_job > 0 ? L.nsweeps / L.ng_job : 0;
-  return (L.side && L.fork && L.join && !L.batch && ncorner * L.ng_job == L.nsweeps &&
-          ncorner % 2 == 0)
+  return (L.side && L.fork && L.join && ncorner * L.ng_job == L.nsweeps && ncorner % 2 == 0)
              ? 2
              : 1;
 }
