@@ -271,12 +271,14 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   const Real *in_y = (ty > 0 || (GEN && A.ggy.base)) ? out_y : nullptr;
   const Real *in_z = (tz > 0 || (GEN && A.ggz.base)) ? out_z : nullptr;
 
-  const double *qg = A.q + ((size_t)g * gstride + coff) * cmom;
-  const Real *dg = (const Real *)A.dinv + ((size_t)g * gstride + coff) * nang;
-  const Real *tg = (const Real *)A.txs + (size_t)g * gstride + coff;
-  double *pfg = A.part_flux + ((size_t)corner * A.ng_job + g) * gstride + coff;
-  double *pmg =
-      A.part_fluxm + (((size_t)corner * A.ng_job + g) * gstride + coff) * (size_t)(cmom - 1);
+  // the dense arrays (the GEN form goes through the accessors instead)
+  const double *qg = GEN ? nullptr : A.q + ((size_t)g * gstride + coff) * cmom;
+  const Real *dg = GEN ? nullptr : (const Real *)A.dinv + ((size_t)g * gstride + coff) * nang;
+  const Real *tg = GEN ? nullptr : (const Real *)A.txs + (size_t)g * gstride + coff;
+  double *pfg = GEN ? nullptr : A.part_flux + ((size_t)corner * A.ng_job + g) * gstride + coff;
+  double *pmg = GEN ? nullptr
+                    : A.part_fluxm + (((size_t)corner * A.ng_job + g) * gstride + coff) *
+                                         (size_t)(cmom - 1);
 
   // ---- per-lane constants (lane = angle; lanes past nang carry angle 0
   // with zero weight and never store)
