@@ -421,8 +421,10 @@ typedef struct {
                                1e-5, whichever is larger (max |flux0/flux0p - 1|);
                                0 = FP32 sweeps throughout */
   int sweep_kernel;         /* SNAP_SWEEP_AUTO (the autotuner times one-wavefront 4x4xTZ
-                               tiles, TZ 4 or 8, and multi-wave tiles), _TILED or
-                               _WAVE_TILE (up to 64 angles; tile[2] sets TZ, else timed) */
+                               tiles, TZ 4, 8 or 16, up to 64 angles, and else the
+                               multi-wave tiles), _TILED (only the multi-wave tiles)
+                               or _WAVE_TILE (up to 64 angles; tile[2] sets TZ, else
+                               timed) */
 } snap_batch_options_t;
 
 typedef struct {

@@ -650,11 +650,15 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
     int bi = -1, bapl = 0;
     bool warmed[3] = {false, false, false};  // per lane layout (one kernel each)
     // the one-wavefront tile sweep first (steady problems of up to 64 angles,
-    // any options that only the multi-wave tiles have left unset)
+    // any options that only the multi-wave tiles have left unset); when it
+    // takes the batch, the multi-wave shapes are not timed: it beat them on
+    // every batch and problem measured from 8 to 64 angles, and timing them
+    // cost more than the solve (3.6 s FP64, 7.2 s mixed on the 48-problem
+    // workload)
     int best_seq_tz = 0;
     if (kern != SNAP_SWEEP_TILED && nang <= 64 && apl <= 1 && waves <= 1) {
       bool warm = true;
-      for (int tz : {4, 8}) {
+      for (int tz : {4, 8, 16}) {
         if (b->build_plan(4, 4, tz, 1, 1, true) != SNAP_OK) continue;
         float ms = 0.f;
         SNAP_TRY(b->time_plan(1, &ms, warm));
@@ -665,7 +669,9 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
         }
       }
     }
-    const int ncand = kern == SNAP_SWEEP_WAVE_TILE ? 0 : (int)(sizeof(cand) / sizeof(cand[0]));
+    const int ncand = (kern == SNAP_SWEEP_WAVE_TILE || best_seq_tz > 0)
+                          ? 0
+                          : (int)(sizeof(cand) / sizeof(cand[0]));
     for (int c = 0; c < ncand; c++) {
       for (int a = 1; a <= 2; a++) {
         if (apl > 0 && a != apl) continue;

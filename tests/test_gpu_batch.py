@@ -16,19 +16,27 @@ def _small_batch(count=5, seed=17):
     return problems.config4(count=count, seed=seed, ng=4, nang=10, nmom=2, edges=(6, 9, 12, 16))
 
 
+@pytest.mark.parametrize("kernel", ["auto", "tiled"])
 @pytest.mark.parametrize("use_graph", [True, False])
 @pytest.mark.parametrize("precision,polish", [("fp64", None), ("mixed", 0.0), ("mixed", None)])
-def test_batch_equals_single_solves(gpu, precision, polish, use_graph):
+def test_batch_equals_single_solves(gpu, precision, polish, use_graph, kernel):
     probs = _small_batch()
-    with Batch(probs, precision=precision, use_graph=use_graph, fp64_polish=polish) as b:
+    with Batch(probs, precision=precision, use_graph=use_graph, fp64_polish=polish,
+               sweep_kernel=kernel) as b:
         st, per = b.run()
         fluxes = [b.flux(i) for i in range(len(probs))]
     print(st, [(x["sweeps_fp64"], x["inner_total"], round(x["time_ms"], 3)) for x in per])
     assert st["converged"] == len(probs)
+    # the single solves on the kernel the batch ran: the autotuner takes the
+    # one-wavefront tiles up to 64 angles (the solver's default here is the
+    # multi-wave tiles, whose fused arithmetic differs in the last bits)
+    seq = st["waves"] == 1 and tuple(st["tile"][:2]) == (4, 4)
+    assert seq == (kernel == "auto")
+    kw = dict(sweep_kernel="wave_tile", tile=(0, 0, st["tile"][2])) if seq else {}
     for i, p in enumerate(probs):
         # the same polish rule (FP32 throughout with 0) picks the same sweeps;
         # batches sweep the whole source (no defect correction)
-        with Solver(p, precision=precision, fp64_polish=polish, refine=False) as s:
+        with Solver(p, precision=precision, fp64_polish=polish, refine=False, **kw) as s:
             ss = s.run()
             f, fm = s.flux()
         assert (per[i]["inner_total"], per[i]["outer_total"]) == (ss["inner_total"], ss["outer_total"])
