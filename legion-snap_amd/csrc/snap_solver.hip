@@ -509,47 +509,50 @@ extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) 
           sweep_ms += ms;
           if (inner_converged) break;  // the iteration enqueued after it does nothing
         }
-      }
-      for (int inno = 0; mixed && inno < s->prob.iitm; inno++) {
-        SNAP_TRY(s->inner_source());
-        SNAP_TRY(s->copy(s->flux0pi, s->flux0, sizeof(double) * gc));
-        const bool refine = mixed && s->opt.refine;
-        // FP64 polish: an inner whose predecessor moved the flux by at most
-        // fp64_polish * epsi is likely one of the outer's last (the change
-        // contracts by the in-group scattering ratio each inner), so it runs
-        // in FP64 and the outer ends on FP64 sweeps; so does every inner once
-        // FP32 sweeps have reached their own rounding level
-        const bool polish_now =
-            polish && change <= polish_threshold(s->opt.fp64_polish, s->prob.epsi);
-        const bool fp32 = mixed && !(s->opt.first_sweep_fp64 && st.inner_total == 0) &&
-                          !(refine && inno == 0) && !polish_now;
-        if (refine && fp32)  // FP64 defect of the source against the outer's base
-          SNAP_TRY(launch_subtract(s->qtot.as<double>(), s->qbase.as<double>(),
-                                   (int64_t)gc * s->cmom, stream));
-        SNAP_HIP_TRY(hipEventRecord(s->ev[0], stream));
-        SNAP_TRY(s->sweep(fp32 ? SNAP_PRECISION_MIXED
-                               : (mixed ? SNAP_PRECISION_FP64 : s->opt.precision),
-                          nullptr, !(refine && fp32)));
-        // the octant sum is part of the sweep: the reference accumulates the
-        // flux moments inside its sweep kernel (gpu_sweep.cu:1680-1733)
-        SNAP_TRY(s->reduce_octants(refine && fp32));
-        SNAP_HIP_TRY(hipEventRecord(s->ev[1], stream));
-        if (refine && !fp32) {  // keep the FP64 base of this outer
-          SNAP_TRY(s->copy(s->qbase, s->qtot, sizeof(double) * gc * s->cmom));
-          SNAP_TRY(s->copy(s->flux0base, s->flux0, sizeof(double) * gc));
-          if (nm1) SNAP_TRY(s->copy(s->fluxmbase, s->fluxm, sizeof(double) * gc * nm1));
+      } else {
+        // mixed precision: the next inner sweep's precision depends on this
+        // test, so the host reads every test before it enqueues more
+        for (int inno = 0; inno < s->prob.iitm; inno++) {
+          SNAP_TRY(s->inner_source());
+          SNAP_TRY(s->copy(s->flux0pi, s->flux0, sizeof(double) * gc));
+          const bool refine = mixed && s->opt.refine;
+          // FP64 polish: an inner whose predecessor moved the flux by at most
+          // fp64_polish * epsi is likely one of the outer's last (the change
+          // contracts by the in-group scattering ratio each inner), so it runs
+          // in FP64 and the outer ends on FP64 sweeps; so does every inner once
+          // FP32 sweeps have reached their own rounding level
+          const bool polish_now =
+              polish && change <= polish_threshold(s->opt.fp64_polish, s->prob.epsi);
+          const bool fp32 = mixed && !(s->opt.first_sweep_fp64 && st.inner_total == 0) &&
+                            !(refine && inno == 0) && !polish_now;
+          if (refine && fp32)  // FP64 defect of the source against the outer's base
+            SNAP_TRY(launch_subtract(s->qtot.as<double>(), s->qbase.as<double>(),
+                                     (int64_t)gc * s->cmom, stream));
+          SNAP_HIP_TRY(hipEventRecord(s->ev[0], stream));
+          SNAP_TRY(s->sweep(fp32 ? SNAP_PRECISION_MIXED
+                                 : (mixed ? SNAP_PRECISION_FP64 : s->opt.precision),
+                            nullptr, !(refine && fp32)));
+          // the octant sum is part of the sweep: the reference accumulates the
+          // flux moments inside its sweep kernel (gpu_sweep.cu:1680-1733)
+          SNAP_TRY(s->reduce_octants(refine && fp32));
+          SNAP_HIP_TRY(hipEventRecord(s->ev[1], stream));
+          if (refine && !fp32) {  // keep the FP64 base of this outer
+            SNAP_TRY(s->copy(s->qbase, s->qtot, sizeof(double) * gc * s->cmom));
+            SNAP_TRY(s->copy(s->flux0base, s->flux0, sizeof(double) * gc));
+            if (nm1) SNAP_TRY(s->copy(s->fluxmbase, s->fluxm, sizeof(double) * gc * nm1));
+          }
+          if (fp32)
+            st.sweeps_fp32++;
+          else
+            st.sweeps_fp64++;
+          st.inner_total++;
+          SNAP_TRY(s->test(s->flux0pi, s->prob.epsi, &inner_converged, polish ? &change : nullptr));
+          record(0, cy, otno, inno, inner_converged);
+          float ms = 0.f;
+          SNAP_HIP_TRY(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+          sweep_ms += ms;
+          if (inner_converged) break;
         }
-        if (fp32)
-          st.sweeps_fp32++;
-        else
-          st.sweeps_fp64++;
-        st.inner_total++;
-        SNAP_TRY(s->test(s->flux0pi, s->prob.epsi, &inner_converged, polish ? &change : nullptr));
-        record(0, cy, otno, inno, inner_converged);
-        float ms = 0.f;
-        SNAP_HIP_TRY(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
-        sweep_ms += ms;
-        if (inner_converged) break;
       }
       if (otno == 0) continue;
       bool outer_converged = false;
