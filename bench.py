@@ -89,7 +89,7 @@ def copy_bandwidth_gbs(torch, gib: int = 2, reps: int = 5) -> float:
     return 2 * n * 8 / best / 1e9
 
 
-def cpu_baseline(problem, budget_s: float = 25.0, reps: int = 3):
+def cpu_baseline(problem, budget_s: float = 25.0, reps: int = 5):
     """Time the CPU oracle on one inner iteration's sweeps (all octants and
     groups) of `problem` using every host core (best of `reps` after a
     warm-up), or of a reduced group count when that would exceed the budget
