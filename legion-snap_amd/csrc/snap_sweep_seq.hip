@@ -23,13 +23,21 @@
 //     scheduler overlaps independent cells (those on one i+j diagonal);
 //  3. only when some outflow of the plane came out negative, the plane is
 //     swept again cell by cell with the reference's fixup loop (from the
-//     plane's saved z inflows), which is rare;
+//     plane's saved z inflows) -- about a third of the planes of the 64^3
+//     test workload, whose vacuum boundaries and source-free regions make
+//     the fixup common;
 //  4. the flux moments phi[l][cell] = sum_a ec[l][a] (w[a] pc[cell][a]) as
 //     FP64 matrix products.
 // The EXACT form (reference operation order, separate roundings, bit-equal
 // to the CPU arithmetic) and ragged tiles take the general path: per cell,
 // the source expansion on the vector units in the reference's order and the
 // fixup test right after the cell (sweep.cc:331-591).
+//
+// Forms: dense (the solver: [g][cell] arrays, per-octant partial moments),
+// BATCH (many problems per launch) and GEN (the drop-in group range: the
+// caller's accessors, moments added by atomics, ghost faces staged through
+// the tile-face slots by k_seq_ghosts).  The host may split a launch's
+// sweeps over two streams (sweep offset s0).
 #include <algorithm>
 #include <cstring>
 #include <type_traits>
