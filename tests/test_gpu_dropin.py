@@ -294,3 +294,45 @@ def test_group_range_fused_sub_chunk(setup, with_ghosts):
         if with_ghosts:
             for k in range(3):
                 assert torch.allclose(ga[k], gb[k], rtol=1e-12, atol=1e-15), (corner, k)
+
+
+@pytest.mark.parametrize("nmom,nang", [(1, 8), (3, 20), (4, 36)])
+def test_group_range_fused_moment_counts(gpu, oracle_mod, nmom, nang):
+    # the fused group range (one-wavefront tiles, one kernel per moment count)
+    # against run_gpu_sweep once per group, for 1, 9 and 16 flux moments; the
+    # 16-moment, 36-angle case is the 64^3 workload's kernel on a small chunk
+    import torch
+    from snap_amd import tasks
+    cmom = nmom * nmom
+    nx, ny, nz, ng = 9, 8, 11, 3
+    quad = oracle_mod.quadrature(nang, nmom)
+    tasks.initialize_gpu_context(quad, nang, nmom, (nx, ny, nz))
+    h = (2.0 / (0.6 / nx), 2.0 / (0.8 / ny), 2.0 / (0.7 / nz))
+    rng = np.random.default_rng(5 + nmom)
+    T = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")  # noqa: E731
+    txs = rng.uniform(0.5, 3.0, (ng, nz, ny, nx))
+    mu, eta, xi, w, ec = quad
+    dinv = T(oracle_mod.geometry_param(txs, nang, h, mu, eta, xi))
+    qtot = rng.uniform(0, 1, (ng, nz, ny, nx, cmom))
+    qtot[..., 1:] *= 0.2
+    qtot_d, txs_d = T(qtot), T(txs)
+    z = lambda *shape: torch.zeros(shape, dtype=torch.float64, device="cuda")  # noqa: E731
+    for corner in (1, 6):
+        gin = (rng.uniform(0, 1, (ng, nz, ny, nang)), rng.uniform(0, 1, (ng, nz, nx, nang)),
+               rng.uniform(0, 1, (ng, ny, nx, nang)))
+        ga, gb = tuple(T(x) for x in gin), tuple(T(x) for x in gin)
+        fa, fb = z(ng, nz, ny, nx), z(ng, nz, ny, nx)
+        ma = mb = None
+        if cmom > 1:
+            ma, mb = z(ng, nz, ny, nx, cmom - 1), z(ng, nz, ny, nx, cmom - 1)
+        tasks.mini_kba(corner, range(ng), qtot_d, fa, ma, dinv, txs_d, ga, h, nang, nmom)
+        tasks.mini_kba(corner, range(ng), qtot_d, fb, mb, dinv, txs_d, gb, h, nang, nmom,
+                       group_range="fused")
+        torch.cuda.synchronize()
+        A, B = fa.cpu().numpy(), fb.cpu().numpy()
+        assert np.max(np.abs(B - A) / A) <= 1e-13, (corner, nmom)
+        if cmom > 1:
+            AM, BM = ma.cpu().numpy(), mb.cpu().numpy()
+            assert np.max(np.abs(BM - AM)) <= 1e-13 * np.max(np.abs(AM)), (corner, nmom)
+        for k in range(3):
+            assert torch.allclose(ga[k], gb[k], rtol=1e-12, atol=1e-15), (corner, nmom, k)
