@@ -294,6 +294,10 @@ typedef struct {
                                fp64_polish * epsi (max relative change), or by at most
                                1e-5 (FP32 sweeps' own rounding level) whatever epsi
                                asks; 0 = never */
+  int groups_per_wave;      /* one-wavefront tile sweeps: 0 = choose (two groups per
+                               wavefront, lanes 0-31 and 32-63, for fused sweeps of up
+                               to 32 angles with an even group count and x and y
+                               extents multiples of 4; bit-equal results), 1 = one */
 } snap_solve_options_t;
 
 enum { SNAP_SWEEP_AUTO = 0, SNAP_SWEEP_TILED = 1, SNAP_SWEEP_WAVE_TILE = 2 };
@@ -316,6 +320,8 @@ typedef struct {
   double units_per_sweep;   /* cells * 8*nang * ng */
   double grind_ns;          /* sweep_ms*1e6 / (units_per_sweep * inner_total) */
   int time_steps;           /* time steps run; converged = every step converged */
+  int groups_per_wave;      /* groups per wavefront of the inner sweeps (2: see
+                               snap_solve_options_t.groups_per_wave) */
 } snap_solve_stats_t;
 
 typedef struct snap_solver snap_solver_t;
@@ -425,6 +431,9 @@ typedef struct {
                                multi-wave tiles), _TILED (only the multi-wave tiles)
                                or _WAVE_TILE (up to 64 angles; tile[2] sets TZ, else
                                timed) */
+  int groups_per_wave;      /* one-wavefront tiles: 0 = choose (two groups per wavefront
+                               where snap_solve_options_t's form applies, every
+                               problem's x and y extents multiples of 4), 1 = one */
 } snap_batch_options_t;
 
 typedef struct {
@@ -450,6 +459,7 @@ typedef struct {
                                tile came from the process's cache of earlier batches
                                of the same shapes) */
   int plan_cached;          /* 1 = tile taken from that cache */
+  int groups_per_wave;      /* groups per wavefront of the sweeps */
 } snap_batch_stats_t;
 
 int snap_hip_batch_create(const snap_problem_t *problems, int count,

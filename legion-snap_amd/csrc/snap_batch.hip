@@ -208,6 +208,7 @@ struct snap_batch {
   QuadDev quad;
   SweepPlan plan;        // tile shape and lane layout (its extent: the largest problem)
   BatchSweep bs;
+  int groups_per_wave = 1;  // of the last sweep pass enqueued
   hipStream_t stream = nullptr;
   // the one-wavefront sweep's second stream (half of the corners) and its
   // fork / join events; a captured outer iteration holds both branches
@@ -317,6 +318,8 @@ struct snap_batch {
     bs.active = active.as<int>();
     bs.done = done.as<int>();
     bs.total_cells = total;
+    bs.full_xy = std::all_of(ext.begin(), ext.end(),
+                             [](const Extent3 &e) { return e.nx % 4 == 0 && e.ny % 4 == 0; });
     if (graph_exec) {  // a captured outer iteration holds the old launches
       (void)hipGraphExecDestroy(graph_exec);
       (void)hipGraphDestroy(graph);
@@ -353,8 +356,11 @@ struct snap_batch {
       L.side = side;
       L.fork = fork_ev;
       L.join = join_ev;
+      L.pair = opt.groups_per_wave != 1;
+      groups_per_wave = sweep_seq_pair(L) ? 2 : 1;
       return sweep_seq_enqueue(L, stream, nullptr);
     }
+    groups_per_wave = 1;
     return sweep_enqueue(L, stream);
   }
 
@@ -481,6 +487,9 @@ extern "C" int snap_hip_batch_create(const snap_problem_t *probs, int count,
   if (!(b->opt.fp64_polish >= 0.0)) return fail(SNAP_ERR_INVALID, "fp64_polish must be >= 0");
   if (b->opt.sweep_kernel < SNAP_SWEEP_AUTO || b->opt.sweep_kernel > SNAP_SWEEP_WAVE_TILE)
     return fail(SNAP_ERR_INVALID, "unknown sweep kernel %d", b->opt.sweep_kernel);
+  if (b->opt.groups_per_wave != 0 && b->opt.groups_per_wave != 1)
+    return fail(SNAP_ERR_INVALID, "groups_per_wave must be 0 (choose) or 1, got %d",
+                b->opt.groups_per_wave);
   if (b->opt.sweep_kernel == SNAP_SWEEP_WAVE_TILE && p0.nang > 64)
     return fail(SNAP_ERR_UNSUPPORTED, "the one-wavefront tile sweep handles up to 64 angles");
   b->polish = b->precision == SNAP_PRECISION_MIXED && b->opt.fp64_polish > 0.0;
@@ -793,6 +802,7 @@ extern "C" int snap_hip_batch_run(snap_batch_t *b, snap_batch_stats_t *stats,
     stats->tile[2] = b->plan.TZ;
     stats->waves = b->plan.waves;
     stats->angles_per_lane = b->plan.apl;
+    stats->groups_per_wave = b->groups_per_wave;
     stats->autotune_ms = b->autotune_ms;
     stats->plan_cached = b->plan_cached ? 1 : 0;
   }

@@ -246,6 +246,7 @@ struct BatchSweep {
   const int *prec = nullptr;        // device flags: sweep precision (1 = FP64); NULL = all
   int prec_want = 0;                // this pass sweeps the problems with prec[p] == prec_want
   int64_t total_cells = 0;
+  bool full_xy = false;             // every problem's x and y extents are multiples of 4
 };
 
 struct SweepLaunch {
@@ -276,6 +277,9 @@ struct SweepLaunch {
   hipStream_t side;
   hipEvent_t fork, join;
   const int32_t *skip;         // solver inner iterations enqueued ahead (see launch_inner_source_dense)
+  // one-wavefront tile sweeps: two groups per wavefront where that form
+  // applies (sweep_seq_pair)
+  bool pair;
   // time-dependent terms (td): vdelt per job group (dense) or vdelt0 (general);
   // time-edge fluxes in / out and the MMS angular source (NULL = none), dense
   // [8][ng_job][cells][nang] doubles or per-cell accessors of nang doubles
@@ -299,6 +303,10 @@ int plan_create_seq(SweepPlan *p, Extent3 e, int nang, int cmom, int TZ);
 // events[2 (D streams + q)] and the next one, on its own stream.
 int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events = nullptr);
 int sweep_seq_streams(const SweepLaunch &L);
+// True when the launch asks for, and can take, two groups per wavefront:
+// fused, dinv formed in the kernel, up to 32 angles, an even group count,
+// full 4 x 4 tiles (bit-equal results to one group per wavefront)
+bool sweep_seq_pair(const SweepLaunch &L);
 // Quadrature of the drop-in context on device `dev` (NULL before init).
 const QuadDev *context_quad(int dev);
 

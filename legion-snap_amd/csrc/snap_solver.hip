@@ -75,6 +75,7 @@ struct snap_solver {
   // inner iterations enqueued one ahead (FP64 solves): two slots, each with
   // the test's device flag, its pinned host copy, and events around the
   // sweep and after the copy
+  int groups_per_wave = 1;         // of the last sweep enqueued
   DeviceBuffer ra_flag;            // [2] int32
   int32_t *h_ra = nullptr;         // pinned [2]
   hipEvent_t ra_ev[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
@@ -177,8 +178,11 @@ struct snap_solver {
       L.side = side;
       L.fork = fork_ev;
       L.join = join_ev;
+      L.pair = opt.groups_per_wave != 1;
+      groups_per_wave = sweep_seq_pair(L) ? 2 : 1;
       return sweep_seq_enqueue(L, stream, events);
     }
+    groups_per_wave = 1;
     return sweep_enqueue(L, stream, events);
   }
 
@@ -294,6 +298,9 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
                   "the one-wavefront tile sweep takes steady problems of up to 64 angles");
     if (s->opt.sweep_kernel < SNAP_SWEEP_AUTO || s->opt.sweep_kernel > SNAP_SWEEP_WAVE_TILE)
       return fail(SNAP_ERR_INVALID, "unknown sweep kernel %d", s->opt.sweep_kernel);
+    if (s->opt.groups_per_wave != 0 && s->opt.groups_per_wave != 1)
+      return fail(SNAP_ERR_INVALID, "groups_per_wave must be 0 (choose) or 1, got %d",
+                  s->opt.groups_per_wave);
     // options that only the multi-wave tile sweep has keep AUTO on it
     const bool tiled_opts = s->opt.angles_per_lane != 0 || s->opt.waves_per_tile != 0 ||
                             s->opt.tile[0] != 0 || s->opt.tile[1] != 0;
@@ -577,6 +584,7 @@ extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) 
   st.sweep_ms = sweep_ms;
   st.solve_ms = solve_ms;
   st.units_per_sweep = (double)s->cells * 8.0 * s->nang * s->ng;
+  st.groups_per_wave = s->groups_per_wave;
   st.grind_ns = st.inner_total ? (sweep_ms * 1e6) / (st.units_per_sweep * st.inner_total) : 0.0;
   if (stats) *stats = st;
   return SNAP_OK;

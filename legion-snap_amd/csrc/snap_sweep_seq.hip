@@ -168,11 +168,23 @@ struct TileGeom {
 // through accessors, ghost faces at the chunk's sides, and the moments added
 // to the caller's fluxes by device atomics (the caller's other corner tasks
 // add to the same cells concurrently).
-template <typename Real, bool EXACT, bool REC, int CMOM, bool BATCH = false, bool GEN = false>
+// PAIR: two groups per wavefront for up to 32 angles: lanes 0-31 sweep group
+// g and lanes 32-63 group g + 1 of the same octant and tile (blockIdx.y
+// counts group pairs), each half with its own LDS tables, faces, sources,
+// fixup counts and moments; the matrix products run once per group.  Full
+// 4 x 4 tiles only (the caller's x and y extents are multiples of 4), fused,
+// dinv formed in the kernel.  Every group's results are bit-equal to the
+// one-group form: a plane swept again cell by cell because of the other
+// group's fixup gives the values of the branch-free pass where no outflow is
+// negative.
+template <typename Real, bool EXACT, bool REC, int CMOM, bool BATCH = false, bool GEN = false,
+          bool PAIR = false>
 __global__ void __launch_bounds__(64 * kWGW) __attribute__((amdgpu_waves_per_eu(SEQ_WAVES, 8)))
 k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   constexpr bool kFP64 = std::is_same<Real, double>::value;
   static_assert(!GEN || (kFP64 && !EXACT && REC && !BATCH), "GEN is the fused FP64 drop-in form");
+  static_assert(!PAIR || (!EXACT && REC && !GEN), "PAIR is a fused form that forms dinv");
+  constexpr int NGW = PAIR ? 2 : 1;  // groups per wavefront
   if (A.skip && *A.skip) return;  // the whole workgroup: an inner iteration enqueued ahead
   constexpr int KS = (CMOM + 3) / 4;  // four-moment steps of the source product
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -181,7 +193,9 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   const int NB = (nang + 15) >> 4;  // 16-angle blocks of the source product
   const int NST = (nang + 3) >> 2;  // four-angle steps of the moment product
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int s = A.s0 + blockIdx.y;
+  // the lane's group of the pair and its angle slot
+  const int hh = PAIR ? (lane >> 5) : 0, la = PAIR ? (lane & 31) : lane;
+  const int s = A.s0 + NGW * (int)blockIdx.y;
   const int corner = A.corner0 + s / A.ng_job, g = s % A.ng_job;
   // the sweep's operand tables, shared by the workgroup's wavefronts:
   // EB[nb][kk][lane] = ec[4kk + lane/16][16nb + lane%16] (the source
@@ -212,11 +226,15 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   // (general); Z: the plane's z inflows [slot][angle] while the fixup may
   // sweep it again (fast path)
   using WT = typename std::conditional<EXACT, double, Real>::type;
-  unsigned char *wbase = (unsigned char *)(EA + (size_t)NST * 64) +
-                         (size_t)wave * seq_wave_bytes(RS, sizeof(Real), sizeof(WT));
-  WT *W = (WT *)wbase;
-  Real *P = (Real *)(wbase + seq_w_bytes(RS, sizeof(WT)));
-  Real *Zb = (Real *)(wbase + seq_w_bytes(RS, sizeof(WT)) + seq_p_bytes(RS, sizeof(Real)));
+  // (PAIR: one set of the three tables per group, the lane's own at W, P, Zb)
+  const size_t gby = seq_wave_bytes(RS, sizeof(Real), sizeof(WT));
+  unsigned char *wbase = (unsigned char *)(EA + (size_t)NST * 64) + (size_t)wave * NGW * gby;
+  auto W_of = [&](int h) { return (WT *)(wbase + h * gby); };
+  auto P_of = [&](int h) { return (Real *)(wbase + h * gby + seq_w_bytes(RS, sizeof(WT))); };
+  WT *W = W_of(hh);
+  Real *P = P_of(hh);
+  Real *Zb = (Real *)(wbase + hh * gby + seq_w_bytes(RS, sizeof(WT)) +
+                      seq_p_bytes(RS, sizeof(Real)));
   Real *qs = P;
   // the chunk: the launch's own, or one problem of a batch
   Extent3 ext = A.ext;
@@ -278,6 +296,8 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   const Real *in_x = (tx > 0 || (GEN && A.ggx.base)) ? out_x : nullptr;
   const Real *in_y = (ty > 0 || (GEN && A.ggy.base)) ? out_y : nullptr;
   const Real *in_z = (tz > 0 || (GEN && A.ggz.base)) ? out_z : nullptr;
+  // (PAIR: group g + 1's faces are sweep s + 1's, one sweep's faces later)
+  const int fs = PAIR ? hh * (int)(nfx + nfy + nfz) : 0;
 
   // the dense arrays (the GEN form goes through the accessors instead)
   const double *qg = GEN ? nullptr : A.q + ((size_t)g * gstride + coff) * cmom;
@@ -290,8 +310,9 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
 
   // ---- per-lane constants (lane = angle; lanes past nang carry angle 0
   // with zero weight and never store)
-  const bool aok = lane < nang;
-  const int a = aok ? lane : 0;
+  const bool aok = la < nang;
+  const int a = aok ? la : 0;
+  const int af = a + fs;  // the lane's angle in its group's faces
   // GEN: a cell's record in an accessor
   auto acc_cell = [&](const Acc3 &X, int i, int j, int k) -> const char * {
     return X.at(T.px(i), T.py(j), T.pz(k), g);
@@ -341,7 +362,17 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   const int r16 = lane & 15, q4 = lane >> 4;  // matrix-core operand coordinates
   // W slot of the lane in row n: its angle's column, or (lanes past the
   // row) its own slot of a junk row after the 16 rows, which nothing reads
-  auto wslot = [&](int n) { return lane < RS ? n * RS + lane : kNP * RS + lane; };
+  auto wslot = [&](int n) { return la < RS ? n * RS + la : kNP * RS + la; };
+  // t_xs of the lane's group in cell slot n: lane n's (group g) or lane
+  // 32 + n's (group g + 1), which hold slot n of their group
+  auto tbcast = [&](Real v, int n) -> Real {
+    if constexpr (PAIR) {
+      const Real v0 = bcast(v, n), v1 = bcast(v, 32 + n);
+      return hh ? v1 : v0;
+    } else {
+      return bcast(v, n);
+    }
+  };
   // Z slot of cell n: by angle, so that lanes past nang (which mirror angle
   // 0 and store its faces again) read and write angle 0's slot, with the
   // values angle 0 has
@@ -376,22 +407,53 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
     bool hx = true, hy = true, hz = true;
     if (A.fixup) {
       // lanes past nang hold angle 0's values and flags (they mirror lane 0
-      // exactly) but are not counted
+      // exactly) but are not counted; PAIR: each half counts its own group,
+      // and a half whose loop has ended (act false) keeps pc and its flags
+      // while the other half's passes go on
+      bool act = true;
+      auto popc = [&](unsigned long long b) -> unsigned {
+        if constexpr (PAIR)
+          return (unsigned)__popc((unsigned)(b >> (32 * hh)));
+        else
+          return (unsigned)__popcll(b);
+      };
       auto count_negative = [&]() -> unsigned {
         const Real p2 = Real(2) * pc;
         const bool nxn = p2 - pi_ < Real(0);
         const bool nyn = p2 - pj_ < Real(0);
         const bool nzn = p2 - pk_ < Real(0);
-        hx = hx & !nxn;
-        hy = hy & !nyn;
-        hz = hz & !nzn;
-        return (unsigned)__popcll(__builtin_amdgcn_ballot_w64(aok & nxn)) +
-               (unsigned)__popcll(__builtin_amdgcn_ballot_w64(aok & nyn)) +
-               (unsigned)__popcll(__builtin_amdgcn_ballot_w64(aok & nzn));
+        hx = hx & !(act & nxn);
+        hy = hy & !(act & nyn);
+        hz = hz & !(act & nzn);
+        return popc(__builtin_amdgcn_ballot_w64(aok & nxn)) +
+               popc(__builtin_amdgcn_ballot_w64(aok & nyn)) +
+               popc(__builtin_amdgcn_ballot_w64(aok & nzn));
       };
       unsigned old_neg = count_negative();
       fixed = old_neg != 0;
-      if (old_neg != 0) {
+      if constexpr (PAIR) {
+        act = fixed;
+        if (__builtin_amdgcn_ballot_w64(act) != 0) {
+          for (int it = 1; it < 3 * nang + 2; ++it) {
+            const Real vx = hx ? Real(1) : Real(0), vy = hy ? Real(1) : Real(0),
+                       vz = hz ? Real(1) : Real(0);
+            const Real t = pi_ * cx * (Real(1) + vx) + pj_ * cy * (Real(1) + vy) +
+                           pk_ * cz * (Real(1) + vz);
+            const Real p = psi + Real(0.5) * t;
+            const Real den = (p <= Real(0)) ? Real(0) : (txs + cx * vx + cy * vy + cz * vz);
+            Real pn;
+            if constexpr (kFP64)
+              pn = (den < tolr) ? Real(0) : p * recip(den);
+            else
+              pn = (den < tolr) ? Real(0) : p / den;
+            pc = act ? pn : pc;
+            const unsigned neg = count_negative();
+            act = act && neg != old_neg;
+            old_neg = neg;
+            if (__builtin_amdgcn_ballot_w64(act) == 0) break;
+          }
+        }
+      } else if (old_neg != 0) {
         // a pass that clears no new flag leaves pc unchanged, so the pass
         // after it repeats the count: 3*nang+2 passes always reach the break
         for (int it = 1; it < 3 * nang + 2; ++it) {
@@ -457,34 +519,39 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
       // phi[l][cell] over four angles per product: A[l][kk] = ec[l][4s+kk]
       // (ec[0] = 1) times w in lane l + 16 kk, B[kk][cell] = pc in lane cell + 16 kk,
       // D[l][cell] in lane cell + 16 (l % 4), register l / 4
-      // two accumulators halve the dependent-product chain
-      f64x4 acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
-      int st = 0;
-      for (; st + 1 < NST; st += 2) {
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
-            EA[st * 64 + lane], (double)W[r16 * RS + 4 * st + q4], acc, 0, 0, 0);
-        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(
-            EA[(st + 1) * 64 + lane], (double)W[r16 * RS + 4 * st + 4 + q4], acc2, 0, 0, 0);
-      }
-      if (st < NST)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
-            EA[st * 64 + lane], (double)W[r16 * RS + 4 * st + q4], acc, 0, 0, 0);
-      acc += acc2;
-      const int n = r16;
-      const int c = T.cell(n & 3, n >> 2, k);
-      if (c >= 0) {
+      // two accumulators halve the dependent-product chain (PAIR: one
+      // product per group, from its own W rows, into its own partials)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int l = 4 * r + q4;
-          if (l >= cmom) break;
-          if constexpr (GEN) {
-            double *dst = l == 0 ? (double *)acc_cell(A.gflux, n & 3, n >> 2, k)
-                                 : (double *)acc_cell(A.gfluxm, n & 3, n >> 2, k) + (l - 1);
-            atomicAdd(dst, acc[r]);
-          } else if (l == 0) {
-            pfg[c] = acc[r];
-          } else {
-            pmg[(size_t)c * (cmom - 1) + (l - 1)] = acc[r];
+      for (int h = 0; h < NGW; h++) {
+        const WT *Wh = W_of(h);
+        f64x4 acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
+        int st = 0;
+        for (; st + 1 < NST; st += 2) {
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              EA[st * 64 + lane], (double)Wh[r16 * RS + 4 * st + q4], acc, 0, 0, 0);
+          acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              EA[(st + 1) * 64 + lane], (double)Wh[r16 * RS + 4 * st + 4 + q4], acc2, 0, 0, 0);
+        }
+        if (st < NST)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              EA[st * 64 + lane], (double)Wh[r16 * RS + 4 * st + q4], acc, 0, 0, 0);
+        acc += acc2;
+        const int n = r16;
+        const int c = T.cell(n & 3, n >> 2, k);
+        if (c >= 0) {
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int l = 4 * r + q4;
+            if (l >= cmom) break;
+            if constexpr (GEN) {
+              double *dst = l == 0 ? (double *)acc_cell(A.gflux, n & 3, n >> 2, k)
+                                   : (double *)acc_cell(A.gfluxm, n & 3, n >> 2, k) + (l - 1);
+              atomicAdd(dst, acc[r]);
+            } else if (l == 0) {
+              pfg[(size_t)h * gstride + c] = acc[r];
+            } else {
+              pmg[((size_t)h * gstride + c) * (cmom - 1) + (l - 1)] = acc[r];
+            }
           }
         }
       }
@@ -496,7 +563,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
 #pragma unroll
   for (int n = 0; n < kNP; n++) {
     const int i = n & 3, j = n >> 2;
-    zf[n] = (in_z && i < T.ex && j < T.ey) ? in_z[(size_t)(j * kSX + i) * nang + a] : Real(0);
+    zf[n] = (in_z && i < T.ex && j < T.ey) ? in_z[(size_t)(j * kSX + i) * nang + af] : Real(0);
   }
 #pragma unroll
   for (int i = 0; i < kSX; i++) yf[i] = Real(0);
@@ -510,40 +577,48 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   auto load_xy_inflows = [&](int k, Real(&xin)[kSY], Real(&yin)[kSX]) {
 #pragma unroll
     for (int j = 0; j < kSY; j++) {
-      const Real v = in_x_ld[(size_t)(k * kSY + min(j, T.ey - 1)) * nang + a];
+      const Real v = in_x_ld[(size_t)(k * kSY + min(j, T.ey - 1)) * nang + af];
       xin[j] = (has_x && j < T.ey) ? v : Real(0);
     }
 #pragma unroll
     for (int i = 0; i < kSX; i++) {
-      const Real v = in_y_ld[(size_t)(k * kSX + min(i, T.ex - 1)) * nang + a];
+      const Real v = in_y_ld[(size_t)(k * kSX + min(i, T.ex - 1)) * nang + af];
       yin[i] = (has_y && i < T.ex) ? v : Real(0);
     }
   };
 
   const bool fast = !EXACT && T.ex == kSX && T.ey == kSY;
+  // (PAIR launches cover full tiles only: their callers' x and y extents are
+  // multiples of 4)
+  if (PAIR && !fast) return;
   if (fast) {
     // ================= fast path: full tile, fused or FP32 =================
     // source operands of plane k: lane L holds q[slot L%16][4 kk + L/16]
-    double qa[KS];
+    // (PAIR: of each group)
+    double qa[NGW][KS];
     auto qa_load = [&](int k) {
       const int c = T.c0 + (r16 & 3) * T.sx + (r16 >> 2) * T.sy + k * T.sz;
 #pragma unroll
-      for (int kk = 0; kk < KS; kk++) {
-        const int l = 4 * kk + q4;
-        if constexpr (GEN)
-          qa[kk] = (l < CMOM) ? ((const double *)acc_cell(A.gq, r16 & 3, r16 >> 2, k))[l] : 0.0;
-        else
-          qa[kk] = (l < CMOM) ? qg[(size_t)c * cmom + l] : 0.0;
-      }
+      for (int h = 0; h < NGW; h++)
+#pragma unroll
+        for (int kk = 0; kk < KS; kk++) {
+          const int l = 4 * kk + q4;
+          if constexpr (GEN)
+            qa[h][kk] =
+                (l < CMOM) ? ((const double *)acc_cell(A.gq, r16 & 3, r16 >> 2, k))[l] : 0.0;
+          else
+            qa[h][kk] = (l < CMOM) ? qg[((size_t)h * gstride + c) * cmom + l] : 0.0;
+        }
     };
-    // t_xs of slot L%16 in lane L (read by the cells with a lane broadcast)
+    // t_xs of slot L%16 in lane L (read by the cells with a lane broadcast;
+    // PAIR: lanes 32-63 hold group g + 1's)
     Real tv = Real(0);
     auto tv_load = [&](int k) {
       const int c = T.c0 + (r16 & 3) * T.sx + (r16 >> 2) * T.sy + k * T.sz;
       if constexpr (GEN)
         tv = *(const Real *)acc_cell(A.gtxs, r16 & 3, r16 >> 2, k);
       else
-        tv = tg[c];
+        tv = tg[(size_t)hh * gstride + c];
     };
     Real dv[kNP];
     auto dv_load = [&](int k) {
@@ -569,16 +644,20 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
       // (1) psi[slot][angle] = sum_l q[slot][l] ec[l][angle] for each block of
       // 16 angles: A = q (cells x moments), B = ec (moments x angles), D[cell]
       // [angle] in lane angle + 16 (cell % 4), register cell / 4
-      for (int nb = 0; nb < NB; nb++) {
-        f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-        const int ang = 16 * nb + r16;
 #pragma unroll
-        for (int kk = 0; kk < KS; kk++)
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[kk], EB[(nb * KS + kk) * 64 + lane], acc,
-                                                     0, 0, 0);
-        if (ang < RS) {
+      for (int h = 0; h < NGW; h++) {
+        Real *Ph = P_of(h);
+        for (int nb = 0; nb < NB; nb++) {
+          f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+          const int ang = 16 * nb + r16;
 #pragma unroll
-          for (int r = 0; r < 4; r++) P[(4 * r + q4) * RS + ang] = (Real)acc[r];
+          for (int kk = 0; kk < KS; kk++)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[h][kk], EB[(nb * KS + kk) * 64 + lane],
+                                                       acc, 0, 0, 0);
+          if (ang < RS) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) Ph[(4 * r + q4) * RS + ang] = (Real)acc[r];
+          }
         }
       }
       // the next plane's operands, in flight while this plane is swept
@@ -619,7 +698,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
             const Real psi = P[n * RS + a];
             Real d;
             if constexpr (REC)
-              d = dinv_of(bcast(tv_c, n));
+              d = dinv_of(tbcast(tv_c, n));
             else
               d = dv_c[n];
             const Real pi_ = (i == 0) ? xin[j] : xf[j];
@@ -636,8 +715,8 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
             yf[i] = oy;
             zf[n] = oz;
             // lanes past nang mirror lane 0 and store its values again
-            if (i == kSX - 1) out_x[(size_t)(k * kSY + j) * nang + a] = ox;
-            if (j == kSY - 1) out_y[(size_t)(k * kSX + i) * nang + a] = oy;
+            if (i == kSX - 1) out_x[(size_t)(k * kSY + j) * nang + af] = ox;
+            if (j == kSY - 1) out_y[(size_t)(k * kSX + i) * nang + af] = oy;
             W[wslot(n)] = w_of(pc);
           }
           __builtin_amdgcn_sched_barrier(0);
@@ -666,7 +745,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
             psi_nx = P[(n + 1) * RS + a];
             pk_nx = Zb[zslot(n + 1)];
           }
-          const Real txs = bcast(tv_c, n);
+          const Real txs = tbcast(tv_c, n);
           Real d;
           if constexpr (REC)
             d = dinv_of(txs);
@@ -686,12 +765,14 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
           yq2 = yq3;
           yq3 = oy;
           Zb[zslot(n)] = oz;
-          if (i == kSX - 1) out_x[(size_t)(k * kSY + j) * nang + a] = ox;
-          if (j == kSY - 1) out_y[(size_t)(k * kSX + i) * nang + a] = oy;
+          if (i == kSX - 1) out_x[(size_t)(k * kSY + j) * nang + af] = ox;
+          if (j == kSY - 1) out_y[(size_t)(k * kSX + i) * nang + af] = oy;
           W[wslot(n)] = w_of(pc);
         }
 #pragma unroll
         for (int n = 0; n < kNP; n++) zf[n] = Zb[zslot(n)];
+        // (PAIR: either group's fixup sends both to the cell-by-cell pass)
+        if constexpr (PAIR) predict_fixup = __builtin_amdgcn_ballot_w64(predict_fixup) != 0;
       }
       wave_sync();
       // (4) flux moments
@@ -702,7 +783,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
 #pragma unroll
       for (int i = 0; i < kSX; i++) yin[i] = yin_n[i];
     }
-  } else {
+  } else if constexpr (!PAIR) {
     // ================= general path: exact form, ragged tiles ==============
     Real ecl[CMOM];  // ecl[l] = ec[l][a], l >= 1
 #pragma unroll
@@ -770,8 +851,8 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
         yf[i] = oy;
         zf[n] = oz;
         if (aok) {
-          if (i == T.ex - 1) out_x[(size_t)(k * kSY + j) * nang + a] = ox;
-          if (j == T.ey - 1) out_y[(size_t)(k * kSX + i) * nang + a] = oy;
+          if (i == T.ex - 1) out_x[(size_t)(k * kSY + j) * nang + af] = ox;
+          if (j == T.ey - 1) out_y[(size_t)(k * kSX + i) * nang + af] = oy;
         }
         W[wslot(n)] = w_of(pc);
       }
@@ -790,7 +871,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
     for (int n = 0; n < kNP; n++) {
       const int i = n & 3, j = n >> 2;
       if (i < T.ex && j < T.ey) {
-        out_z[(size_t)(j * kSX + i) * nang + a] = zf[n];
+        out_z[(size_t)(j * kSX + i) * nang + af] = zf[n];
       }
     }
   }
@@ -848,11 +929,11 @@ namespace {
 // LDS row stride of the [cell][angle] tables: the angles rounded up to the
 // four-angle matrix steps, plus one so the 16 rows start on different banks
 int seq_row_stride(int nang) { return ((nang + 3) & ~3) + 1; }
-size_t seq_lds_bytes(int nang, int cmom, bool fp32, bool exact) {
+size_t seq_lds_bytes(int nang, int cmom, bool fp32, bool exact, int ngw) {
   const int rs = seq_row_stride(nang);
   const size_t tables = (size_t)((nang + 15) / 16) * ((cmom + 3) / 4) * 64 + (size_t)((nang + 3) / 4) * 64;
   const int rsz = fp32 ? 4 : 8;
-  return sizeof(double) * tables + kWGW * seq_wave_bytes(rs, rsz, exact ? 8 : rsz);
+  return sizeof(double) * tables + (size_t)kWGW * ngw * seq_wave_bytes(rs, rsz, exact ? 8 : rsz);
 }
 }  // namespace
 
@@ -936,11 +1017,38 @@ static SeqKernel seq_kernel_batch(int cmom) {
   }
 }
 
+// Two groups per wavefront: fused FP64 or FP32, dense or batch.
+template <typename Real, bool BATCH>
+static SeqKernel seq_kernel_pair(int cmom) {
+  switch (cmom) {
+    case 1: return k_sweep_seq<Real, false, true, 1, BATCH, false, true>;
+    case 4: return k_sweep_seq<Real, false, true, 4, BATCH, false, true>;
+    case 9: return k_sweep_seq<Real, false, true, 9, BATCH, false, true>;
+    case 16: return k_sweep_seq<Real, false, true, 16, BATCH, false, true>;
+    default: return nullptr;
+  }
+}
+
 int sweep_seq_streams(const SweepLaunch &L) {
   const int ncorner = L.ng_job > 0 ? L.nsweeps / L.ng_job : 0;
   return (L.side && L.fork && L.join && ncorner * L.ng_job == L.nsweeps && ncorner % 2 == 0)
              ? 2
              : 1;
+}
+
+// Two groups per wavefront (lanes 0-31 and 32-63): asked for by the caller
+// and taken for fused sweeps that form dinv, of up to 32 angles, with group
+// pairs inside every stream's part of the sweeps, on full 4 x 4 tiles (x and
+// y extents multiples of 4: the chunk's, or every batch problem's)
+bool sweep_seq_pair(const SweepLaunch &L) {
+  const SweepPlan &p = *L.plan;
+  if (!L.pair || !p.seq || L.general || L.exact || L.td || !L.recompute_dinv || p.nang > 32)
+    return false;
+  if (L.ng_job < 2 || L.ng_job % 2 != 0 || (L.nsweeps / sweep_seq_streams(L)) % L.ng_job != 0)
+    return false;
+  if (L.batch ? !L.batch->full_xy : (p.ext.nx % kSX != 0 || p.ext.ny % kSY != 0)) return false;
+  // the second group's face offset is a 32-bit lane value
+  return p.face_doubles_per_sweep() <= (size_t)INT32_MAX / 2;
 }
 
 int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) {
@@ -1008,14 +1116,21 @@ int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) 
     A.prec_want = L.batch->prec_want;
     A.total_cells = L.batch->total_cells;
   }
-  const SeqKernel k = L.general ? seq_kernel_gen(p.cmom)
-                   : L.batch  ? (L.fp32 ? seq_kernel_batch<float>(p.cmom)
-                                        : seq_kernel_batch<double>(p.cmom))
-                   : L.fp32 ? seq_kernel<float, false>(p.cmom, L.recompute_dinv)
-                   : L.exact ? seq_kernel<double, true>(p.cmom, L.recompute_dinv)
-                             : seq_kernel<double, false>(p.cmom, L.recompute_dinv);
+  const int nstr = sweep_seq_streams(L);
+  const int part = L.nsweeps / nstr;
+  const int ngw = sweep_seq_pair(L) ? 2 : 1;
+  const SeqKernel k =
+      ngw == 2 ? (L.batch ? (L.fp32 ? seq_kernel_pair<float, true>(p.cmom)
+                                    : seq_kernel_pair<double, true>(p.cmom))
+                          : (L.fp32 ? seq_kernel_pair<float, false>(p.cmom)
+                                    : seq_kernel_pair<double, false>(p.cmom)))
+      : L.general ? seq_kernel_gen(p.cmom)
+      : L.batch   ? (L.fp32 ? seq_kernel_batch<float>(p.cmom) : seq_kernel_batch<double>(p.cmom))
+      : L.fp32    ? seq_kernel<float, false>(p.cmom, L.recompute_dinv)
+      : L.exact   ? seq_kernel<double, true>(p.cmom, L.recompute_dinv)
+                  : seq_kernel<double, false>(p.cmom, L.recompute_dinv);
   if (!k) return fail(SNAP_ERR_UNSUPPORTED, "no one-wavefront tile sweep for %d moments", p.cmom);
-  const size_t lds = seq_lds_bytes(p.nang, p.cmom, L.fp32, L.exact && !L.fp32);
+  const size_t lds = seq_lds_bytes(p.nang, p.cmom, L.fp32, L.exact && !L.fp32, ngw);
   SNAP_TRY(ensure_dynamic_lds((const void *)k, lds));
   const int nplanes = L.batch ? L.batch->nplanes : p.nplanes;
   const std::vector<int> &poff = L.batch ? L.batch->plane_off : p.plane_off;
@@ -1025,8 +1140,6 @@ int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) 
   // launch overlaps the other's; measured 5% off the 64^3 FP64 solve (more
   // streams were slower).  The side stream starts after, and the caller's
   // stream resumes after, everything enqueued on the other.
-  const int nstr = sweep_seq_streams(L);
-  const int part = L.nsweeps / nstr;
   const hipStream_t ss[2] = {st, nstr > 1 ? L.side : st};
   // the drop-in form's ghost faces (the drop-in runs on one stream)
   const Acc2 *ghosts[3] = {&L.gx, &L.gy, &L.gz};
@@ -1053,8 +1166,8 @@ int sweep_seq_enqueue(const SweepLaunch &L, hipStream_t st, hipEvent_t *events) 
       A.s0 = q * part;
       hipEvent_t *ev = events ? events + 2 * ((size_t)D * nstr + q) : nullptr;
       if (ev) SNAP_HIP_TRY(hipEventRecord(ev[0], ss[q]));
-      hipLaunchKernelGGL(k, dim3((count + kWGW - 1) / kWGW, part), dim3(64 * kWGW), lds, ss[q], A,
-                         poff[D], count);
+      hipLaunchKernelGGL(k, dim3((count + kWGW - 1) / kWGW, part / ngw), dim3(64 * kWGW), lds,
+                         ss[q], A, poff[D], count);
       if (ev) SNAP_HIP_TRY(hipEventRecord(ev[1], ss[q]));
     }
   }

@@ -83,6 +83,7 @@ class SolveOptions(ctypes.Structure):
         ("dinv_mode", ctypes.c_int), ("angles_per_lane", ctypes.c_int), ("refine", ctypes.c_int),
         ("outer_ordered", ctypes.c_int), ("wide_sweep", ctypes.c_int),
         ("sweep_kernel", ctypes.c_int), ("fp64_polish", ctypes.c_double),
+        ("groups_per_wave", ctypes.c_int),
     ]
 
 
@@ -90,7 +91,7 @@ class BatchOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("tile", ctypes.c_int * 3),
                 ("waves_per_tile", ctypes.c_int), ("angles_per_lane", ctypes.c_int),
                 ("use_graph", ctypes.c_int), ("fp64_polish", ctypes.c_double),
-                ("sweep_kernel", ctypes.c_int)]
+                ("sweep_kernel", ctypes.c_int), ("groups_per_wave", ctypes.c_int)]
 
 
 class ProblemStats(ctypes.Structure):
@@ -104,7 +105,7 @@ class BatchStats(ctypes.Structure):
                 ("solve_ms", ctypes.c_double), ("units", ctypes.c_double),
                 ("grind_ns", ctypes.c_double), ("tile", ctypes.c_int * 3), ("waves", ctypes.c_int),
                 ("angles_per_lane", ctypes.c_int), ("autotune_ms", ctypes.c_double),
-                ("plan_cached", ctypes.c_int)]
+                ("plan_cached", ctypes.c_int), ("groups_per_wave", ctypes.c_int)]
 
 
 class IterationRecord(ctypes.Structure):
@@ -119,7 +120,7 @@ class SolveStats(ctypes.Structure):
         ("sweeps_fp32", ctypes.c_int),
         ("sweep_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
         ("units_per_sweep", ctypes.c_double), ("grind_ns", ctypes.c_double),
-        ("time_steps", ctypes.c_int),
+        ("time_steps", ctypes.c_int), ("groups_per_wave", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -17,12 +17,16 @@ from .solver import _PREC, MIXED_FP64_POLISH, _ptr, problem_struct
 class Batch:
     def __init__(self, problems: List[Problem], precision: str = "fp64", tile=(0, 0, 0),
                  waves_per_tile: int = 0, angles_per_lane: int = 0, use_graph: bool = True,
-                 fp64_polish: Optional[float] = None, sweep_kernel: str = "auto"):
+                 fp64_polish: Optional[float] = None, sweep_kernel: str = "auto",
+                 groups_per_wave: int = 0):
         """fp64_polish (mixed): a problem's next inner sweep runs in FP64 once
         its previous inner of the same outer changed its flux by at most
         fp64_polish * epsi; None = the solver's default, 0 = FP32 throughout.
         sweep_kernel: "auto" (autotuned), "tiled" or "wave_tile" (one wavefront
-        per 4 x 4 x TZ tile; tile[2] sets TZ)."""
+        per 4 x 4 x TZ tile; tile[2] sets TZ).
+        groups_per_wave: 0 = two groups per wavefront on one-wavefront tiles
+        where that form applies (every problem's x and y extents multiples of
+        4), 1 = one."""
         self.problems = [q.contiguous() for q in problems]
         for q in self.problems:
             q.validate()
@@ -40,6 +44,7 @@ class Batch:
             fp64_polish = MIXED_FP64_POLISH if precision == "mixed" else 0.0
         o.fp64_polish = float(fp64_polish)
         o.sweep_kernel = {"auto": 0, "tiled": 1, "wave_tile": 2}[sweep_kernel]
+        o.groups_per_wave = groups_per_wave
         h = ctypes.c_void_p()
         _lib.check(self.lib.snap_hip_batch_create(arr, len(self.problems), ctypes.byref(o),
                                                   ctypes.byref(h)), "snap_hip_batch_create")

@@ -79,6 +79,9 @@ class Solver:
     default for mixed precision is MIXED_REFINE_FP64_POLISH with defect
     correction and MIXED_FP64_POLISH without; 0 sweeps every inner after the
     first in FP32.
+    groups_per_wave (one-wavefront tiles): 0 = two groups per wavefront
+    where that form applies (fused sweeps of up to 32 angles, an even group
+    count, x and y extents multiples of 4; bit-equal results), 1 = one.
     """
 
     def __init__(self, problem: Problem, precision: str = FP64, tile=(0, 0, 0),
@@ -86,7 +89,8 @@ class Solver:
                  dinv: str = "default", angles_per_lane: int = 0,
                  refine: Optional[bool] = None,
                  outer_ordered: bool = False, wide_sweep: bool = False,
-                 sweep_kernel: str = "auto", fp64_polish: Optional[float] = None):
+                 sweep_kernel: str = "auto", fp64_polish: Optional[float] = None,
+                 groups_per_wave: int = 0):
         problem.validate()
         self.problem = p = problem.contiguous()
         self.lib = _lib.load()
@@ -109,6 +113,7 @@ class Solver:
             fp64_polish = (0.0 if precision != MIXED else
                            MIXED_REFINE_FP64_POLISH if refine else MIXED_FP64_POLISH)
         o.fp64_polish = float(fp64_polish)
+        o.groups_per_wave = groups_per_wave
         self.precision = precision
         h = ctypes.c_void_p()
         _lib.check(self.lib.snap_hip_solver_create(ctypes.byref(s), ctypes.byref(o), ctypes.byref(h)),
