@@ -476,6 +476,7 @@ def main() -> int:
                 "time_to_convergence_p50_ms": float(np.percentile(times, 50)),
                 "time_to_convergence_p99_ms": float(np.percentile(times, 99)),
                 "tile": list(st["tile"]), "angles_per_lane": st["angles_per_lane"],
+                "groups_per_wave": st["groups_per_wave"],
                 "autotune_s": st0["autotune_ms"] / 1e3,
                 "second_batch_autotune_s": st2["autotune_ms"] / 1e3,
                 "second_batch_plan_cached": st2["plan_cached"]}
