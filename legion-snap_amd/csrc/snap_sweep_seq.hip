@@ -37,7 +37,9 @@
 // BATCH (many problems per launch) and GEN (the drop-in group range: the
 // caller's accessors, moments added by atomics, ghost faces staged through
 // the tile-face slots by k_seq_ghosts).  The host may split a launch's
-// sweeps over two streams (sweep offset s0).
+// sweeps over two streams (sweep offset s0).  Up to 32 angles, the dense and
+// BATCH forms can run two groups per wavefront (PAIR: lanes 0-31 and 32-63),
+// which fills the lanes that one group of 24 angles leaves idle.
 #include <algorithm>
 #include <cstring>
 #include <type_traits>
@@ -422,9 +424,15 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
         const bool nxn = p2 - pi_ < Real(0);
         const bool nyn = p2 - pj_ < Real(0);
         const bool nzn = p2 - pk_ < Real(0);
-        hx = hx & !(act & nxn);
-        hy = hy & !(act & nyn);
-        hz = hz & !(act & nzn);
+        if constexpr (PAIR) {
+          hx = hx & !(act & nxn);
+          hy = hy & !(act & nyn);
+          hz = hz & !(act & nzn);
+        } else {
+          hx = hx & !nxn;
+          hy = hy & !nyn;
+          hz = hz & !nzn;
+        }
         return popc(__builtin_amdgcn_ballot_w64(aok & nxn)) +
                popc(__builtin_amdgcn_ballot_w64(aok & nyn)) +
                popc(__builtin_amdgcn_ballot_w64(aok & nzn));
