@@ -31,6 +31,13 @@
 
 namespace snap {
 
+// The adjoint's tile edge (x, y and z).  On the 48^3, 24-angle, 16-group
+// gradient (FP64 / FP32 reverse sweeps, seconds): 4x4x4 0.137 / 0.114,
+// 8x4x2 0.131 / 0.109, 4x8x2 0.134 / 0.111, 4x4x8 0.138 / 0.110, 4x4x2
+// 0.150 / 0.138, 8x4x4 0.152 / 0.115, 8x8x2 0.168 / 0.129, 8x8x4 (the forward
+// solve's) 0.32 / 0.17, 2x2x2 0.30 / 0.28.
+constexpr int kAdjTile = 4;
+
 struct DeviceBuffer {
   void *p = nullptr;
   size_t bytes = 0;
@@ -44,6 +51,17 @@ struct DeviceBuffer {
       return fail(SNAP_ERR_NO_MEMORY, "device allocation of %zu bytes failed", bytes);
     }
     return SNAP_OK;
+  }
+  // at least n bytes (the old contents are not kept; the stream's work on
+  // them is finished first)
+  int grow(size_t n, hipStream_t st) {
+    if (p && bytes >= n) return SNAP_OK;
+    if (p) {
+      SNAP_HIP_TRY(hipStreamSynchronize(st));
+      SNAP_HIP_TRY(hipFree(p));
+      p = nullptr;
+    }
+    return alloc(n);
   }
   template <typename T>
   T *as() const {
@@ -68,6 +86,7 @@ struct snap_solver {
   // one-wavefront tile sweep (steady problems up to 64 angles): used for
   // every inner sweep when present; the adjoint keeps the multi-wave plan
   SweepPlan plan_seq;
+  SweepPlan plan_adj;  // the adjoint's tiles (and its checkpoint sweep's), made at the first gradient
   hipStream_t stream = nullptr;
   // the one-wavefront tile sweep's second stream and its fork / join events
   hipStream_t side = nullptr;
@@ -101,6 +120,7 @@ struct snap_solver {
     plan_free(&plan);
     plan_free(&plan64);
     plan_free(&plan_seq);
+    plan_free(&plan_adj);
     quad_free(&quad);
     for (auto &x : ev)
       if (x) (void)hipEventDestroy(x);
@@ -133,14 +153,14 @@ struct snap_solver {
   // streams an inner sweep's launches use (launches per tile diagonal)
   int sweep_streams() const { return (plan_seq.nplanes && side) ? 2 : 1; }
 
-  // tiled_only: the multi-wave tile plan even when the one-wavefront plan
-  // exists (the adjoint's checkpoint faces are in that plan's layout)
+  // tiled: a multi-wave tile plan to sweep on instead of the solver's plans
+  // (the adjoint's checkpoint faces are in its own plan's layout)
   int sweep(int precision, hipEvent_t *events, bool fixup = true, bool recompute = false,
-            bool tiled_only = false, const int32_t *skip = nullptr) {
+            const SweepPlan *tiled = nullptr, const int32_t *skip = nullptr) {
     const bool fp32 = precision == SNAP_PRECISION_MIXED;
     SweepLaunch L;
     memset(&L, 0, sizeof(L));
-    L.plan = &plan_for(precision);
+    L.plan = tiled ? tiled : &plan_for(precision);
     L.quad = &quad;
     L.nsweeps = 8 * ng;
     L.ng_job = ng;
@@ -162,7 +182,7 @@ struct snap_solver {
     // the one-wavefront tile sweep forms dinv from t_xs unless told to
     // stream it (measured faster: it saves 8 bytes per unit of HBM traffic)
     L.recompute_dinv = recompute || opt.dinv_mode == SNAP_DINV_RECOMPUTE ||
-                       (opt.dinv_mode == SNAP_DINV_DEFAULT && plan_seq.nplanes && !tiled_only);
+                       (opt.dinv_mode == SNAP_DINV_DEFAULT && plan_seq.nplanes && !tiled);
     if (fp32 && !L.recompute_dinv && !dinv32.p) return fail(SNAP_ERR_INVALID, "no FP32 dinv");
     L.td = td;
     if (td) {
@@ -173,7 +193,7 @@ struct snap_solver {
       }
       L.qim = prob.mms ? qim.as<double>() : nullptr;
     }
-    if (plan_seq.nplanes && !tiled_only) {
+    if (plan_seq.nplanes && !tiled) {
       L.plan = &plan_seq;
       L.side = side;
       L.fork = fork_ev;
@@ -492,7 +512,7 @@ extern "C" int snap_hip_solver_run(snap_solver_t *s, snap_solve_stats_t *stats) 
           SNAP_TRY(launch_copy(s->flux0pi.as<double>(), s->flux0.as<double>(), (int64_t)gc, stream,
                                skip));
           SNAP_HIP_TRY(hipEventRecord(s->ra_ev[sl][0], stream));
-          SNAP_TRY(s->sweep(s->opt.precision, nullptr, true, false, false, skip));
+          SNAP_TRY(s->sweep(s->opt.precision, nullptr, true, false, nullptr, skip));
           // the octant sum is part of the sweep: the reference accumulates
           // the flux moments inside its sweep kernel (gpu_sweep.cu:1680-1733)
           SNAP_TRY(s->reduce_octants(false, skip));
@@ -689,14 +709,30 @@ extern "C" int snap_hip_solver_gradient(snap_solver_t *s, const double *resp, in
   if (!(tol > 0.0) || maxit < 1) return fail(SNAP_ERR_INVALID, "bad tolerance or iteration cap");
   const bool fp32 = precision == SNAP_PRECISION_MIXED;
   // the checkpoint and the reverse kernel's forward recompute are FP64 (the
-  // fixup's decisions of the FP64 solve); `fp32` sets the reverse recurrence
-  const SweepPlan &pl = s->plan_for(SNAP_PRECISION_FP64);
+  // fixup's decisions of the FP64 solve); `fp32` sets the reverse recurrence.
+  // The reverse sweep takes 4 x 4 x 4 tiles of its own (kAdjTile: 0.14 s
+  // against 0.32 s on the forward solve's 8 x 8 x 4 tiles, the same gradient
+  // to the last bit); the forward plan is the fallback where that tile does
+  // not fit.
+  const hipStream_t st = s->stream;
+  if (!s->plan_adj.nplanes) {
+    int tx = std::min(kAdjTile, s->e.nx), ty = std::min(kAdjTile, s->e.ny),
+        tz = std::min(kAdjTile, s->e.nz);
+    int w = 0, a = 0, cpw, waves, sb;
+    plan_choose(s->e, s->nang, s->cmom, 8, &tx, &ty, &tz, &w, &a, false);
+    if (plan_create(&s->plan_adj, s->e, s->nang, s->cmom, tx, ty, tz, w, a, false) != SNAP_OK ||
+        adjoint_plan_fits(s->plan_adj, s->nang, 8, &cpw, &waves, &sb) != SNAP_OK)
+      plan_free(&s->plan_adj);
+  }
+  const SweepPlan &pl = s->plan_adj.nplanes ? s->plan_adj : s->plan_for(SNAP_PRECISION_FP64);
   {
     int cpw, waves, sb;
     SNAP_TRY(adjoint_plan_fits(pl, s->nang, fp32 ? 4 : 8, &cpw, &waves, &sb));
   }
   const size_t gc = (size_t)s->ng * s->cells;
-  const hipStream_t st = s->stream;
+  // checkpoint and adjoint faces in the plan's layout, every (octant, group)
+  const size_t face_bytes = sizeof(double) * 8 * s->ng * pl.face_doubles_per_sweep();
+  SNAP_TRY(s->faces.grow(face_bytes, st));
   DeviceBuffer r, zeta, zn, u, gsum, norms;
   SNAP_TRY(r.alloc(sizeof(double) * gc));
   SNAP_TRY(zeta.alloc(sizeof(double) * gc));
@@ -705,13 +741,13 @@ extern "C" int snap_hip_solver_gradient(snap_solver_t *s, const double *resp, in
   SNAP_TRY(gsum.alloc(sizeof(double) * gc));
   SNAP_TRY(norms.alloc(2 * sizeof(unsigned long long)));
   if (!s->part_grad.p) SNAP_TRY(s->part_grad.alloc(sizeof(double) * 8 * gc));
-  if (!s->adj_faces.p) SNAP_TRY(s->adj_faces.alloc(s->faces.bytes));
+  SNAP_TRY(s->adj_faces.grow(face_bytes, st));
   SNAP_HIP_TRY(hipMemcpyAsync(r.p, resp, sizeof(double) * gc, hipMemcpyHostToDevice, st));
   // forward source of the current flux, q = qi + S flux0, and one forward
   // sweep of it: its tile faces are the checkpoint the reverse sweeps start from
   SNAP_TRY(s->outer_source());
   SNAP_TRY(s->inner_source());
-  SNAP_TRY(s->sweep(SNAP_PRECISION_FP64, nullptr, true, true, true));
+  SNAP_TRY(s->sweep(SNAP_PRECISION_FP64, nullptr, true, true, &pl));
   SNAP_HIP_TRY(hipMemcpyAsync(zeta.p, r.p, sizeof(double) * gc, hipMemcpyDeviceToDevice, st));
   auto reverse = [&](const DeviceBuffer &z) -> int {
     SNAP_TRY(adjoint_enqueue(pl, s->quad, s->ng, s->prob.fixup ? 1 : 0, s->hi, s->hj, s->hk,
