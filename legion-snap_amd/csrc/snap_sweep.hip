@@ -173,11 +173,14 @@ void plan_choose(Extent3 e, int nang, int cmom, int real_bytes, int *TX, int *TY
   const int ms = max_steps(*apl), wmax = max_threads(*apl, td) / 64;
   if (*TX <= 0 || *TY <= 0 || *TZ <= 0) {
     // largest candidate tile whose biggest cell diagonal the workgroup
-    // covers and whose LDS fits; measured best first (8x8x4 at FP64, 8x8x8
-    // with FP32 faces and two angles per lane)
+    // covers and whose LDS fits; measured best first: 8x8x8 with FP32 faces
+    // and at FP64 up to 24 angles, else 8x8x4 (ms per FP64 sweep, 8x8x8
+    // against 8x8x4: 48^3 16 groups 24 angles 4.13 / 5.18, 48^3 16 g 16 a
+    // 3.78 / 4.88, 64^3 8 g 24 a 6.81 / 8.15, but 32^3 16 g 32 a 2.91 / 2.47
+    // and 64^3 8 g 36 a 12.5 / 10.8; FP32 8x8x8 ahead on all five)
     static const int cand[][3] = {{8, 8, 8}, {8, 8, 4}, {8, 4, 4}, {4, 4, 4}, {4, 4, 2},
                                   {4, 2, 2}, {2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
-    const int first = (real_bytes == 4 && *apl == 2) ? 0 : 1;
+    const int first = (real_bytes == 4 || nang <= 24) ? 0 : 1;
     for (int c = first; c < (int)(sizeof(cand) / sizeof(cand[0])); c++) {
       const int tx = std::min(cand[c][0], e.nx), ty = std::min(cand[c][1], e.ny),
                 tz = std::min(cand[c][2], e.nz);
