@@ -18,6 +18,8 @@ CASES = {
     "config3_small": lambda: problems.config3(n=16, ng=4, nang=24),
     "p3_nang32": lambda: problems.config1(n=16, ng=6, nang=32, nmom=4),
     "p2_nang10": lambda: problems.config1(n=24, ng=4, nang=10, nmom=3),
+    # 20 cells deep: the last tile of each column is 4 of its 8 planes
+    "ragged_z": lambda: problems.config1(n=20, ng=4, nang=12, nmom=2),
 }
 
 
