@@ -694,8 +694,9 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
         for (int n = 0; n < kNP; n++) Zb[zslot(n)] = zf[n];
       }
       bool redo = predict_fixup;
+      int r0 = 0;  // the first row the cell-by-cell pass sweeps again
       if (!predict_fixup) {
-        bool neg = false;
+        bool neg[kSY] = {};  // per row of the plane
 #pragma unroll
         for (int dg2 = 0; dg2 < kSX + kSY - 1; dg2++) {
 #pragma unroll
@@ -718,7 +719,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
             const Real pc = p * d;
             const Real p2 = Real(2) * pc;
             const Real ox = p2 - pi_, oy = p2 - pj_, oz = p2 - pk_;
-            neg = neg | (ox < Real(0)) | (oy < Real(0)) | (oz < Real(0));
+            neg[j] = neg[j] | (ox < Real(0)) | (oy < Real(0)) | (oz < Real(0));
             xf[j] = ox;
             yf[i] = oy;
             zf[n] = oz;
@@ -729,7 +730,13 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
           }
           __builtin_amdgcn_sched_barrier(0);
         }
-        redo = A.fixup && __builtin_amdgcn_ballot_w64(aok & neg) != 0;
+        // the rows above the first one with a negative outflow in any lane
+        // are final: no cell of theirs is fixed or downstream of a fixed cell
+        r0 = kSY;
+#pragma unroll
+        for (int j = kSY - 1; j >= 0; j--)
+          if (__builtin_amdgcn_ballot_w64(aok & neg[j]) != 0) r0 = j;
+        redo = A.fixup && r0 < kSY;
       }
       // (3) a negative outflow anywhere in the plane: sweep it again cell by
       // cell with the fixup, from the plane's z inflows (zf) and tile inflows
@@ -741,11 +748,31 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
         // backup and each z outflow overwrites its own, now consumed, backup
         // slot; the tile inflows are picked from their registers by uniform
         // selects and dinv is formed again.
+        // It starts at row r0: the y inflows of that row are the branch-free
+        // y outflows of the row above, formed again from the rows' pc
+        // exactly as that pass formed them (pc read from angle a's slot,
+        // which lanes past nang share with lane 0).
         Real xc = Real(0), yq0 = Real(0), yq1 = Real(0), yq2 = Real(0), yq3 = Real(0);
-        // each cell's LDS operands are read one cell ahead
-        Real psi_nx = P[a], pk_nx = Zb[zslot(0)];
+        if (r0 > 0) {
+          wave_sync();
+          Real yo[kSX];
+#pragma unroll
+          for (int i = 0; i < kSX; i++) yo[i] = yin[i];
 #pragma unroll 1
-        for (int n = 0; n < kNP; n++) {
+          for (int j = 0; j < r0; j++) {
+#pragma unroll
+            for (int i = 0; i < kSX; i++) yo[i] = Real(2) * (Real)W[(j * kSX + i) * RS + a] - yo[i];
+          }
+          yq0 = yo[0];
+          yq1 = yo[1];
+          yq2 = yo[2];
+          yq3 = yo[3];
+        }
+        const int n0 = kSX * r0;
+        // each cell's LDS operands are read one cell ahead
+        Real psi_nx = P[n0 * RS + a], pk_nx = Zb[zslot(n0)];
+#pragma unroll 1
+        for (int n = n0; n < kNP; n++) {
           const int i = n & 3, j = n >> 2;
           const int c = T.c0 + i * T.sx + j * T.sy + k * T.sz;
           const Real psi = psi_nx, pk_ = pk_nx;
@@ -778,7 +805,8 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
           W[wslot(n)] = w_of(pc);
         }
 #pragma unroll
-        for (int n = 0; n < kNP; n++) zf[n] = Zb[zslot(n)];
+        for (int n = 0; n < kNP; n++)
+          if (n >= n0) zf[n] = Zb[zslot(n)];
         // (PAIR: either group's fixup sends both to the cell-by-cell pass)
         if constexpr (PAIR) predict_fixup = __builtin_amdgcn_ballot_w64(predict_fixup) != 0;
       }
