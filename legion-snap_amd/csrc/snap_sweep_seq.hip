@@ -54,6 +54,9 @@ constexpr int kSX = 4, kSY = 4, kNP = kSX * kSY;  // tile cross-section, cells p
 #ifndef SEQ_WAVES
 #define SEQ_WAVES 2  // waves per SIMD the register allocation aims for
 #endif
+// (FP32 sweeps of two groups per wavefront aim for 3: at P3 the batch form
+// then spills 5 registers and still runs the 48-problem mixed batch 2.5%
+// faster than at 2 waves per SIMD)
 // Wavefronts per workgroup: each takes its own tile of the same sweep, so
 // they share the sweep's matrix-core operand tables in LDS.  Two, not four:
 // a workgroup's LDS is freed only when its slowest tile (the one with the
@@ -181,7 +184,8 @@ struct TileGeom {
 // negative.
 template <typename Real, bool EXACT, bool REC, int CMOM, bool BATCH = false, bool GEN = false,
           bool PAIR = false>
-__global__ void __launch_bounds__(64 * kWGW) __attribute__((amdgpu_waves_per_eu(SEQ_WAVES, 8)))
+__global__ void __launch_bounds__(64 * kWGW)
+__attribute__((amdgpu_waves_per_eu((PAIR && !std::is_same<Real, double>::value) ? 3 : SEQ_WAVES, 8)))
 k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   constexpr bool kFP64 = std::is_same<Real, double>::value;
   static_assert(!GEN || (kFP64 && !EXACT && REC && !BATCH), "GEN is the fused FP64 drop-in form");
