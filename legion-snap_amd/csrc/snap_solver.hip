@@ -303,6 +303,9 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
   SNAP_TRY(quad_upload(&s->quad, ec.data(), mu.data(), eta.data(), xi.data(), w.data(), p->nang, p->nmom));
   int TX = s->opt.tile[0], TY = s->opt.tile[1], TZ = s->opt.tile[2], waves = s->opt.waves_per_tile;
   int apl = s->opt.angles_per_lane;
+  if (s->opt.groups_per_wave != 0 && s->opt.groups_per_wave != 1)
+    return fail(SNAP_ERR_INVALID, "groups_per_wave must be 0 (choose) or 1, got %d",
+                s->opt.groups_per_wave);
   // more angles than a wavefront's lanes hold twice over: one workgroup per cell
   const bool wide = s->nang > 128 || s->opt.wide_sweep;
   if (wide) {
@@ -318,9 +321,6 @@ extern "C" int snap_hip_solver_create(const snap_problem_t *p, const snap_solve_
                   "the one-wavefront tile sweep takes steady problems of up to 64 angles");
     if (s->opt.sweep_kernel < SNAP_SWEEP_AUTO || s->opt.sweep_kernel > SNAP_SWEEP_WAVE_TILE)
       return fail(SNAP_ERR_INVALID, "unknown sweep kernel %d", s->opt.sweep_kernel);
-    if (s->opt.groups_per_wave != 0 && s->opt.groups_per_wave != 1)
-      return fail(SNAP_ERR_INVALID, "groups_per_wave must be 0 (choose) or 1, got %d",
-                  s->opt.groups_per_wave);
     // options that only the multi-wave tile sweep has keep AUTO on it
     const bool tiled_opts = s->opt.angles_per_lane != 0 || s->opt.waves_per_tile != 0 ||
                             s->opt.tile[0] != 0 || s->opt.tile[1] != 0;

@@ -82,3 +82,12 @@ def test_batch_pair_bit_equal_to_one_group(gpu, precision, polish):
             (b_["inner_total"], b_["outer_total"], b_["sweeps_fp64"])
     for (fa, fma), (fb, fmb) in zip(fl2, fl1):
         assert np.array_equal(fa, fb) and np.array_equal(fma, fmb)
+
+
+def test_groups_per_wave_option_checked(gpu):
+    from snap_amd._lib import SnapError
+    p = problems.config1(n=8, ng=2, nang=12, nmom=1)
+    with pytest.raises(SnapError, match="groups_per_wave"):
+        Solver(p, precision="fp64", sweep_kernel="wave_tile", groups_per_wave=2)
+    with pytest.raises(SnapError, match="groups_per_wave"):
+        Batch([p], precision="fp64", sweep_kernel="wave_tile", groups_per_wave=-1)
