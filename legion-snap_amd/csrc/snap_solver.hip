@@ -31,12 +31,12 @@
 
 namespace snap {
 
-// The adjoint's tile edge (x, y and z).  On the 48^3, 24-angle, 16-group
-// gradient (FP64 / FP32 reverse sweeps, seconds): 4x4x4 0.137 / 0.114,
-// 8x4x2 0.131 / 0.109, 4x8x2 0.134 / 0.111, 4x4x8 0.138 / 0.110, 4x4x2
-// 0.150 / 0.138, 8x4x4 0.152 / 0.115, 8x8x2 0.168 / 0.129, 8x8x4 (the forward
-// solve's) 0.32 / 0.17, 2x2x2 0.30 / 0.28.
-constexpr int kAdjTile = 4;
+// The adjoint's tile edges (x, y, z).  On the 48^3, 24-angle, 16-group
+// gradient (FP64 / FP32 reverse sweeps, seconds): 8x4x2 0.131 / 0.109
+// (three runs against 4x4x4's 0.137 / 0.114 on one box), 4x8x2 0.134 /
+// 0.111, 4x4x8 0.138 / 0.110, 4x4x2 0.150 / 0.138, 8x4x4 0.152 / 0.115, 8x8x2
+// 0.168 / 0.129, 8x8x4 (the forward solve's) 0.32 / 0.17, 2x2x2 0.30 / 0.28.
+constexpr int kAdjTile[3] = {8, 4, 2};
 
 struct DeviceBuffer {
   void *p = nullptr;
@@ -710,14 +710,14 @@ extern "C" int snap_hip_solver_gradient(snap_solver_t *s, const double *resp, in
   const bool fp32 = precision == SNAP_PRECISION_MIXED;
   // the checkpoint and the reverse kernel's forward recompute are FP64 (the
   // fixup's decisions of the FP64 solve); `fp32` sets the reverse recurrence.
-  // The reverse sweep takes 4 x 4 x 4 tiles of its own (kAdjTile: 0.14 s
+  // The reverse sweep takes 8 x 4 x 2 tiles of its own (kAdjTile: 0.13 s
   // against 0.32 s on the forward solve's 8 x 8 x 4 tiles, the same gradient
   // to the last bit); the forward plan is the fallback where that tile does
   // not fit.
   const hipStream_t st = s->stream;
   if (!s->plan_adj.nplanes) {
-    int tx = std::min(kAdjTile, s->e.nx), ty = std::min(kAdjTile, s->e.ny),
-        tz = std::min(kAdjTile, s->e.nz);
+    int tx = std::min(kAdjTile[0], s->e.nx), ty = std::min(kAdjTile[1], s->e.ny),
+        tz = std::min(kAdjTile[2], s->e.nz);
     int w = 0, a = 0, cpw, waves, sb;
     plan_choose(s->e, s->nang, s->cmom, 8, &tx, &ty, &tz, &w, &a, false);
     if (plan_create(&s->plan_adj, s->e, s->nang, s->cmom, tx, ty, tz, w, a, false) != SNAP_OK ||
