@@ -23,11 +23,24 @@ CASES = {
 }
 
 
-def _solve(prob, precision, gpw):
-    with Solver(prob, precision=precision, sweep_kernel="wave_tile", groups_per_wave=gpw) as s:
+def _solve(prob, precision, gpw, **kw):
+    with Solver(prob, precision=precision, sweep_kernel="wave_tile", groups_per_wave=gpw,
+                **kw) as s:
         st = s.run()
         f, fm = s.flux()
     return st, f, fm
+
+
+@pytest.mark.parametrize("name", ["config3_small", "p3_nang32"])
+def test_pair_defect_correction_bit_equal(gpu, name):
+    # defect correction sweeps FP32 corrections without the fixup
+    s2, f2, fm2 = _solve(CASES[name](), "mixed", 0, refine=True)
+    s1, f1, fm1 = _solve(CASES[name](), "mixed", 1, refine=True)
+    assert s2["groups_per_wave"] == 2 and s1["groups_per_wave"] == 1
+    assert (s2["inner_total"], s2["sweeps_fp64"]) == (s1["inner_total"], s1["sweeps_fp64"])
+    assert np.array_equal(f2, f1)
+    if fm1 is not None:
+        assert np.array_equal(fm2, fm1)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
