@@ -71,14 +71,14 @@ constexpr int kWGW = 2;
 // sources, or its 16 x 16 source moments on the general path, in the sweep
 // type) and Zb (the plane's z inflows, in the sweep type).  An FP32 sweep
 // thus takes half the per-wavefront LDS of an FP64 one.
-__host__ __device__ constexpr size_t seq_w_bytes(int rs, int wsz) {
-  return (size_t)wsz * ((size_t)kNP * rs + 64);
+__host__ __device__ constexpr size_t seq_w_bytes(int rs, int wsz, bool junk = true) {
+  return (size_t)wsz * ((size_t)kNP * rs + (junk ? 64 : 0));
 }
 __host__ __device__ constexpr size_t seq_p_bytes(int rs, int rsz) {
   return (size_t)rsz * (kNP * rs > kNP * 16 ? kNP * rs : kNP * 16);
 }
-__host__ __device__ constexpr size_t seq_wave_bytes(int rs, int rsz, int wsz) {
-  return seq_w_bytes(rs, wsz) + seq_p_bytes(rs, rsz) + (size_t)rsz * kNP * rs;
+__host__ __device__ constexpr size_t seq_wave_bytes(int rs, int rsz, int wsz, bool junk = true) {
+  return seq_w_bytes(rs, wsz, junk) + seq_p_bytes(rs, rsz) + (size_t)rsz * kNP * rs;
 }
 
 // a*b + c with one rounding in the sweep's type
@@ -191,6 +191,10 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   static_assert(!GEN || (kFP64 && !EXACT && REC && !BATCH), "GEN is the fused FP64 drop-in form");
   static_assert(!PAIR || (!EXACT && REC && !GEN), "PAIR is a fused form that forms dinv");
   constexpr int NGW = PAIR ? 2 : 1;  // groups per wavefront
+  // FP64 pairs hold the operand tables in registers (two 16-angle blocks,
+  // eight four-angle steps: up to 32 angles) and no junk W row, so that
+  // four two-wave workgroups fit a CU's LDS at 24 angles
+  constexpr bool kRT = PAIR && kFP64;
   if (A.skip && *A.skip) return;  // the whole workgroup: an inner iteration enqueued ahead
   constexpr int KS = (CMOM + 3) / 4;  // four-moment steps of the source product
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -209,8 +213,8 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   // (the moment product's A, times the angle's weight unless EXACT), zero
   // outside the moments and angles
   double *EB = (double *)smem_raw;
-  double *EA = EB + (size_t)NB * KS * 64;
-  {
+  double *EA = EB + (kRT ? 0 : (size_t)NB * KS * 64);
+  if constexpr (!kRT) {
     const double *ecc0 = A.ec + (size_t)(A.corner0 + s / A.ng_job) * CMOM * nang;
     for (int idx = threadIdx.x; idx < NB * KS * 64; idx += 64 * kWGW) {
       const int L = idx & 63, blk = idx >> 6, kk = blk % KS, nb = blk / KS;
@@ -233,13 +237,15 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
   // sweep it again (fast path)
   using WT = typename std::conditional<EXACT, double, Real>::type;
   // (PAIR: one set of the three tables per group, the lane's own at W, P, Zb)
-  const size_t gby = seq_wave_bytes(RS, sizeof(Real), sizeof(WT));
-  unsigned char *wbase = (unsigned char *)(EA + (size_t)NST * 64) + (size_t)wave * NGW * gby;
+  const size_t gby = seq_wave_bytes(RS, sizeof(Real), sizeof(WT), !PAIR);
+  unsigned char *wbase = (unsigned char *)(EA + (kRT ? 0 : (size_t)NST * 64)) + (size_t)wave * NGW * gby;
   auto W_of = [&](int h) { return (WT *)(wbase + h * gby); };
-  auto P_of = [&](int h) { return (Real *)(wbase + h * gby + seq_w_bytes(RS, sizeof(WT))); };
+  auto P_of = [&](int h) {
+    return (Real *)(wbase + h * gby + seq_w_bytes(RS, sizeof(WT), !PAIR));
+  };
   WT *W = W_of(hh);
   Real *P = P_of(hh);
-  Real *Zb = (Real *)(wbase + hh * gby + seq_w_bytes(RS, sizeof(WT)) +
+  Real *Zb = (Real *)(wbase + hh * gby + seq_w_bytes(RS, sizeof(WT), !PAIR) +
                       seq_p_bytes(RS, sizeof(Real)));
   Real *qs = P;
   // the chunk: the launch's own, or one problem of a batch
@@ -366,9 +372,37 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
     return __builtin_fma(r, e, r);
   };
   const int r16 = lane & 15, q4 = lane >> 4;  // matrix-core operand coordinates
+  // the register form of the operand tables (kRT): ebr[nb][kk] is EB's entry
+  // of this lane and ear[st] EA's
+  double ebr[kRT ? 2 : 1][KS], ear[kRT ? 8 : 1];
+  if constexpr (kRT) {
+    const double *ecc0 = A.ec + (size_t)corner * CMOM * nang;
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+      for (int kk = 0; kk < KS; kk++) {
+        const int l = 4 * kk + q4, ang = 16 * nb + r16;
+        ebr[nb][kk] =
+            (l < CMOM && ang < nang) ? ((l == 0) ? 1.0 : ecc0[(size_t)l * nang + ang]) : 0.0;
+      }
+#pragma unroll
+    for (int st = 0; st < 8; st++) {
+      const int l = r16, ang = 4 * st + q4;
+      const double e =
+          (l < CMOM && ang < nang) ? ((l == 0) ? 1.0 : ecc0[(size_t)l * nang + ang]) : 0.0;
+      ear[st] = ang >= nang ? e : e * A.w[ang];
+    }
+  }
   // W slot of the lane in row n: its angle's column, or (lanes past the
   // row) its own slot of a junk row after the 16 rows, which nothing reads
-  auto wslot = [&](int n) { return la < RS ? n * RS + la : kNP * RS + la; };
+  // (PAIR: lanes past the row write their zeros to the row's last slot,
+  // which is past nang and which nothing reads)
+  auto wslot = [&](int n) {
+    if constexpr (PAIR)
+      return n * RS + (la < RS ? la : RS - 1);
+    else
+      return la < RS ? n * RS + la : kNP * RS + la;
+  };
   // t_xs of the lane's group in cell slot n: lane n's (group g) or lane
   // 32 + n's (group g + 1), which hold slot n of their group
   auto tbcast = [&](Real v, int n) -> Real {
@@ -537,6 +571,18 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
       for (int h = 0; h < NGW; h++) {
         const WT *Wh = W_of(h);
         f64x4 acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
+        if constexpr (kRT) {
+#pragma unroll
+          for (int st = 0; st < 8; st++) {
+            if (st >= NST) break;
+            if (st & 1)
+              acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                  ear[st], (double)Wh[r16 * RS + 4 * st + q4], acc2, 0, 0, 0);
+            else
+              acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                  ear[st], (double)Wh[r16 * RS + 4 * st + q4], acc, 0, 0, 0);
+          }
+        } else {
         int st = 0;
         for (; st + 1 < NST; st += 2) {
           acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
@@ -547,6 +593,7 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
         if (st < NST)
           acc = __builtin_amdgcn_mfma_f64_16x16x4f64(
               EA[st * 64 + lane], (double)Wh[r16 * RS + 4 * st + q4], acc, 0, 0, 0);
+        }
         acc += acc2;
         const int n = r16;
         const int c = T.cell(n & 3, n >> 2, k);
@@ -659,16 +706,32 @@ k_sweep_seq(SeqArgs A, int plane_off, int plane_count) {
 #pragma unroll
       for (int h = 0; h < NGW; h++) {
         Real *Ph = P_of(h);
-        for (int nb = 0; nb < NB; nb++) {
-          f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-          const int ang = 16 * nb + r16;
+        if constexpr (kRT) {
 #pragma unroll
-          for (int kk = 0; kk < KS; kk++)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[h][kk], EB[(nb * KS + kk) * 64 + lane],
-                                                       acc, 0, 0, 0);
-          if (ang < RS) {
+          for (int nb = 0; nb < 2; nb++) {
+            if (nb >= NB) break;
+            f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+            const int ang = 16 * nb + r16;
 #pragma unroll
-            for (int r = 0; r < 4; r++) Ph[(4 * r + q4) * RS + ang] = (Real)acc[r];
+            for (int kk = 0; kk < KS; kk++)
+              acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[h][kk], ebr[nb][kk], acc, 0, 0, 0);
+            if (ang < RS) {
+#pragma unroll
+              for (int r = 0; r < 4; r++) Ph[(4 * r + q4) * RS + ang] = (Real)acc[r];
+            }
+          }
+        } else {
+          for (int nb = 0; nb < NB; nb++) {
+            f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+            const int ang = 16 * nb + r16;
+#pragma unroll
+            for (int kk = 0; kk < KS; kk++)
+              acc = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[h][kk], EB[(nb * KS + kk) * 64 + lane],
+                                                         acc, 0, 0, 0);
+            if (ang < RS) {
+#pragma unroll
+              for (int r = 0; r < 4; r++) Ph[(4 * r + q4) * RS + ang] = (Real)acc[r];
+            }
           }
         }
       }
@@ -971,9 +1034,14 @@ namespace {
 int seq_row_stride(int nang) { return ((nang + 3) & ~3) + 1; }
 size_t seq_lds_bytes(int nang, int cmom, bool fp32, bool exact, int ngw) {
   const int rs = seq_row_stride(nang);
-  const size_t tables = (size_t)((nang + 15) / 16) * ((cmom + 3) / 4) * 64 + (size_t)((nang + 3) / 4) * 64;
+  // (FP64 pairs keep the operand tables in registers)
+  const size_t tables = (ngw == 2 && !fp32)
+                            ? 0
+                            : (size_t)((nang + 15) / 16) * ((cmom + 3) / 4) * 64 +
+                                  (size_t)((nang + 3) / 4) * 64;
   const int rsz = fp32 ? 4 : 8;
-  return sizeof(double) * tables + (size_t)kWGW * ngw * seq_wave_bytes(rs, rsz, exact ? 8 : rsz);
+  return sizeof(double) * tables +
+         (size_t)kWGW * ngw * seq_wave_bytes(rs, rsz, exact ? 8 : rsz, ngw == 1);
 }
 }  // namespace
 
