@@ -55,8 +55,8 @@ constexpr int kSX = 4, kSY = 4, kNP = kSX * kSY;  // tile cross-section, cells p
 #define SEQ_WAVES 2  // waves per SIMD the register allocation aims for
 #endif
 // (FP32 sweeps of two groups per wavefront aim for 3: at P3 the batch form
-// then spills 5 registers and still runs the 48-problem mixed batch 2.5%
-// faster than at 2 waves per SIMD)
+// then spills a few registers and still runs the 48-problem mixed batch
+// 2.5% faster than at 2 waves per SIMD)
 // Wavefronts per workgroup: each takes its own tile of the same sweep, so
 // they share the sweep's matrix-core operand tables in LDS.  Two, not four:
 // a workgroup's LDS is freed only when its slowest tile (the one with the
